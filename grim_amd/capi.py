@@ -1,0 +1,263 @@
+"""ctypes binding of the C ABI in ``include/grim_hip.h``.
+
+The library ``grim_amd/libgrim_hip.so`` is the only implementation: when it
+is missing, or no gfx950 device is visible, calls raise ``GrimError``; there
+is no CPU fallback.  Device buffers are torch tensors (``torch.float64`` on
+``cuda``), passed to the library by ``data_ptr()``; torch is used for
+allocation and streams only.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgrim_hip.so")
+
+GRIM_OK = 0
+GRIM_ERR_HIP = -1
+GRIM_ERR_BAD_ARG = -2
+GRIM_ERR_NO_DEVICE = -3
+GRIM_ERR_OOM = -4
+
+CLOSURE_LINEAR_MODES = 0
+CLOSURE_PER_ZONE = 1
+CLOSURE_TORUS = 2
+JAC_FD = 0
+JAC_ANALYTIC = 1
+LU_FP64 = 0
+LU_FP32 = 1
+NEWTON_PER_ZONE = 0
+NEWTON_GLOBAL = 1
+GEOM_PLANES = 16
+MAX_VARS = 10
+
+
+class GrimError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what} failed with status {code}: {last_error()}")
+        self.code = code
+
+
+class Params(C.Structure):
+    """grim_params: grim's ``namespace params`` subset (src/params.hpp:96-210)."""
+
+    _fields_ = [
+        ("conduction", C.c_int),
+        ("viscosity", C.c_int),
+        ("highOrderTermsConduction", C.c_int),
+        ("highOrderTermsViscosity", C.c_int),
+        ("adiabaticIndex", C.c_double),
+        ("rhoFloorInFluidElement", C.c_double),
+        ("uFloorInFluidElement", C.c_double),
+        ("bSqrFloorInFluidElement", C.c_double),
+        ("temperatureFloorInFluidElement", C.c_double),
+        ("ConductionAlpha", C.c_double),
+        ("ViscosityAlpha", C.c_double),
+        ("closure", C.c_int),
+        ("tau0", C.c_double),
+        ("ConductionClosureFactor", C.c_double),
+        ("ViscosityClosureFactor", C.c_double),
+        ("maxNonLinearIter", C.c_int),
+        ("maxLineSearchIters", C.c_int),
+        ("nonlinearsolve_atol", C.c_double),
+        ("JacobianAssembleEpsilon", C.c_double),
+        ("linesearchfloor", C.c_double),
+    ]
+
+    @property
+    def nf(self) -> int:
+        return 5 + self.conduction + self.viscosity
+
+    @property
+    def dof(self) -> int:
+        return 8 + self.conduction + self.viscosity
+
+
+class ZoneBatch(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("prim", C.c_void_p),
+        ("geom", C.c_void_p),
+        ("rhs", C.c_void_p),
+        ("prim_old", C.c_void_p),
+        ("prim_coef", C.c_void_p),
+        ("closure", C.c_void_p),
+        ("radius", C.c_void_p),
+        ("residual_mask", C.c_void_p),
+        ("dt_prime", C.c_double),
+    ]
+
+
+class NewtonOpts(C.Structure):
+    _fields_ = [
+        ("jacobian", C.c_int),
+        ("lu_precision", C.c_int),
+        ("semantics", C.c_int),
+        ("compaction", C.c_int),
+        ("block_size", C.c_int),
+    ]
+
+
+class NewtonStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_void_p),
+        ("residual_evals", C.c_void_p),
+        ("final_l2", C.c_void_p),
+    ]
+
+
+# Every symbol include/grim_hip.h declares, with its ctypes signature.
+_vp, _i64, _d, _i = C.c_void_p, C.c_int64, C.c_double, C.c_int
+_P = C.POINTER(Params)
+_B = C.POINTER(ZoneBatch)
+SIGNATURES = {
+    "grim_default_params": (None, [_P, _i]),
+    "grim_last_error": (C.c_char_p, []),
+    "grim_device_info": (_i, [C.c_char_p, _i]),
+    "grim_conserved": (_i, [_P, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "grim_residual": (_i, [_P, _B, _vp, _vp]),
+    "grim_jacobian": (_i, [_P, _B, _i, _vp, _vp]),
+    "grim_batch_linear_solve": (_i, [_i64, _i, _vp, _vp, _vp, _i, _vp]),
+    "grim_batched_newton": (_i, [_P, _B, C.POINTER(NewtonOpts), C.POINTER(NewtonStats), _vp]),
+    "grim_count_above": (_i, [_i64, _vp, _d, C.POINTER(C.c_int64), _vp]),
+    "grim_generate_kerr_schild_batch": (_i, [_P, _i64, C.c_uint64, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "grim_known_root_rhs": (_i, [_P, _B, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GrimError(GRIM_ERR_NO_DEVICE, f"loading {LIB_PATH} (not built: run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    try:
+        msg = lib().grim_last_error()
+    except Exception:  # noqa: BLE001 - only used while formatting an error
+        return "(library not loaded)"
+    return msg.decode(errors="replace") if msg else ""
+
+
+def check(rc: int, what: str) -> None:
+    if rc != GRIM_OK:
+        raise GrimError(rc, what)
+
+
+def default_params(emhd: bool = False) -> Params:
+    p = Params()
+    lib().grim_default_params(C.byref(p), 1 if emhd else 0)
+    return p
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+
+        return torch.cuda.current_stream().cuda_stream
+    return getattr(stream, "cuda_stream", stream)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    import torch
+
+    assert isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous(), "device tensors only"
+    return t.data_ptr()
+
+
+def device_info() -> str:
+    buf = C.create_string_buffer(64)
+    check(lib().grim_device_info(buf, 64), "grim_device_info")
+    return buf.value.decode()
+
+
+def make_batch(prim, geom, rhs, prim_old=None, prim_coef=None, closure=None, radius=None,
+               residual_mask=None, dt_prime=1.0) -> ZoneBatch:
+    n = prim.shape[-1]
+    return ZoneBatch(n, _ptr(prim), _ptr(geom), _ptr(rhs), _ptr(prim_old), _ptr(prim_coef),
+                     _ptr(closure), _ptr(radius), _ptr(residual_mask), float(dt_prime))
+
+
+def conserved(p: Params, prim, geom, closure=None, radius=None, out=None, stream=None):
+    import torch
+
+    n = prim.shape[-1]
+    if out is None:
+        out = torch.empty((p.dof, n), dtype=torch.float64, device=prim.device)
+    check(lib().grim_conserved(C.byref(p), n, _ptr(prim), _ptr(geom), _ptr(closure), _ptr(radius),
+                               _ptr(out), _stream(stream)), "grim_conserved")
+    return out
+
+
+def residual(p: Params, batch: ZoneBatch, out, stream=None):
+    check(lib().grim_residual(C.byref(p), C.byref(batch), _ptr(out), _stream(stream)), "grim_residual")
+    return out
+
+
+def jacobian(p: Params, batch: ZoneBatch, kind: int, out, stream=None):
+    check(lib().grim_jacobian(C.byref(p), C.byref(batch), kind, _ptr(out), _stream(stream)), "grim_jacobian")
+    return out
+
+
+def batch_linear_solve(A, b, x, lu_precision=LU_FP64, stream=None):
+    nvar = b.shape[0]
+    n = b.shape[-1]
+    check(lib().grim_batch_linear_solve(n, nvar, _ptr(A), _ptr(b), _ptr(x), lu_precision, _stream(stream)),
+          "grim_batch_linear_solve")
+    return x
+
+
+def batched_newton(p: Params, batch: ZoneBatch, jacobian=JAC_FD, lu_precision=LU_FP64,
+                   semantics=NEWTON_PER_ZONE, compaction=0, block_size=0, iterations=None,
+                   residual_evals=None, final_l2=None, stream=None):
+    opts = NewtonOpts(jacobian, lu_precision, semantics, compaction, block_size)
+    st = NewtonStats(_ptr(iterations), _ptr(residual_evals), _ptr(final_l2))
+    check(lib().grim_batched_newton(C.byref(p), C.byref(batch), C.byref(opts), C.byref(st), _stream(stream)),
+          "grim_batched_newton")
+
+
+def count_above(final_l2, tol, stream=None) -> int:
+    c = C.c_int64(0)
+    check(lib().grim_count_above(final_l2.shape[-1], _ptr(final_l2), float(tol), C.byref(c), _stream(stream)),
+          "grim_count_above")
+    return int(c.value)
+
+
+def generate_kerr_schild_batch(p: Params, n: int, seed: int, a: float = 0.9375, h: float = 0.3,
+                               device="cuda", stream=None):
+    """Synthetic isolated-zone batch; returns dict of device tensors."""
+    import torch
+
+    f = dict(dtype=torch.float64, device=device)
+    out = {
+        "geom": torch.empty((GEOM_PLANES, n), **f),
+        "prim_star": torch.empty((p.dof, n), **f),
+        "guess": torch.empty((p.dof, n), **f),
+        "prim_old": torch.empty((p.dof, n), **f) if (p.conduction or p.viscosity) else None,
+        "closure": torch.empty((2, n), **f) if (p.conduction or p.viscosity) else None,
+    }
+    check(lib().grim_generate_kerr_schild_batch(C.byref(p), n, seed, a, h, _ptr(out["geom"]),
+                                                 _ptr(out["prim_star"]), _ptr(out["guess"]),
+                                                 _ptr(out["prim_old"]), _ptr(out["closure"]),
+                                                 _stream(stream)), "grim_generate_kerr_schild_batch")
+    return out
+
+
+def known_root_rhs(p: Params, batch: ZoneBatch, rhs, mask=None, stream=None):
+    check(lib().grim_known_root_rhs(C.byref(p), C.byref(batch), _ptr(rhs), _ptr(mask), _stream(stream)),
+          "grim_known_root_rhs")
+    return rhs
