@@ -41,25 +41,46 @@ def parse():
     return ap.parse_args()
 
 
-def flops_per_zone_solve(iters, res_evals, nf=5):
-    """Algorithmic FP64 operations of the FD-Jacobian solve from per-zone counts.
+# Operation tallies per zone (add/sub/mul/div/sqrt = 1 each), counted from
+# grim_amd/csrc/zone_physics.hpp, analytic.hpp and newton.hip:
+#   exact residual (grim order, ideal MHD)          151 FP64
+#   fast residual (reciprocal form, mixed path)     147 FP64
+#   per test / line-search evaluation bookkeeping    24 FP64 (trial, 0.5 sum R^2, Armijo)
+#   FD quotients per Jacobian                        65 FP64 (5 x (3 + 2 x 5))
+#   LU 5x5 with partial pivoting                    120 (FP64 or FP32)
+#   analytic Jacobian (primal 132 + rho/u columns 23 + 3 x 106)
+#                                                   473 (FP64 or FP32)
+#   row/column equilibration around the FP32 LU      65 FP32
+#   update bookkeeping (f0, f'0, P += lambda d)      12 FP64
+OPS = {
+    "res_exact": 151, "res_fast": 147, "eval_bookkeeping": 24, "fd_quotients": 65,
+    "lu": 120, "analytic_jac": 473, "equilibration": 65, "update": 12,
+}
 
-    Tallied by hand from grim_amd/csrc/zone_physics.hpp and newton.hip
-    (add/sub/mul/div/sqrt = 1 each, dead code excluded):
-      residual evaluation (ideal MHD)   151  (kinematics 110, t-row + U 26,
-                                              (U - rhs)/dt 10, mask 5)
-      FD quotients per Jacobian          65  (5 columns x (3 + 2 x 5))
-      LU solve 5x5, partial pivoting    120  (elimination 95, substitution 25)
-      update bookkeeping                 12  (f0, f'0, P += lambda d)
-      per line-search / test evaluation  24  (trial, 0.5 sum R^2, Armijo test)
-    """
-    return (res_evals * RESIDUAL_OPS[nf] + (res_evals - nf * iters) * 24
-            + iters * (FD_QUOTIENT_OPS[nf] + LU_OPS[nf] + 12))
+
+def path_flops(path, iters, res_evals, nf=5):
+    """(FP64 ops, FP32 ops) of one solve of the batch from per-zone counts."""
+    if path == "fd_fp64":
+        ls = res_evals - nf * iters  # tests + line-search evaluations
+        f64 = (res_evals * OPS["res_exact"] + ls * OPS["eval_bookkeeping"]
+               + iters * (OPS["fd_quotients"] + OPS["lu"] + OPS["update"]))
+        return f64, 0.0
+    if path == "analytic_fp64":
+        f64 = (res_evals * (OPS["res_exact"] + OPS["eval_bookkeeping"])
+               + iters * (OPS["analytic_jac"] + OPS["lu"] + OPS["update"]))
+        return f64, 0.0
+    f64 = res_evals * (OPS["res_fast"] + OPS["eval_bookkeeping"]) + iters * OPS["update"]
+    f32 = iters * (OPS["analytic_jac"] + OPS["equilibration"] + OPS["lu"])
+    return f64, f32
 
 
-RESIDUAL_OPS = {5: 151}
-FD_QUOTIENT_OPS = {5: 65}
-LU_OPS = {5: 120}
+PATHS = {
+    "fd_fp64": ("JAC_FD", "LU_FP64", "newtonPerZone<Layout<false,false>, MODE_FD>"),
+    "analytic_fp64": ("JAC_ANALYTIC", "LU_FP64", "newtonPerZone<Layout<false,false>, MODE_ANALYTIC64>"),
+    "mixed": ("JAC_ANALYTIC", "LU_FP32", "newtonPerZone<Layout<false,false>, MODE_MIXED>"),
+}
+HEADLINE = "mixed"
+FP32_PEAK_TFLOPS = 157.3
 
 
 def main():
@@ -91,86 +112,106 @@ def main():
     batch = capi.make_batch(prim, b["geom"], rhs, residual_mask=mask)
     stream = torch.cuda.current_stream()
     torch.cuda.synchronize()
+    results = {}
+    finals = {}
+    for path, (jac, lu, kname) in PATHS.items():
+        jac, lu = getattr(capi, jac), getattr(capi, lu)
 
-    def step(ev_mid=None):
-        prim[: p.nf].copy_(b["guess"][: p.nf])
-        if ev_mid is not None:
-            ev_mid.record(stream)
-        capi.batched_newton(p, batch, capi.JAC_FD, capi.LU_FP64, capi.NEWTON_PER_ZONE,
-                            iterations=it, residual_evals=ev, final_l2=l2, stream=stream)
+        def step(ev_mid=None):
+            prim[: p.nf].copy_(b["guess"][: p.nf])
+            if ev_mid is not None:
+                ev_mid.record(stream)
+            capi.batched_newton(p, batch, jac, lu, capi.NEWTON_PER_ZONE,
+                                iterations=it, residual_evals=ev, final_l2=l2, stream=stream)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    print("# timing", file=sys.stderr, flush=True)
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
-            torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for e0, e1, e2 in evs:
-        e0.record(stream)
-        step(e1)
-        e2.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    step_ms = [e0.elapsed_time(e2) for e0, _, e2 in evs]
-    kern_ms = [e1.elapsed_time(e2) for _, e1, e2 in evs]
-    ms_per_step = wall * 1e3 / args.steps
-    value = n / (ms_per_step * 1e-3)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        print(f"# timing {path}", file=sys.stderr, flush=True)
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+                torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for e0, e1, e2 in evs:
+            e0.record(stream)
+            step(e1)
+            e2.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        step_ms = [e0.elapsed_time(e2) for e0, _, e2 in evs]
+        kern_ms = [e1.elapsed_time(e2) for _, e1, e2 in evs]
+        ms_per_step = wall * 1e3 / args.steps
+        iters = float(it.to(torch.int64).sum())
+        evals = float(ev.to(torch.int64).sum())
+        f64, f32 = path_flops(path, iters, evals)
+        kern_s = float(np.mean(kern_ms)) * 1e-3
+        t_peak = f64 / (FP64_PEAK_TFLOPS * 1e12) + f32 / (FP32_PEAK_TFLOPS * 1e12)
+        rel = ((prim[: p.nf] - b["prim_star"][: p.nf]).abs() / b["prim_star"][: p.nf].abs()).amax(dim=0)
+        finals[path] = prim[: p.nf].clone()
+        results[path] = {
+            "zone_updates_per_s": n / (ms_per_step * 1e-3),
+            "ms_per_step": ms_per_step,
+            "kernel": kname,
+            "kernel_ms": float(np.mean(kern_ms)),
+            "kernel_ms_min": float(min(kern_ms)),
+            "step_ms_min": float(min(step_ms)),
+            "unconverged": capi.count_above(l2, p.nonlinearsolve_atol),
+            "iterations_hist": torch.bincount(it.to(torch.int64)).tolist(),
+            "mean_residual_evals": evals / n,
+            "fp64_ops": f64,
+            "fp32_ops": f32,
+            "achieved_tflops": (f64 + f32) / kern_s / 1e12,
+            "valu_frac": t_peak / kern_s,
+            "err_vs_root_p99": float(torch.quantile(rel.float(), 0.99)),
+            "err_vs_root_max": float(rel.max()),
+        }
+    for path in ("analytic_fp64", "mixed"):
+        d = ((finals[path] - finals["fd_fp64"]).abs() / finals["fd_fp64"].abs()).amax(dim=0)
+        results[path]["err_vs_fd_fp64_p99"] = float(torch.quantile(d.float(), 0.99))
+        results[path]["err_vs_fd_fp64_max"] = float(d.max())
+    d = ((finals["mixed"] - finals["analytic_fp64"]).abs() / finals["analytic_fp64"].abs()).amax(dim=0)
+    results["mixed"]["err_vs_analytic_fp64_p99"] = float(torch.quantile(d.float(), 0.99))
+    results["mixed"]["err_vs_analytic_fp64_max"] = float(d.max())
 
-    unconverged = capi.count_above(l2, p.nonlinearsolve_atol)
-    iters = it.to(torch.int64)
-    evals = ev.to(torch.int64)
-    tot_flops = flops_per_zone_solve(float(iters.sum()), float(evals.sum()))
-    kern_s = float(np.mean(kern_ms)) * 1e-3
-    achieved_tflops = tot_flops / kern_s / 1e12
-    hist = torch.bincount(it.to(torch.int64)).tolist()
-    rel = ((prim[: p.nf] - b["prim_star"][: p.nf]).abs() / b["prim_star"][: p.nf].abs()).amax(dim=0)
-    # algorithmic HBM bytes of one solve: read 8 prim + 16 geom + 5 rhs + mask,
-    # write 5 prim + iterations/evals (int32) + final l2
+    head = results[HEADLINE]
+    fd = results["fd_fp64"]
+    kern_s = head["kernel_ms"] * 1e-3
     bytes_per_zone = (8 + 16 + 5 + 1) * 8 + 5 * 8 + 4 + 4 + 8
-
+    mixed_peak = (head["fp64_ops"] + head["fp32_ops"]) / (
+        head["fp64_ops"] / FP64_PEAK_TFLOPS + head["fp32_ops"] / FP32_PEAK_TFLOPS)
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["zone_updates_per_s"],
         "unit": "zone-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64 residual/update + f32 Jacobian/LU",
         "data": "synthetic (seeded Kerr-Schild zone batch, known roots)",
         "config": {
             "workload": "isolated batched per-zone Newton, ideal MHD, 2^22 zones, MKS a=0.9375, "
-                        "FD Jacobian + FP64 LU, resid<=1e-10 (per-zone normalised)",
-            "zones": n, "newton_unknowns": p.nf, "primitives": p.dof, "path": "fd_fp64",
-            "semantics": "per_zone",
+                        "resid<=1e-10 (per-zone normalised); headline = analytic Jacobian + FP32 LU",
+            "zones": n, "newton_unknowns": p.nf, "primitives": p.dof, "semantics": "per_zone",
         },
+        "speedup_vs_fd_fp64": head["zone_updates_per_s"] / fd["zone_updates_per_s"],
         "roofline": {
             "bound": "valu",
-            "pipe": "VALU FP64 (no MFMA: 5x5 per-zone systems)",
-            "achieved": achieved_tflops,
-            "peak": FP64_PEAK_TFLOPS,
+            "pipe": "VALU FP64 + FP32 (no MFMA: 5x5 per-zone systems)",
+            "achieved": head["achieved_tflops"],
+            "peak": mixed_peak,
             "unit": "TFLOP/s",
-            "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+            "frac": head["valu_frac"],
             "traffic": None,
-            "kernel": "newtonPerZoneFD<Layout<false,false>>",
-            "kernel_ms": float(np.mean(kern_ms)),
-            "flops_per_launch": tot_flops,
+            "kernel": head["kernel"],
+            "kernel_ms": head["kernel_ms"],
             "hbm_gbs_algorithmic": bytes_per_zone * n / kern_s / 1e9,
             "hbm_frac": bytes_per_zone * n / kern_s / 1e9 / HBM_PEAK_GBS,
         },
-        "newton": {
-            "unconverged": unconverged,
-            "iterations_hist": hist,
-            "mean_residual_evals": float(evals.float().mean()),
-            "max_rel_err_vs_root_median": float(rel.median()),
-            "step_ms_min": float(min(step_ms)),
-            "kernel_ms_min": float(min(kern_ms)),
-        },
+        "paths": results,
     }
 
     if not args.no_cpu:

@@ -13,6 +13,7 @@
 // wavefront access.  FP64 arithmetic throughout this file (the reference path).
 #include <hip/hip_runtime.h>
 
+#include "analytic.hpp"
 #include "grim_hip.h"
 #include "runtime.hpp"
 #include "zone_physics.hpp"
@@ -20,6 +21,29 @@
 namespace grim {
 
 constexpr int kDefaultBlock = 256;
+
+// Jacobian / precision modes of the per-zone Newton kernel.
+//  MODE_FD:          grim's residual and FD Jacobian, FP64 LU (reference path)
+//  MODE_ANALYTIC64:  grim's residual, analytic Jacobian and LU in FP64
+//  MODE_MIXED:       FP64 residual (reciprocal form) and update; analytic
+//                    Jacobian and equilibrated LU in FP32
+enum { MODE_FD = 0, MODE_ANALYTIC64 = 1, MODE_MIXED = 2 };
+
+// Per-zone quantities of the residual formed once per zone.
+struct ZoneScale {
+  double invAlpha;  // 1/alpha
+  double scale;     // mask/dt
+};
+
+template <class L, int MODE>
+__device__ __forceinline__ void evalResidual(const double* P, const ZoneGeom& G,
+                                             const ZoneConst<L>& c, const DevParams& prm,
+                                             double dt, const ZoneScale& zs, double* R) {
+  if constexpr (MODE == MODE_MIXED)
+    residualFastIdeal<L>(P, G, c, prm, zs.invAlpha, zs.scale, R);
+  else
+    residual<L>(P, G, c, prm, dt, R);
+}
 
 // grim's forward-difference Jacobian (solve.cpp:82-109): A[i][j] = dR_i/dP_j
 // with P_j perturbed by eps*P_j, or by eps when |P_j| < eps/2.
@@ -49,18 +73,34 @@ __device__ __forceinline__ void fdJacobian(const double* P, const double* R, con
 // Returns true when the last evaluated trial was accepted; Rt then holds the
 // residual at the new P (bit-identical to a fresh evaluation, since the new P
 // is computed by the same expression as the trial).
-template <class L>
+template <class L, int MODE>
 __device__ __forceinline__ bool newtonUpdate(double* P, const double* R, double l2,
                                              const ZoneGeom& G, const ZoneConst<L>& c,
-                                             const DevParams& prm, double dt, double* Rt,
-                                             int& evals) {
-  double A[L::NF][L::NF];
-  fdJacobian<L>(P, R, G, c, prm, dt, A);
-  evals += L::NF;
+                                             const DevParams& prm, double dt, const ZoneScale& zs,
+                                             double* Rt, int& evals) {
   double d[L::NF];
+  if constexpr (MODE == MODE_FD) {
+    double A[L::NF][L::NF];
+    fdJacobian<L>(P, R, G, c, prm, dt, A);
+    evals += L::NF;
 #pragma unroll
-  for (int v = 0; v < L::NF; v++) d[v] = -R[v];
-  luSolve<double, L::NF>(A, d);
+    for (int v = 0; v < L::NF; v++) d[v] = -R[v];
+    luSolve<double, L::NF>(A, d);
+  } else if constexpr (MODE == MODE_ANALYTIC64) {
+    double A[L::NF][L::NF];
+    analyticJacobianIdeal<double, L>(P, G, prm, c.mask / dt, A);
+#pragma unroll
+    for (int v = 0; v < L::NF; v++) d[v] = -R[v];
+    luSolve<double, L::NF>(A, d);
+  } else {
+    float A[L::NF][L::NF], df[L::NF];
+    analyticJacobianIdeal<float, L>(P, G, prm, zs.scale, A);
+#pragma unroll
+    for (int v = 0; v < L::NF; v++) df[v] = (float)(-R[v]);
+    equilibratedSolve<float, L::NF>(A, df, P);
+#pragma unroll
+    for (int v = 0; v < L::NF; v++) d[v] = (double)df[v];
+  }
 
   const double f0 = 0.5 * l2;
   const double fPrime0 = -2. * f0;
@@ -72,7 +112,7 @@ __device__ __forceinline__ bool newtonUpdate(double* P, const double* R, double 
   for (int ls = 0; ls < prm.maxLS; ls++) {
 #pragma unroll
     for (int v = 0; v < L::NF; v++) Pt[v] = P[v] + lambda * d[v];
-    residual<L>(Pt, G, c, prm, dt, Rt);
+    evalResidual<L, MODE>(Pt, G, c, prm, dt, zs, Rt);
     evals += 1;
     const double f1 = 0.5 * sumSq<L::NF>(Rt);
     const bool condition = f1 > f0 * (1. - 1e-4 * lambda) + prm.lsFloor;
@@ -90,10 +130,9 @@ __device__ __forceinline__ bool newtonUpdate(double* P, const double* R, double 
 
 // Per-zone semantics: each thread iterates its own zone until
 // sum R^2 <= atol or maxNonLinearIter updates.
-template <class L>
-__global__ __launch_bounds__(kDefaultBlock) void newtonPerZoneFD(DevParams prm,
-                                                                 grim_zone_batch b,
-                                                                 grim_newton_stats st) {
+template <class L, int MODE>
+__global__ __launch_bounds__(kDefaultBlock) void newtonPerZone(DevParams prm, grim_zone_batch b,
+                                                               grim_newton_stats st) {
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= b.n) return;
   const int64_t n = b.n;
@@ -102,6 +141,11 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonPerZoneFD(DevParams prm,
   loadGeom(b.geom, n, z, G);
   ZoneConst<L> c;
   loadZoneConst<L>(b, prm, G, z, c);
+  ZoneScale zs;
+  if constexpr (MODE == MODE_MIXED) {
+    zs.invAlpha = 1. / G.alpha;
+    zs.scale = c.mask / dt;
+  }
   double P[L::DOF];
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) P[v] = b.prim[v * n + z];
@@ -119,7 +163,7 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonPerZoneFD(DevParams prm,
     P[RHO] = p0;
     P[UU] = p1;
     if (!haveR) {
-      residual<L>(P, G, c, prm, dt, R);
+      evalResidual<L, MODE>(P, G, c, prm, dt, zs, R);
       evals++;
     }
     l2 = sumSq<L::NF>(R);
@@ -128,7 +172,7 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonPerZoneFD(DevParams prm,
       break;
     }
     double Rt[L::NF];
-    haveR = newtonUpdate<L>(P, R, l2, G, c, prm, dt, Rt, evals);
+    haveR = newtonUpdate<L, MODE>(P, R, l2, G, c, prm, dt, zs, Rt, evals);
     iters++;
     if (haveR) {
 #pragma unroll
@@ -136,7 +180,7 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonPerZoneFD(DevParams prm,
     }
   }
   if (!converged) {
-    if (!haveR) residual<L>(P, G, c, prm, dt, R);
+    if (!haveR) evalResidual<L, MODE>(P, G, c, prm, dt, zs, R);
     l2 = sumSq<L::NF>(R);
   }
 #pragma unroll
@@ -206,7 +250,8 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonGlobalUpdate(DevParams pr
 #pragma unroll
   for (int v = 0; v < L::NF; v++) R[v] = Rbuf[v * n + z];
   int evals = 0;
-  newtonUpdate<L>(P, R, l2buf[z], G, c, prm, b.dt_prime, Rt, evals);
+  ZoneScale zs{};
+  newtonUpdate<L, MODE_FD>(P, R, l2buf[z], G, c, prm, b.dt_prime, zs, Rt, evals);
 #pragma unroll
   for (int v = 0; v < L::NF; v++) b.prim[v * n + z] = P[v];
   evalsBuf[z] += evals;
@@ -286,6 +331,25 @@ __global__ __launch_bounds__(kDefaultBlock) void fdJacobianKernel(DevParams prm,
   for (int v = 0; v < L::DOF; v++) P[v] = b.prim[v * n + z];
   residual<L>(P, G, c, prm, b.dt_prime, R);
   fdJacobian<L>(P, R, G, c, prm, b.dt_prime, A);
+#pragma unroll
+  for (int i = 0; i < L::NF; i++)
+#pragma unroll
+    for (int j = 0; j < L::NF; j++) J[(i * L::NF + j) * n + z] = A[i][j];
+}
+
+template <class L>
+__global__ __launch_bounds__(kDefaultBlock) void analyticJacobianKernel(DevParams prm,
+                                                                       grim_zone_batch b, double* J) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= b.n) return;
+  const int64_t n = b.n;
+  ZoneGeom G;
+  loadGeom(b.geom, n, z, G);
+  const double mask = b.residual_mask ? b.residual_mask[z] : 1.;
+  double P[L::DOF], A[L::NF][L::NF];
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) P[v] = b.prim[v * n + z];
+  analyticJacobianIdeal<double, L>(P, G, prm, mask / b.dt_prime, A);
 #pragma unroll
   for (int i = 0; i < L::NF; i++)
 #pragma unroll
@@ -380,14 +444,22 @@ int grim_residual(const grim_params* p, const grim_zone_batch* b, double* R,
 int grim_jacobian(const grim_params* p, const grim_zone_batch* b, int jacobian, double* J,
                   grim_stream_t stream) {
   GRIM_REQUIRE(validBatch(p, b) && (b->n == 0 || J), "grim_jacobian: bad arguments");
-  GRIM_REQUIRE(jacobian == GRIM_JAC_FD, "grim_jacobian: only GRIM_JAC_FD is built");
+  GRIM_REQUIRE(jacobian == GRIM_JAC_FD || jacobian == GRIM_JAC_ANALYTIC,
+               "grim_jacobian: bad Jacobian kind");
+  GRIM_REQUIRE(jacobian == GRIM_JAC_FD || !(p->conduction || p->viscosity),
+               "grim_jacobian: the analytic Jacobian is built for ideal MHD only");
   if (b->n == 0) return GRIM_OK;
   if (int rc = checkDevice()) return rc;
   hipStream_t s = (hipStream_t)stream;
   const DevParams prm = makeDevParams(*p);
   const grim_zone_batch bb = *b;
   return dispatchLayout(*p, [&]<class L>() -> int {
-    fdJacobianKernel<L><<<gridFor(bb.n, kDefaultBlock), kDefaultBlock, 0, s>>>(prm, bb, J);
+    if (jacobian == GRIM_JAC_FD) {
+      fdJacobianKernel<L><<<gridFor(bb.n, kDefaultBlock), kDefaultBlock, 0, s>>>(prm, bb, J);
+    } else {
+      if constexpr (!L::EMHD)
+        analyticJacobianKernel<L><<<gridFor(bb.n, kDefaultBlock), kDefaultBlock, 0, s>>>(prm, bb, J);
+    }
     return checkLaunch("grim_jacobian");
   });
 }
@@ -433,16 +505,32 @@ int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
   if (stats) st = *stats;
   GRIM_REQUIRE(o.semantics == GRIM_NEWTON_PER_ZONE || o.semantics == GRIM_NEWTON_GLOBAL,
                "grim_batched_newton: bad semantics");
-  GRIM_REQUIRE(o.jacobian == GRIM_JAC_FD && o.lu_precision == GRIM_LU_FP64 && !o.compaction,
-               "grim_batched_newton: only the FD Jacobian with FP64 LU is built");
+  GRIM_REQUIRE(!o.compaction, "grim_batched_newton: compaction is not built");
+  int mode = -1;
+  if (o.jacobian == GRIM_JAC_FD && o.lu_precision == GRIM_LU_FP64) mode = MODE_FD;
+  if (o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP64) mode = MODE_ANALYTIC64;
+  if (o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP32) mode = MODE_MIXED;
+  GRIM_REQUIRE(mode >= 0, "grim_batched_newton: unsupported Jacobian / LU precision pair");
+  GRIM_REQUIRE(mode == MODE_FD || !(p->conduction || p->viscosity),
+               "grim_batched_newton: the analytic Jacobian is built for ideal MHD only");
+  GRIM_REQUIRE(mode == MODE_FD || o.semantics == GRIM_NEWTON_PER_ZONE,
+               "grim_batched_newton: lock-step semantics run grim's FD-FP64 algorithm only");
   if (b->n == 0) return GRIM_OK;
   if (int rc = checkDevice()) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (o.semantics == GRIM_NEWTON_GLOBAL) return launchGlobal(*p, *b, st, s);
   const DevParams prm = makeDevParams(*p);
   const grim_zone_batch bb = *b;
+  const unsigned grid = gridFor(bb.n, kDefaultBlock);
   return dispatchLayout(*p, [&]<class L>() -> int {
-    newtonPerZoneFD<L><<<gridFor(bb.n, kDefaultBlock), kDefaultBlock, 0, s>>>(prm, bb, st);
+    if (mode == MODE_FD) {
+      newtonPerZone<L, MODE_FD><<<grid, kDefaultBlock, 0, s>>>(prm, bb, st);
+    } else if constexpr (!L::EMHD) {
+      if (mode == MODE_ANALYTIC64)
+        newtonPerZone<L, MODE_ANALYTIC64><<<grid, kDefaultBlock, 0, s>>>(prm, bb, st);
+      else
+        newtonPerZone<L, MODE_MIXED><<<grid, kDefaultBlock, 0, s>>>(prm, bb, st);
+    }
     return checkLaunch("grim_batched_newton");
   });
 }
