@@ -38,32 +38,40 @@ def parse():
     ap.add_argument("--zones", type=int, default=N_ZONES)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--paths", default=",".join(["fd_fp64", "fd_fp64_fused", "analytic_fp64", "mixed"]),
+                    help="comma-separated subset of the Newton paths to time (profiling)")
     return ap.parse_args()
 
 
-# Operation tallies per zone (add/sub/mul/div/sqrt = 1 each), counted from
-# grim_amd/csrc/zone_physics.hpp, analytic.hpp and newton.hip:
+# Operation tallies per zone (add/sub/mul/div/sqrt/max = 1 each; a fused
+# multiply-add counts as its two operations), counted from
+# grim_amd/csrc/zone_physics.hpp, analytic.hpp and newton_kernels.hpp:
 #   exact residual (grim order, ideal MHD)          151 FP64
-#   fast residual (reciprocal form, mixed path)     147 FP64
+#   fast residual (identity form, fused paths)      115 FP64
 #   per test / line-search evaluation bookkeeping    24 FP64 (trial, 0.5 sum R^2, Armijo)
-#   FD quotients per Jacobian                        65 FP64 (5 x (3 + 2 x 5))
+#   FD quotients per Jacobian, grim order            65 FP64 (5 x (3 + 2 x 5))
+#   FD quotients per Jacobian, reciprocal form       75 FP64 (5 x (5 + 2 x 5))
 #   LU 5x5 with partial pivoting                    120 (FP64 or FP32)
-#   analytic Jacobian (primal 132 + rho/u columns 23 + 3 x 106)
-#                                                   473 (FP64 or FP32)
-#   row/column equilibration around the FP32 LU      65 FP32
+#   analytic Jacobian (primal 85, rho/u columns 21, velocity columns 5 + 3 x 57)
+#                                                   282 (FP64 or FP32)
+#   row/column equilibration around the FP32 LU      90 FP32
 #   update bookkeeping (f0, f'0, P += lambda d)      12 FP64
 OPS = {
-    "res_exact": 151, "res_fast": 147, "eval_bookkeeping": 24, "fd_quotients": 65,
-    "lu": 120, "analytic_jac": 473, "equilibration": 65, "update": 12,
+    "res_exact": 151, "res_fast": 115, "eval_bookkeeping": 24, "fd_quotients": 65,
+    "fd_quotients_fused": 75, "lu": 120, "analytic_jac": 282, "equilibration": 90, "update": 12,
 }
 
 
 def path_flops(path, iters, res_evals, nf=5):
     """(FP64 ops, FP32 ops) of one solve of the batch from per-zone counts."""
+    ls = res_evals - nf * iters  # tests + line-search evaluations (FD paths)
     if path == "fd_fp64":
-        ls = res_evals - nf * iters  # tests + line-search evaluations
         f64 = (res_evals * OPS["res_exact"] + ls * OPS["eval_bookkeeping"]
                + iters * (OPS["fd_quotients"] + OPS["lu"] + OPS["update"]))
+        return f64, 0.0
+    if path == "fd_fp64_fused":
+        f64 = (res_evals * OPS["res_fast"] + ls * OPS["eval_bookkeeping"]
+               + iters * (OPS["fd_quotients_fused"] + OPS["lu"] + OPS["update"]))
         return f64, 0.0
     if path == "analytic_fp64":
         f64 = (res_evals * (OPS["res_exact"] + OPS["eval_bookkeeping"])
@@ -75,12 +83,15 @@ def path_flops(path, iters, res_evals, nf=5):
 
 
 PATHS = {
-    "fd_fp64": ("JAC_FD", "LU_FP64", "newtonPerZone<Layout<false,false>, MODE_FD>"),
-    "analytic_fp64": ("JAC_ANALYTIC", "LU_FP64", "newtonPerZone<Layout<false,false>, MODE_ANALYTIC64>"),
-    "mixed": ("JAC_ANALYTIC", "LU_FP32", "newtonPerZone<Layout<false,false>, MODE_MIXED>"),
+    "fd_fp64": ("JAC_FD", "LU_FP64", 0, "newtonPerZone<Layout<false,false>, MODE_FD>"),
+    "fd_fp64_fused": ("JAC_FD", "LU_FP64", 1, "newtonPerZone<Layout<false,false>, MODE_FD_FAST>"),
+    "analytic_fp64": ("JAC_ANALYTIC", "LU_FP64", 0, "newtonPerZone<Layout<false,false>, MODE_ANALYTIC64>"),
+    "mixed": ("JAC_ANALYTIC", "LU_FP32", 1, "newtonPerZone<Layout<false,false>, MODE_MIXED>"),
 }
 HEADLINE = "mixed"
-FP32_PEAK_TFLOPS = 157.3
+# FP32 VALU without packed math (this build disables packed FP32) issues one
+# FP32 FMA per lane per clock, the same rate as FP64: 78.6 TFLOP/s.
+FP32_PEAK_TFLOPS = 78.6
 
 
 def main():
@@ -114,7 +125,9 @@ def main():
     torch.cuda.synchronize()
     results = {}
     finals = {}
-    for path, (jac, lu, kname) in PATHS.items():
+    selected = [q for q in PATHS if q in args.paths.split(",")]
+    for path in selected:
+        jac, lu, fused, kname = PATHS[path]
         jac, lu = getattr(capi, jac), getattr(capi, lu)
 
         def step(ev_mid=None):
@@ -122,7 +135,8 @@ def main():
             if ev_mid is not None:
                 ev_mid.record(stream)
             capi.batched_newton(p, batch, jac, lu, capi.NEWTON_PER_ZONE,
-                                iterations=it, residual_evals=ev, final_l2=l2, stream=stream)
+                                iterations=it, residual_evals=ev, final_l2=l2, stream=stream,
+                                fused=fused)
 
         for _ in range(args.warmup):
             step()
@@ -165,16 +179,16 @@ def main():
             "err_vs_root_p99": float(torch.quantile(rel.float(), 0.99)),
             "err_vs_root_max": float(rel.max()),
         }
-    for path in ("analytic_fp64", "mixed"):
-        d = ((finals[path] - finals["fd_fp64"]).abs() / finals["fd_fp64"].abs()).amax(dim=0)
-        results[path]["err_vs_fd_fp64_p99"] = float(torch.quantile(d.float(), 0.99))
-        results[path]["err_vs_fd_fp64_max"] = float(d.max())
-    d = ((finals["mixed"] - finals["analytic_fp64"]).abs() / finals["analytic_fp64"].abs()).amax(dim=0)
-    results["mixed"]["err_vs_analytic_fp64_p99"] = float(torch.quantile(d.float(), 0.99))
-    results["mixed"]["err_vs_analytic_fp64_max"] = float(d.max())
+    for ref in ("fd_fp64", "analytic_fp64"):
+        for path in selected:
+            if path == ref or ref not in finals:
+                continue
+            d = ((finals[path] - finals[ref]).abs() / finals[ref].abs()).amax(dim=0)
+            results[path][f"err_vs_{ref}_p99"] = float(torch.quantile(d.float(), 0.99))
+            results[path][f"err_vs_{ref}_max"] = float(d.max())
 
-    head = results[HEADLINE]
-    fd = results["fd_fp64"]
+    head = results[HEADLINE] if HEADLINE in results else results[selected[-1]]
+    fd = results.get("fd_fp64", head)
     kern_s = head["kernel_ms"] * 1e-3
     bytes_per_zone = (8 + 16 + 5 + 1) * 8 + 5 * 8 + 4 + 4 + 8
     mixed_peak = (head["fp64_ops"] + head["fp32_ops"]) / (
@@ -198,6 +212,7 @@ def main():
             "zones": n, "newton_unknowns": p.nf, "primitives": p.dof, "semantics": "per_zone",
         },
         "speedup_vs_fd_fp64": head["zone_updates_per_s"] / fd["zone_updates_per_s"],
+        "speedup_vs_fd_fp64_fused": head["zone_updates_per_s"] / results.get("fd_fp64_fused", head)["zone_updates_per_s"],
         "roofline": {
             "bound": "valu",
             "pipe": "VALU FP64 + FP32 (no MFMA: 5x5 per-zone systems)",

@@ -5,7 +5,8 @@ into one shared library exporting the C ABI of ``include/grim_hip.h``.
 
 Flags that matter:
   -ffp-contract=off   no fused multiply-add contraction, so FP64 results are
-                      rounded operation by operation as written;
+                      rounded operation by operation as written (every file
+                      except *_fast.hip, which use -ffp-contract=fast);
   -Xclang -target-feature -Xclang -packed-fp32-ops
                       keeps the compiler from emitting packed FP32 math.
 """
@@ -52,7 +53,11 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     for src in srcs:
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
         objs.append(obj)
-        cmd = [hipcc(), *HIPCC_FLAGS, *inc, "-c", src, "-o", obj]
+        flags = list(HIPCC_FLAGS)
+        if src.endswith("_fast.hip"):
+            # fused translation units: a*b+c may become one FMA
+            flags[flags.index("-ffp-contract=off")] = "-ffp-contract=fast"
+        cmd = [hipcc(), *flags, *inc, "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

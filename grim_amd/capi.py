@@ -96,6 +96,7 @@ class NewtonOpts(C.Structure):
         ("semantics", C.c_int),
         ("compaction", C.c_int),
         ("block_size", C.c_int),
+        ("fused", C.c_int),
     ]
 
 
@@ -223,8 +224,8 @@ def batch_linear_solve(A, b, x, lu_precision=LU_FP64, stream=None):
 
 def batched_newton(p: Params, batch: ZoneBatch, jacobian=JAC_FD, lu_precision=LU_FP64,
                    semantics=NEWTON_PER_ZONE, compaction=0, block_size=0, iterations=None,
-                   residual_evals=None, final_l2=None, stream=None):
-    opts = NewtonOpts(jacobian, lu_precision, semantics, compaction, block_size)
+                   residual_evals=None, final_l2=None, stream=None, fused=0):
+    opts = NewtonOpts(jacobian, lu_precision, semantics, compaction, block_size, fused)
     st = NewtonStats(_ptr(iterations), _ptr(residual_evals), _ptr(final_l2))
     check(lib().grim_batched_newton(C.byref(p), C.byref(batch), C.byref(opts), C.byref(st), _stream(stream)),
           "grim_batched_newton")

@@ -1,24 +1,31 @@
 // Analytic Jacobian of the per-zone residual (ideal MHD) and the fast FP64
 // residual of the mixed-precision Newton path.
 //
-// Tangent (forward-mode, by hand).  For R = s (U(P) - rhs), s = mask/dt, with
-// the magnetic field held fixed, the columns of dU/dP are:
-//   rho:  dT^0_nu = u^0 u_nu,                       dU_rho = g u^0
-//   u:    dT^0_nu = Gamma u^0 u_nu + (Gamma-1) d_0nu, dU_rho = 0
-//   utilde^i (i = 1..3), with gamma_i = d gamma/d utilde^i = utilde_i / gamma
-//   (utilde_i lowered with the spatial metric):
-//     du^0     = gamma_i / alpha
-//     du^j     = delta_ij - gamma_i alpha g^0j
-//     du_mu    = g_mu,i - alpha gamma_i delta_mu0
-//                (lowering du^nu = gamma_i (-alpha g^0nu) + delta^nu_i and
-//                 g_mu,nu g^nu,0 = delta_mu0)
-//     db^0     = B_i = g_ij B^j
-//     db^mu    = (db^0 u^mu + b^0 du^mu - b^mu du^0) / u^0   (same for b_mu)
-//     d(b^2)   = 2 b_mu db^mu
-//     dT^0_nu  = d(b^2) u^0 u_nu + w du^0 u_nu + w u^0 du_nu
-//                + 1/2 d(b^2) d_0nu - db^0 b_nu - b^0 db_nu
-//     dU_rho   = g rho du^0
+// Identities used (normal-observer split, beta^i = alpha^2 g^0i,
+// beta_j = g_0j, gamma_ij = g_ij):
+//   u^0 = gamma/alpha, u_i = gamma_ij utilde^j, u_0 = -alpha gamma + g_0j utilde^j
+//   b^0 = B^i u_i,  b_mu = (B_mu + b^0 u_mu)/u^0  with B_mu = g_mu,j B^j
+//   b^mu b_mu = (B^i B_i + (b^0)^2)/(u^0)^2
+// so the t-row of the stress tensor needs no contravariant spatial vector.
+//
+// Tangent (forward mode, by hand).  For R = s (U(P) - rhs), s = mask/dt, the
+// magnetic field held fixed, the columns of dU/dP are:
+//   rho:  dT^0_nu = u^0 u_nu,                        dU_rho = g u^0
+//   u:    dT^0_nu = Gamma u^0 u_nu + (Gamma-1) d_0nu,  dU_rho = 0
+//   utilde^i, with gamma_i = utilde_i / gamma:
+//     du^0  = gamma_i / alpha
+//     du_nu = g_nu,i - alpha gamma_i d_0nu
+//     db^0  = B_i
+//     d(b^2)= 2 (b^0 (b_i - alpha gamma_i b^0) - (b.b) du^0) / u^0
+//     dT^0_nu = c1 u_nu + c2 du_nu + c3 b_nu + 1/2 d(b^2) d_0nu
+//       c1 = d(b^2) u^0 + w du^0 - b^0 db^0 / u^0
+//       c2 = w u^0 - (b^0)^2 / u^0
+//       c3 = b^0 du^0 / u^0 - db^0
+//     dU_rho = g rho du^0
 //   and dU_U = g dT^0_0 + dU_rho, dU_Ui = g dT^0_i.
+// (Expanding db_nu = (db^0 u_nu + b^0 du_nu - b_nu du^0)/u^0 in
+//  dT^0_nu = d(b^2) u^0 u_nu + w du^0 u_nu + w u^0 du_nu - db^0 b_nu - b^0 db_nu
+//  + 1/2 d(b^2) d_0nu gives the c1, c2, c3 grouping.)
 // Floors: set() uses max(P, floor); the derivative follows the one-sided
 // forward difference of grim's FD rule (1 when P >= floor, else 0).
 //
@@ -42,7 +49,6 @@ __device__ __forceinline__ void analyticJacobianIdeal(const double* Pd, const Zo
 #pragma unroll
     for (int k = 0; k < 4; k++) gc[m][k] = (T)Gd.cov(m, k);
   const T alpha = (T)Gd.alpha, g = (T)Gd.g;
-  const T gcon0[4] = {(T)Gd.gcon0[0], (T)Gd.gcon0[1], (T)Gd.gcon0[2], (T)Gd.gcon0[3]};
   const T sRho = (Pd[RHO] >= prm.rhoFloor) ? T(1) : T(0);
   const T sU = (Pd[UU] >= prm.uFloor) ? T(1) : T(0);
   const T rho = (T)smax(Pd[RHO], prm.rhoFloor);
@@ -61,70 +67,55 @@ __device__ __forceinline__ void analyticJacobianIdeal(const double* Pd, const Zo
   }
   const T lor = sqrt(T(1) + ut[0] * utLow[0] + ut[1] * utLow[1] + ut[2] * utLow[2]);
   const T invLor = T(1) / lor;
-  T uCon[4], uCov[4], bCon[4], bCov[4];
-  uCon[0] = lor / alpha;
+  const T invAlpha = T(1) / alpha;
+  const T u0 = lor * invAlpha;
+  const T invU0 = alpha * invLor;
+  T uCov[4];
+  uCov[0] = -alpha * lor + gc[0][1] * ut[0] + gc[0][2] * ut[1] + gc[0][3] * ut[2];
 #pragma unroll
-  for (int j = 0; j < 3; j++) uCon[j + 1] = ut[j] - lor * gcon0[j + 1] * alpha;
+  for (int i = 0; i < 3; i++) uCov[i + 1] = utLow[i];
+  const T b0 = B[0] * utLow[0] + B[1] * utLow[1] + B[2] * utLow[2];
+  T bCov[4];
+  bCov[0] = (gc[0][1] * B[0] + gc[0][2] * B[1] + gc[0][3] * B[2] + b0 * uCov[0]) * invU0;
 #pragma unroll
-  for (int m = 0; m < 4; m++)
-    uCov[m] = gc[m][0] * uCon[0] + gc[m][1] * uCon[1] + gc[m][2] * uCon[2] + gc[m][3] * uCon[3];
-  const T invU0 = T(1) / uCon[0];
-  bCon[0] = B[0] * uCov[1] + B[1] * uCov[2] + B[2] * uCov[3];
-#pragma unroll
-  for (int j = 0; j < 3; j++) bCon[j + 1] = (B[j] + bCon[0] * uCon[j + 1]) * invU0;
-#pragma unroll
-  for (int m = 0; m < 4; m++)
-    bCov[m] = gc[m][0] * bCon[0] + gc[m][1] * bCon[1] + gc[m][2] * bCon[2] + gc[m][3] * bCon[3];
-  const T bsq = bCon[0] * bCov[0] + bCon[1] * bCov[1] + bCon[2] * bCov[2] + bCon[3] * bCov[3] +
-                (T)prm.bSqrFloor;
+  for (int i = 0; i < 3; i++) bCov[i + 1] = (BLow[i] + b0 * utLow[i]) * invU0;
+  const T BB = B[0] * BLow[0] + B[1] * BLow[1] + B[2] * BLow[2];
+  const T bb = (BB + b0 * b0) * invU0 * invU0;
+  const T bsq = bb + (T)prm.bSqrFloor;
   const T w = rho + u + p + bsq;
   const T s = (T)scale;
   const T gs = g * s;
 
   // rho and u columns
   {
-    const T a0 = sRho * uCon[0];
+    const T a0 = sRho * u0;
     A[RHO][RHO] = gs * a0;
     A[UU][RHO] = gs * (a0 * uCov[0] + a0);
 #pragma unroll
     for (int i = 0; i < 3; i++) A[U1 + i][RHO] = gs * (a0 * uCov[i + 1]);
-    const T wu = gam * sU;
-    const T a1 = wu * uCon[0];
+    const T a1 = gam * sU * u0;
     A[RHO][UU] = T(0);
     A[UU][UU] = gs * (a1 * uCov[0] + (gam - T(1)) * sU);
 #pragma unroll
     for (int i = 0; i < 3; i++) A[U1 + i][UU] = gs * (a1 * uCov[i + 1]);
   }
   // velocity columns
-  const T wu0 = w * uCon[0];
+  const T c2 = w * u0 - b0 * b0 * invU0;
+  const T gRho = g * rho;
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     const T gi = utLow[i] * invLor;
-    const T du0 = gi / alpha;
-    T duCon[4], duCov[4], dbCon[4], dbCov[4];
-    duCon[0] = du0;
-#pragma unroll
-    for (int j = 0; j < 3; j++) duCon[j + 1] = ((i == j) ? T(1) : T(0)) - gi * alpha * gcon0[j + 1];
-#pragma unroll
-    for (int m = 0; m < 4; m++) duCov[m] = gc[m][i + 1];
-    duCov[0] = duCov[0] - alpha * gi;
+    const T du0 = gi * invAlpha;
     const T db0 = BLow[i];
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
-      dbCon[m] = (db0 * uCon[m] + bCon[0] * duCon[m] - bCon[m] * du0) * invU0;
-      dbCov[m] = (db0 * uCov[m] + bCon[0] * duCov[m] - bCov[m] * du0) * invU0;
-    }
-    dbCon[0] = db0;
-    dbCov[0] = (db0 * uCov[0] + bCon[0] * duCov[0] - bCov[0] * du0) * invU0;
-    const T dbsq = T(2) * (bCov[0] * dbCon[0] + bCov[1] * dbCon[1] + bCov[2] * dbCon[2] +
-                           bCov[3] * dbCon[3]);
+    const T bdu = bCov[i + 1] - alpha * gi * b0;
+    const T dbsq = T(2) * (b0 * bdu - bb * du0) * invU0;
+    const T c1 = dbsq * u0 + w * du0 - b0 * db0 * invU0;
+    const T c3 = b0 * du0 * invU0 - db0;
     T dT[4];
 #pragma unroll
-    for (int nu = 0; nu < 4; nu++)
-      dT[nu] = (dbsq * uCon[0] + w * du0) * uCov[nu] + wu0 * duCov[nu] - db0 * bCov[nu] -
-               bCon[0] * dbCov[nu];
-    dT[0] = dT[0] + T(0.5) * dbsq;
-    const T dUrho = g * rho * du0;
+    for (int nu = 0; nu < 4; nu++) dT[nu] = c1 * uCov[nu] + c2 * gc[nu][i + 1] + c3 * bCov[nu];
+    dT[0] = dT[0] - c2 * alpha * gi + T(0.5) * dbsq;
+    const T dUrho = gRho * du0;
     A[RHO][U1 + i] = s * dUrho;
     A[UU][U1 + i] = s * (g * dT[0] + dUrho);
 #pragma unroll
@@ -132,11 +123,11 @@ __device__ __forceinline__ void analyticJacobianIdeal(const double* Pd, const Zo
   }
 }
 
-// Residual of the mixed-precision path (ideal MHD), FP64 throughout but with
-// the divisions of set() and of (U - rhs)/dt replaced by multiplications with
-// reciprocals formed once per evaluation (1/alpha and mask/dt once per zone):
-// 2 FP64 divisions instead of 9.  Same mathematics as residual(); rounding
-// differs at the 1e-16 level, far below the 1e-10 relative tolerance.
+// Residual of the mixed-precision path (ideal MHD): FP64 throughout, the same
+// mathematics as residual() evaluated through the identities above, with one
+// FP64 division (1/gamma) and one square root per evaluation instead of nine
+// divisions; 1/alpha and mask/dt come in per zone.  Rounding differs from the
+// grim-order residual at the 1e-16 level, far below the 1e-10 tolerance.
 template <class L>
 __device__ __forceinline__ void residualFastIdeal(const double* P, const ZoneGeom& G,
                                                   const ZoneConst<L>& c, const DevParams& prm,
@@ -144,42 +135,33 @@ __device__ __forceinline__ void residualFastIdeal(const double* P, const ZoneGeo
   static_assert(!L::EMHD, "ideal-MHD residual");
   const double rho = smax(P[RHO], prm.rhoFloor);
   const double u = smax(P[UU], prm.uFloor);
-  const double u1 = P[U1], u2 = P[U2], u3 = P[U3];
-  const double B1 = P[L::B1], B2 = P[L::B2], B3 = P[L::B3];
+  const double ut[3] = {P[U1], P[U2], P[U3]};
+  const double B[3] = {P[L::B1], P[L::B2], P[L::B3]};
   const double p = (prm.gam - 1.) * u;
-  const double lor = sqrt(1. + G.cov(1, 1) * u1 * u1 + G.cov(2, 2) * u2 * u2 + G.cov(3, 3) * u3 * u3 +
-                          2. * (G.cov(1, 2) * u1 * u2 + G.cov(1, 3) * u1 * u3 + G.cov(2, 3) * u2 * u3));
-  double uCon[4], uCov[4], bCon[4], bCov[4];
-  uCon[0] = lor * invAlpha;
-  uCon[1] = u1 - lor * G.gcon0[1] * G.alpha;
-  uCon[2] = u2 - lor * G.gcon0[2] * G.alpha;
-  uCon[3] = u3 - lor * G.gcon0[3] * G.alpha;
+  double utLow[3], BLow[3];
 #pragma unroll
-  for (int mu = 0; mu < 4; mu++)
-    uCov[mu] = G.cov(mu, 0) * uCon[0] + G.cov(mu, 1) * uCon[1] + G.cov(mu, 2) * uCon[2] +
-               G.cov(mu, 3) * uCon[3];
+  for (int i = 0; i < 3; i++) {
+    utLow[i] = G.cov(i + 1, 1) * ut[0] + G.cov(i + 1, 2) * ut[1] + G.cov(i + 1, 3) * ut[2];
+    BLow[i] = G.cov(i + 1, 1) * B[0] + G.cov(i + 1, 2) * B[1] + G.cov(i + 1, 3) * B[2];
+  }
+  const double lor = sqrt(1. + ut[0] * utLow[0] + ut[1] * utLow[1] + ut[2] * utLow[2]);
+  const double u0 = lor * invAlpha;
   const double invU0 = G.alpha / lor;
-  bCon[0] = B1 * uCov[1] + B2 * uCov[2] + B3 * uCov[3];
-  bCon[1] = (B1 + bCon[0] * uCon[1]) * invU0;
-  bCon[2] = (B2 + bCon[0] * uCon[2]) * invU0;
-  bCon[3] = (B3 + bCon[0] * uCon[3]) * invU0;
-#pragma unroll
-  for (int mu = 0; mu < 4; mu++)
-    bCov[mu] = G.cov(mu, 0) * bCon[0] + G.cov(mu, 1) * bCon[1] + G.cov(mu, 2) * bCon[2] +
-               G.cov(mu, 3) * bCon[3];
-  const double bsq = bCon[0] * bCov[0] + bCon[1] * bCov[1] + bCon[2] * bCov[2] + bCon[3] * bCov[3] +
-                     prm.bSqrFloor;
-  const double wu0 = (rho + u + p + bsq) * uCon[0];
-  double T0[4];
-#pragma unroll
-  for (int nu = 0; nu < 4; nu++) T0[nu] = wu0 * uCov[nu] - bCon[0] * bCov[nu];
-  T0[0] = T0[0] + (p + 0.5 * bsq);
-  const double Urho = G.g * (rho * uCon[0]);
+  const double uCov0 = -G.alpha * lor + G.cov(0, 1) * ut[0] + G.cov(0, 2) * ut[1] + G.cov(0, 3) * ut[2];
+  const double b0 = B[0] * utLow[0] + B[1] * utLow[1] + B[2] * utLow[2];
+  const double bCov0 = (G.cov(0, 1) * B[0] + G.cov(0, 2) * B[1] + G.cov(0, 3) * B[2] + b0 * uCov0) * invU0;
+  const double BB = B[0] * BLow[0] + B[1] * BLow[1] + B[2] * BLow[2];
+  const double bsq = (BB + b0 * b0) * invU0 * invU0 + prm.bSqrFloor;
+  const double wu0 = (rho + u + p + bsq) * u0;
+  const double Urho = G.g * (rho * u0);
+  const double T00 = wu0 * uCov0 - b0 * bCov0 + (p + 0.5 * bsq);
   R[RHO] = (Urho - c.rhs[RHO]) * scale;
-  R[UU] = (G.g * T0[0] + Urho - c.rhs[UU]) * scale;
-  R[U1] = (G.g * T0[1] - c.rhs[U1]) * scale;
-  R[U2] = (G.g * T0[2] - c.rhs[U2]) * scale;
-  R[U3] = (G.g * T0[3] - c.rhs[U3]) * scale;
+  R[UU] = (G.g * T00 + Urho - c.rhs[UU]) * scale;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const double bCovi = (BLow[i] + b0 * utLow[i]) * invU0;
+    R[U1 + i] = (G.g * (wu0 * utLow[i] - b0 * bCovi) - c.rhs[U1 + i]) * scale;
+  }
 }
 
 // Row and column equilibration around the FP32 LU: solve (Dr A Dc) y = Dr b,

@@ -1,0 +1,32 @@
+// Batched per-zone Newton solve: the fused translation unit, compiled with
+// -ffp-contract=fast so that a*b+c becomes one fused multiply-add.
+//
+// Holds the modes whose results are checked against tolerances rather than
+// bit for bit: the mixed-precision path (FP64 residual and update, FP32
+// analytic Jacobian and LU) and the fast FD-FP64 variant.
+#include <hip/hip_runtime.h>
+
+#include "grim_hip.h"
+#include "newton_kernels.hpp"
+#include "runtime.hpp"
+
+namespace grim {
+
+int launchNewtonFast(int mode, const grim_params& p, const grim_zone_batch& b,
+                     const grim_newton_stats& st, hipStream_t s) {
+  const DevParams prm = makeDevParams(p);
+  const unsigned grid = gridFor(b.n, kDefaultBlock);
+  return dispatchLayout(p, [&]<class L>() -> int {
+    if (mode == MODE_FD_FAST) {
+      newtonPerZone<L, MODE_FD_FAST><<<grid, kDefaultBlock, 0, s>>>(prm, b, st);
+    } else if constexpr (!L::EMHD) {
+      newtonPerZone<L, MODE_MIXED><<<grid, kDefaultBlock, 0, s>>>(prm, b, st);
+    } else {
+      setError("launchNewtonFast: the mixed path is built for ideal MHD only");
+      return GRIM_ERR_BAD_ARG;
+    }
+    return checkLaunch("grim_batched_newton (fused)");
+  });
+}
+
+}  // namespace grim

@@ -98,6 +98,10 @@ typedef struct grim_newton_opts {
   int semantics;     /* grim_semantics */
   int compaction;    /* 1: persistent kernel over a compacted work list */
   int block_size;    /* threads per workgroup (0 = library default) */
+  int fused;         /* FD path only: 0 = grim's operation order, no fused
+                        multiply-add (bit-reproducible against a CPU FP64
+                        evaluation); 1 = fused multiply-adds and reciprocal
+                        forms.  The mixed path is always fused. */
 } grim_newton_opts;
 
 /* Per-zone outputs are optional device arrays (NULL to skip). */
