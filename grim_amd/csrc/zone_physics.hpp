@@ -22,11 +22,18 @@ namespace grim {
 
 enum { RHO = 0, UU = 1, U1 = 2, U2 = 3, U3 = 4 };
 
-// gCov plane index of the symmetric (mu, nu) component.
+// gCov plane index of the symmetric (mu, nu) component: upper-triangle
+// packing 00 01 02 03 11 12 13 22 23 33.  Closed form (no recursion or
+// branches) so that it folds to a constant in every unrolled loop; a
+// runtime index would demote the register array to scratch memory.
 __host__ __device__ constexpr int gcovIdx(int mu, int nu) {
-  return (mu > nu) ? gcovIdx(nu, mu)
-                   : (mu == 0 ? nu : (mu == 1 ? 3 + nu : (mu == 2 ? 5 + nu : 9)));
+  return (mu < nu ? mu : nu) * 4 - ((mu < nu ? mu : nu) * ((mu < nu ? mu : nu) + 1)) / 2 +
+         (mu < nu ? nu : mu);
 }
+static_assert(gcovIdx(0, 0) == 0 && gcovIdx(3, 0) == 3 && gcovIdx(1, 1) == 4 &&
+                  gcovIdx(3, 1) == 6 && gcovIdx(2, 2) == 7 && gcovIdx(2, 3) == 8 &&
+                  gcovIdx(3, 3) == 9,
+              "gCov packing");
 enum { GEOM_GCON0 = 10, GEOM_ALPHA = 14, GEOM_G = 15 };
 
 // Scalars of grim_params the device code reads, plus the closure selector.
@@ -364,35 +371,31 @@ __device__ __forceinline__ double sumSq(const double* R) {
 // Dense LU with partial pivoting (pivot: first row with the largest |A(i,k)|),
 // reciprocal-of-pivot multipliers, in place on register arrays.  Solves
 // A x = b, x returned in b.  Real is double or float.
+//
+// The pivot row is brought up by compare-and-swap: row k trades places with
+// row i whenever |A(i,k)| exceeds the current |A(k,k)|.  That selects the
+// same pivot as LAPACK's search-then-swap, and every remaining row is
+// eliminated by the same operations wherever it sits, so the solution is
+// bit-identical to the search-then-swap order (exact ties aside).  Every
+// access stays at a compile-time index: a swap with a computed row index
+// would force the matrix out of registers into scratch memory.
 template <class Real, int N>
 __device__ __forceinline__ void luSolve(Real (&A)[N][N], Real (&b)[N]) {
   Real rinv[N];
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    // pivot search and row swap as selects over static indices
-    Real maxv = fabs(A[k][k]);
-    int piv = k;
 #pragma unroll
     for (int i = k + 1; i < N; i++) {
-      const Real v = fabs(A[i][k]);
-      if (v > maxv) {
-        maxv = v;
-        piv = i;
-      }
-    }
+      const bool swap = fabs(A[i][k]) > fabs(A[k][k]);
 #pragma unroll
-    for (int i = k + 1; i < N; i++) {
-      if (piv == i) {
-#pragma unroll
-        for (int j = 0; j < N; j++) {
-          const Real t = A[k][j];
-          A[k][j] = A[i][j];
-          A[i][j] = t;
-        }
-        const Real t = b[k];
-        b[k] = b[i];
-        b[i] = t;
+      for (int j = k; j < N; j++) {
+        const Real hi = A[k][j], lo = A[i][j];
+        A[k][j] = swap ? lo : hi;
+        A[i][j] = swap ? hi : lo;
       }
+      const Real hi = b[k], lo = b[i];
+      b[k] = swap ? lo : hi;
+      b[i] = swap ? hi : lo;
     }
     rinv[k] = Real(1.) / A[k][k];
 #pragma unroll
