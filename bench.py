@@ -38,7 +38,10 @@ def parse():
     ap.add_argument("--zones", type=int, default=N_ZONES)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-reps", type=int, default=5)
-    ap.add_argument("--paths", default=",".join(["fd_fp64", "fd_fp64_fused", "analytic_fp64", "mixed"]),
+    ap.add_argument("--phase1", type=int, default=2)
+    ap.add_argument("--phase2-group", type=int, default=4)
+    ap.add_argument("--paths", default=",".join(["fd_fp64", "fd_fp64_fused", "analytic_fp64", "mixed",
+                                                 "mixed_compacted"]),
                     help="comma-separated subset of the Newton paths to time (profiling)")
     return ap.parse_args()
 
@@ -73,6 +76,10 @@ def path_flops(path, iters, res_evals, nf=5):
         f64 = (res_evals * OPS["res_fast"] + ls * OPS["eval_bookkeeping"]
                + iters * (OPS["fd_quotients_fused"] + OPS["lu"] + OPS["update"]))
         return f64, 0.0
+    if path.startswith("mixed"):
+        f64 = res_evals * (OPS["res_fast"] + OPS["eval_bookkeeping"]) + iters * OPS["update"]
+        f32 = iters * (OPS["analytic_jac"] + OPS["equilibration"] + OPS["lu"])
+        return f64, f32
     if path == "analytic_fp64":
         f64 = (res_evals * (OPS["res_exact"] + OPS["eval_bookkeeping"])
                + iters * (OPS["analytic_jac"] + OPS["lu"] + OPS["update"]))
@@ -87,6 +94,7 @@ PATHS = {
     "fd_fp64_fused": ("JAC_FD", "LU_FP64", 1, "newtonPerZone<Layout<false,false>, MODE_FD_FAST>"),
     "analytic_fp64": ("JAC_ANALYTIC", "LU_FP64", 0, "newtonPerZone<Layout<false,false>, MODE_ANALYTIC64>"),
     "mixed": ("JAC_ANALYTIC", "LU_FP32", 1, "newtonPerZone<Layout<false,false>, MODE_MIXED>"),
+    "mixed_compacted": ("JAC_ANALYTIC", "LU_FP32", 1, "newtonPhase1+newtonPhase2<Layout<false,false>, MODE_MIXED>"),
 }
 HEADLINE = "mixed"
 # FP32 VALU without packed math (this build disables packed FP32) issues one
@@ -121,6 +129,7 @@ def main():
     ev = torch.empty(n, dtype=torch.int32, device="cuda")
     l2 = torch.empty(n, dtype=torch.float64, device="cuda")
     batch = capi.make_batch(prim, b["geom"], rhs, residual_mask=mask)
+    ws = torch.empty(capi.workspace_bytes(n), dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
     torch.cuda.synchronize()
     results = {}
@@ -130,13 +139,17 @@ def main():
         jac, lu, fused, kname = PATHS[path]
         jac, lu = getattr(capi, jac), getattr(capi, lu)
 
+        opts = capi.newton_opts(jac, lu, fused=fused)
+        if path == "mixed_compacted":
+            opts = capi.newton_opts(jac, lu, fused=fused, compaction=1, phase1_iters=args.phase1,
+                                    phase2_group=args.phase2_group, workspace=ws)
+
         def step(ev_mid=None):
             prim[: p.nf].copy_(b["guess"][: p.nf])
             if ev_mid is not None:
                 ev_mid.record(stream)
-            capi.batched_newton(p, batch, jac, lu, capi.NEWTON_PER_ZONE,
-                                iterations=it, residual_evals=ev, final_l2=l2, stream=stream,
-                                fused=fused)
+            capi.batched_newton(p, batch, iterations=it, residual_evals=ev, final_l2=l2, stream=stream,
+                                opts=opts)
 
         for _ in range(args.warmup):
             step()

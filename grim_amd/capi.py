@@ -97,6 +97,10 @@ class NewtonOpts(C.Structure):
         ("compaction", C.c_int),
         ("block_size", C.c_int),
         ("fused", C.c_int),
+        ("phase1_iters", C.c_int),
+        ("phase2_group", C.c_int),
+        ("workspace", C.c_void_p),
+        ("workspace_bytes", C.c_int64),
     ]
 
 
@@ -124,6 +128,8 @@ SIGNATURES = {
     "grim_count_above": (_i, [_i64, _vp, _d, C.POINTER(C.c_int64), _vp]),
     "grim_generate_kerr_schild_batch": (_i, [_P, _i64, C.c_uint64, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
     "grim_known_root_rhs": (_i, [_P, _B, _vp, _vp, _vp]),
+    "grim_newton_workspace_bytes": (C.c_int64, [_i64]),
+    "grim_autotune_newton": (_i, [_P, _B, C.POINTER(NewtonOpts), _vp]),
 }
 
 _lib = None
@@ -222,10 +228,30 @@ def batch_linear_solve(A, b, x, lu_precision=LU_FP64, stream=None):
     return x
 
 
+def newton_opts(jacobian=JAC_FD, lu_precision=LU_FP64, semantics=NEWTON_PER_ZONE, compaction=0,
+                block_size=0, fused=0, phase1_iters=0, phase2_group=0, workspace=None) -> NewtonOpts:
+    return NewtonOpts(jacobian, lu_precision, semantics, compaction, block_size, fused, phase1_iters,
+                      phase2_group, _ptr(workspace), 0 if workspace is None else workspace.numel() *
+                      workspace.element_size())
+
+
+def workspace_bytes(n: int) -> int:
+    return int(lib().grim_newton_workspace_bytes(n))
+
+
+def autotune_newton(p: Params, batch: ZoneBatch, opts: NewtonOpts, stream=None) -> NewtonOpts:
+    """Pick the fastest launch configuration for this batch (in place on a copy)."""
+    o = NewtonOpts.from_buffer_copy(opts)
+    check(lib().grim_autotune_newton(C.byref(p), C.byref(batch), C.byref(o), _stream(stream)),
+          "grim_autotune_newton")
+    return o
+
+
 def batched_newton(p: Params, batch: ZoneBatch, jacobian=JAC_FD, lu_precision=LU_FP64,
                    semantics=NEWTON_PER_ZONE, compaction=0, block_size=0, iterations=None,
-                   residual_evals=None, final_l2=None, stream=None, fused=0):
-    opts = NewtonOpts(jacobian, lu_precision, semantics, compaction, block_size, fused)
+                   residual_evals=None, final_l2=None, stream=None, fused=0, opts=None):
+    if opts is None:
+        opts = newton_opts(jacobian, lu_precision, semantics, compaction, block_size, fused)
     st = NewtonStats(_ptr(iterations), _ptr(residual_evals), _ptr(final_l2))
     check(lib().grim_batched_newton(C.byref(p), C.byref(batch), C.byref(opts), C.byref(st), _stream(stream)),
           "grim_batched_newton")

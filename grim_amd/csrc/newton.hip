@@ -320,17 +320,25 @@ int grim_batch_linear_solve(int64_t n, int nvar, const double* A, const double* 
   return checkLaunch("grim_batch_linear_solve");
 }
 
+int64_t grim_newton_workspace_bytes(int64_t n) {
+  // one 64-bit mask per 64 zones, rounded up to 256 bytes
+  const int64_t bytes = ((n + 63) / 64) * (int64_t)sizeof(unsigned long long);
+  return ((bytes + 255) / 256) * 256;
+}
+
 int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
                         const grim_newton_opts* opts, grim_newton_stats* stats,
                         grim_stream_t stream) {
   GRIM_REQUIRE(validBatch(p, b), "grim_batched_newton: bad params or batch");
-  grim_newton_opts o = {GRIM_JAC_FD, GRIM_LU_FP64, GRIM_NEWTON_PER_ZONE, 0, 0, 0};
+  grim_newton_opts o = {};
+  o.jacobian = GRIM_JAC_FD;
+  o.lu_precision = GRIM_LU_FP64;
+  o.semantics = GRIM_NEWTON_PER_ZONE;
   if (opts) o = *opts;
   grim_newton_stats st = {nullptr, nullptr, nullptr};
   if (stats) st = *stats;
   GRIM_REQUIRE(o.semantics == GRIM_NEWTON_PER_ZONE || o.semantics == GRIM_NEWTON_GLOBAL,
                "grim_batched_newton: bad semantics");
-  GRIM_REQUIRE(!o.compaction, "grim_batched_newton: compaction is not built");
   int mode = -1;
   if (o.jacobian == GRIM_JAC_FD && o.lu_precision == GRIM_LU_FP64) mode = o.fused ? MODE_FD_FAST : MODE_FD;
   if (o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP64) mode = MODE_ANALYTIC64;
@@ -340,22 +348,58 @@ int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
                "grim_batched_newton: the analytic Jacobian is built for ideal MHD only");
   GRIM_REQUIRE(mode == MODE_FD || o.semantics == GRIM_NEWTON_PER_ZONE,
                "grim_batched_newton: lock-step semantics run grim's exact FD-FP64 algorithm only");
+  GRIM_REQUIRE(o.semantics == GRIM_NEWTON_PER_ZONE || !o.compaction,
+               "grim_batched_newton: compaction applies to per-zone semantics");
+  GRIM_REQUIRE(o.block_size == 0 || o.block_size == 64 || o.block_size == 128 || o.block_size == 256,
+               "grim_batched_newton: block_size must be 64, 128 or 256");
+  GRIM_REQUIRE(o.phase1_iters >= 0 && o.phase2_group >= 0 && o.phase2_group <= kMaxGroup,
+               "grim_batched_newton: bad compaction options");
   if (b->n == 0) return GRIM_OK;
   if (int rc = checkDevice()) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (o.semantics == GRIM_NEWTON_GLOBAL) return launchGlobal(*p, *b, st, s);
+
+  LaunchCfg cfg;
+  cfg.block = o.block_size ? o.block_size : kDefaultBlock;
+  cfg.compaction = o.compaction != 0;
+  cfg.budget1 = o.phase1_iters ? o.phase1_iters : 2;
+  cfg.group = o.phase2_group ? o.phase2_group : 4;
+  cfg.masks = nullptr;
+  void* owned = nullptr;
+  const int64_t need = grim_newton_workspace_bytes(b->n);
+  if (cfg.compaction) {
+    void* ws = o.workspace;
+    if (ws) {
+      GRIM_REQUIRE(o.workspace_bytes >= need, "grim_batched_newton: workspace too small");
+    } else {
+      GRIM_HIP_TRY(hipMallocAsync(&owned, (size_t)need, s));
+      ws = owned;
+    }
+    cfg.masks = (unsigned long long*)ws;
+  }
   const DevParams prm = makeDevParams(*p);
   const grim_zone_batch bb = *b;
-  if (mode == MODE_MIXED || mode == MODE_FD_FAST) return launchNewtonFast(mode, *p, bb, st, s);
-  const unsigned grid = gridFor(bb.n, kDefaultBlock);
-  return dispatchLayout(*p, [&]<class L>() -> int {
-    if (mode == MODE_FD) {
-      newtonPerZone<L, MODE_FD><<<grid, kDefaultBlock, 0, s>>>(prm, bb, st);
-    } else if constexpr (!L::EMHD) {
-      newtonPerZone<L, MODE_ANALYTIC64><<<grid, kDefaultBlock, 0, s>>>(prm, bb, st);
+  int rc;
+  if (mode == MODE_MIXED || mode == MODE_FD_FAST) {
+    rc = launchNewtonFast(mode, *p, bb, st, cfg, s);
+  } else {
+    rc = dispatchLayout(*p, [&]<class L>() -> int {
+      if (mode == MODE_FD) {
+        launchSolve<L, MODE_FD>(prm, bb, st, cfg, s);
+      } else if constexpr (!L::EMHD) {
+        launchSolve<L, MODE_ANALYTIC64>(prm, bb, st, cfg, s);
+      }
+      return checkLaunch("grim_batched_newton");
+    });
+  }
+  if (owned) {
+    hipError_t e = hipFreeAsync(owned, s);
+    if (rc == GRIM_OK && e != hipSuccess) {
+      setError(std::string("hipFreeAsync: ") + hipGetErrorString(e));
+      rc = GRIM_ERR_HIP;
     }
-    return checkLaunch("grim_batched_newton");
-  });
+  }
+  return rc;
 }
 
 }  // extern "C"

@@ -13,14 +13,13 @@
 namespace grim {
 
 int launchNewtonFast(int mode, const grim_params& p, const grim_zone_batch& b,
-                     const grim_newton_stats& st, hipStream_t s) {
+                     const grim_newton_stats& st, const LaunchCfg& cfg, hipStream_t s) {
   const DevParams prm = makeDevParams(p);
-  const unsigned grid = gridFor(b.n, kDefaultBlock);
   return dispatchLayout(p, [&]<class L>() -> int {
     if (mode == MODE_FD_FAST) {
-      newtonPerZone<L, MODE_FD_FAST><<<grid, kDefaultBlock, 0, s>>>(prm, b, st);
+      launchSolve<L, MODE_FD_FAST>(prm, b, st, cfg, s);
     } else if constexpr (!L::EMHD) {
-      newtonPerZone<L, MODE_MIXED><<<grid, kDefaultBlock, 0, s>>>(prm, b, st);
+      launchSolve<L, MODE_MIXED>(prm, b, st, cfg, s);
     } else {
       setError("launchNewtonFast: the mixed path is built for ideal MHD only");
       return GRIM_ERR_BAD_ARG;

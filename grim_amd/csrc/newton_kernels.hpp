@@ -24,6 +24,7 @@
 namespace grim {
 
 constexpr int kDefaultBlock = 256;
+constexpr int kMaxGroup = 8;  // phase-1 masks per phase-2 wavefront, at most
 
 // Jacobian / precision modes of the per-zone Newton kernel.
 //  MODE_FD:          grim's residual and FD Jacobian, FP64 LU (reference path)
@@ -153,13 +154,17 @@ __device__ __forceinline__ bool newtonUpdate(double* P, const double* R, double 
   return accepted;
 }
 
-// Per-zone semantics: each thread iterates its own zone until
-// sum R^2 <= atol or maxNonLinearIter updates.
+// Newton iterations of one zone, from its current primitives, for at most
+// `budget` updates (solve.cpp:10-200 read per zone: the zone stops when its
+// own sum R^2 <= atol).  Writes the primitives back; stats are stored, or
+// added to the earlier phase's when `accumulate`.  Returns true when the
+// zone ran out of budget unconverged (a later phase may continue it: every
+// iteration is a function of P alone, so stopping and resuming changes no
+// result).
 template <class L, int MODE>
-__global__ __launch_bounds__(kDefaultBlock) void newtonPerZone(DevParams prm, grim_zone_batch b,
-                                                               grim_newton_stats st) {
-  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (z >= b.n) return;
+__device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_batch& b,
+                                          const grim_newton_stats& st, int64_t z, int budget,
+                                          bool accumulate, bool lastPhase) {
   const int64_t n = b.n;
   const double dt = b.dt_prime;
   ZoneGeom G;
@@ -180,7 +185,7 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonPerZone(DevParams prm, gr
   bool converged = false;
   int iters = 0, evals = 0;
   double l2 = 0.;
-  for (int it = 0; it < prm.maxIter; it++) {
+  for (int it = 0; it < budget; it++) {
     // floors at the top of every iteration (solve.cpp:17-18)
     const double p0 = smax(P[RHO], prm.rhoFloor);
     const double p1 = smax(P[UU], prm.uFloor);
@@ -205,14 +210,102 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonPerZone(DevParams prm, gr
     }
   }
   if (!converged) {
-    if (!haveR) evalResidual<L, MODE>(P, G, c, prm, dt, zs, R);
+    if (!haveR) {
+      evalResidual<L, MODE>(P, G, c, prm, dt, zs, R);
+      if (!lastPhase) evals++;
+    }
     l2 = sumSq<L::NF>(R);
+    if (!lastPhase && !(l2 > prm.atol)) {
+      // the one-pass solve would floor P at the top of its next iteration
+      // and then stop: do the same here
+      P[RHO] = smax(P[RHO], prm.rhoFloor);
+      P[UU] = smax(P[UU], prm.uFloor);
+      converged = true;
+    }
   }
 #pragma unroll
   for (int v = 0; v < L::NF; v++) b.prim[v * n + z] = P[v];
-  if (st.iterations) st.iterations[z] = iters;
-  if (st.residual_evals) st.residual_evals[z] = evals;
+  if (st.iterations) st.iterations[z] = (accumulate ? st.iterations[z] : 0) + iters;
+  if (st.residual_evals) st.residual_evals[z] = (accumulate ? st.residual_evals[z] : 0) + evals;
   if (st.final_l2) st.final_l2[z] = l2;
+  return !converged && !lastPhase;
+}
+
+// Per-zone semantics, one pass: each thread iterates its own zone until
+// sum R^2 <= atol or maxNonLinearIter updates.
+template <class L, int MODE>
+__global__ __launch_bounds__(kDefaultBlock) void newtonPerZone(DevParams prm, grim_zone_batch b,
+                                                               grim_newton_stats st) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= b.n) return;
+  solveZone<L, MODE>(prm, b, st, z, prm.maxIter, false, true);
+}
+
+// Compacting solve, phase 1: at most `budget1` updates per zone.  Each
+// wavefront stores the 64-bit ballot of its lanes left unconverged (no
+// atomics: a single global counter serialises 65536 wavefronts).
+template <class L, int MODE>
+__global__ __launch_bounds__(kDefaultBlock) void newtonPhase1(DevParams prm, grim_zone_batch b,
+                                                              grim_newton_stats st, int budget1,
+                                                              unsigned long long* __restrict__ masks) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool more = false;
+  if (z < b.n) more = solveZone<L, MODE>(prm, b, st, z, budget1, false, false);
+  const unsigned long long ballot = __ballot(more);
+  // lane 0 of a wavefront holding at least one zone (z < n) owns mask z/64;
+  // wavefronts wholly past the end of the batch own none
+  if ((threadIdx.x & 63) == 0 && z < b.n) masks[z >> 6] = ballot;
+}
+
+// Position of the k-th (0-based) set bit of m, by popcount bisection.
+__device__ __forceinline__ int selectBit(unsigned long long m, int k) {
+  int pos = 0;
+#pragma unroll
+  for (int step = 32; step >= 1; step >>= 1) {
+    const unsigned long long low = m & ((1ull << step) - 1ull);
+    const int c = __popcll(low);
+    if (k >= c) {
+      k -= c;
+      m >>= step;
+      pos += step;
+    } else {
+      m = low;
+    }
+  }
+  return pos;
+}
+
+// Compacting solve, phase 2: wavefront q takes the `group` phase-1 masks
+// [q*group, (q+1)*group), numbers their set bits in order, and its lanes
+// continue those zones for the remaining maxNonLinearIter - budget1 updates
+// (lane j takes bits j, j+64, ...).  The zones of one wavefront lie within
+// 64*group consecutive zones, so plane reads stay local.
+template <class L, int MODE>
+__global__ __launch_bounds__(kDefaultBlock) void newtonPhase2(DevParams prm, grim_zone_batch b,
+                                                              grim_newton_stats st, int budget2,
+                                                              const unsigned long long* __restrict__ masks,
+                                                              int group) {
+  const int lane = threadIdx.x & 63;
+  const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nMasks = (b.n + 63) >> 6;
+  const int64_t first = q * group;
+  if (first >= nMasks) return;
+  int prefix[kMaxGroup + 1];
+  unsigned long long m[kMaxGroup];
+  prefix[0] = 0;
+#pragma unroll
+  for (int r = 0; r < kMaxGroup; r++) {
+    m[r] = (r < group && first + r < nMasks) ? masks[first + r] : 0ull;
+    prefix[r + 1] = prefix[r] + __popcll(m[r]);
+  }
+  const int total = prefix[kMaxGroup];
+  for (int j = lane; j < total; j += 64) {
+    int64_t zone = 0;
+#pragma unroll
+    for (int r = 0; r < kMaxGroup; r++)
+      if (j >= prefix[r] && j < prefix[r + 1]) zone = (first + r) * 64 + selectBit(m[r], j - prefix[r]);
+    solveZone<L, MODE>(prm, b, st, zone, budget2, true, true);
+  }
 }
 
 template <class L, int MODE>
@@ -237,8 +330,35 @@ __device__ __forceinline__ void fdJacobianFast(const double* P, const double* R,
   }
 }
 
+// Launch configuration of the per-zone solve (chosen by the autotuner or the
+// caller).
+struct LaunchCfg {
+  int block;                 // threads per workgroup: 64, 128 or 256
+  bool compaction;           // two-phase compacting solve
+  int budget1;               // phase-1 update budget
+  int group;                 // phase-1 masks per phase-2 wavefront (1..kMaxGroup)
+  unsigned long long* masks; // workspace: one 64-bit mask per phase-1 wavefront
+};
+
+// Enqueue the per-zone solve of mode MODE on stream s.
+template <class L, int MODE>
+void launchSolve(const DevParams& prm, const grim_zone_batch& b, const grim_newton_stats& st,
+                 const LaunchCfg& cfg, hipStream_t s) {
+  const unsigned grid = (unsigned)((b.n + cfg.block - 1) / cfg.block);
+  if (!cfg.compaction || cfg.budget1 >= prm.maxIter) {
+    newtonPerZone<L, MODE><<<grid, cfg.block, 0, s>>>(prm, b, st);
+    return;
+  }
+  newtonPhase1<L, MODE><<<grid, cfg.block, 0, s>>>(prm, b, st, cfg.budget1, cfg.masks);
+  const int64_t nMasks = (b.n + 63) / 64;
+  const int64_t waves2 = (nMasks + cfg.group - 1) / cfg.group;
+  const unsigned grid2 = (unsigned)((waves2 * 64 + cfg.block - 1) / cfg.block);
+  newtonPhase2<L, MODE><<<grid2, cfg.block, 0, s>>>(prm, b, st, prm.maxIter - cfg.budget1, cfg.masks,
+                                                    cfg.group);
+}
+
 // Host launcher of the fast translation unit (newton_fast.hip).
 int launchNewtonFast(int mode, const grim_params& p, const grim_zone_batch& b,
-                     const grim_newton_stats& st, hipStream_t s);
+                     const grim_newton_stats& st, const LaunchCfg& cfg, hipStream_t s);
 
 }  // namespace grim

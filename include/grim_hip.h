@@ -102,7 +102,18 @@ typedef struct grim_newton_opts {
                         multiply-add (bit-reproducible against a CPU FP64
                         evaluation); 1 = fused multiply-adds and reciprocal
                         forms.  The mixed path is always fused. */
+  int phase1_iters;  /* compaction: Newton updates before the work list is
+                        compacted (0 = library default) */
+  int phase2_group;  /* compaction: phase-1 wavefronts whose unconverged zones
+                        one phase-2 wavefront takes over, 1..8 (0 = default) */
+  void* workspace;   /* compaction: device scratch of at least
+                        grim_newton_workspace_bytes(n) bytes, or NULL to let
+                        the library allocate it on the stream */
+  int64_t workspace_bytes;
 } grim_newton_opts;
+
+/* Device scratch the compacting solve needs for n zones. */
+int64_t grim_newton_workspace_bytes(int64_t n);
 
 /* Per-zone outputs are optional device arrays (NULL to skip). */
 typedef struct grim_newton_stats {
@@ -148,6 +159,14 @@ int grim_batch_linear_solve(int64_t n, int nvar, const double* A,
 int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
                         const grim_newton_opts* opts, grim_newton_stats* stats,
                         grim_stream_t stream);
+
+/* Project autotuner for the batched solve: times the candidate launch
+ * configurations (workgroup size, compaction on/off, phase-1 budget,
+ * phase-2 grid) on this batch -- on a scratch copy of batch->prim, so the
+ * batch is left unchanged -- and writes the fastest into *opts (jacobian,
+ * lu_precision and fused are taken from *opts).  Synchronises the stream. */
+int grim_autotune_newton(const grim_params* p, const grim_zone_batch* b,
+                         grim_newton_opts* opts, grim_stream_t stream);
 
 /* Number of zones whose final_l2 exceeds tol (device reduction, synchronises
  * the stream). */

@@ -271,3 +271,94 @@ def test_analytic_newton_converges_to_fd_solution(capi, torch_cuda, oracle, mode
     # the conditioning of each zone times the tolerance
     assert np.median(rel_fd) < 1e-10
     assert np.percentile(rel_star, 99) < 10 * np.percentile(rel_fd_star, 99) + 1e-12
+
+
+# ------------------------------------------------ compaction, autotune, graphs
+@pytest.mark.parametrize("case", [("fd", False, 1), ("fd", True, 2), ("mixed", False, 1), ("mixed", False, 2),
+                                  ("fd_fused", True, 3)])
+def test_compaction_is_bit_identical_to_one_pass(capi, torch_cuda, case):
+    """Stopping after k1 updates and resuming the compacted list changes nothing."""
+    torch = torch_cuda
+    path, emhd, k1 = case
+    n = 65536
+    p, b, rhs, mask = make_problem(capi, torch, n, emhd)
+    jac = capi.JAC_FD if path.startswith("fd") else capi.JAC_ANALYTIC
+    lu = capi.LU_FP32 if path == "mixed" else capi.LU_FP64
+    fused = 1 if path in ("mixed", "fd_fused") else 0
+    out = {}
+    for compaction in (0, 1):
+        prim = b["guess"].clone()
+        it = torch.zeros(n, dtype=torch.int32, device="cuda")
+        l2 = torch.zeros(n, dtype=torch.float64, device="cuda")
+        ws = torch.empty(capi.workspace_bytes(n), dtype=torch.uint8, device="cuda")
+        o = capi.newton_opts(jac, lu, compaction=compaction, fused=fused, phase1_iters=k1, phase2_group=4,
+                             workspace=ws)
+        capi.batched_newton(p, capi.make_batch(prim, b["geom"], rhs, b["prim_old"], None, b["closure"],
+                                               residual_mask=mask), iterations=it, final_l2=l2, opts=o)
+        torch.cuda.synchronize()
+        out[compaction] = (host(prim), host(it), host(l2))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+
+
+def test_autotune_returns_a_working_configuration(capi, torch_cuda):
+    torch = torch_cuda
+    n = 1 << 20
+    p, b, rhs, mask = make_problem(capi, torch, n, False)
+    prim = b["guess"].clone()
+    batch = capi.make_batch(prim, b["geom"], rhs, residual_mask=mask)
+    o = capi.autotune_newton(p, batch, capi.newton_opts(capi.JAC_ANALYTIC, capi.LU_FP32, fused=1))
+    torch.cuda.synchronize()
+    assert torch.equal(prim, b["guess"])  # the batch is left untouched
+    assert o.block_size in (64, 128, 256)
+    print("autotuned: block %d compaction %d phase1 %d phase2 group %d" % (
+        o.block_size, o.compaction, o.phase1_iters, o.phase2_group))
+    ws = torch.empty(capi.workspace_bytes(n), dtype=torch.uint8, device="cuda")
+    o.workspace = ws.data_ptr()
+    o.workspace_bytes = ws.numel()
+    l2 = torch.empty(n, dtype=torch.float64, device="cuda")
+    capi.batched_newton(p, batch, final_l2=l2, opts=o)
+    assert capi.count_above(l2, p.nonlinearsolve_atol) == 0
+
+
+def test_graph_captured_step_matches_eager(capi, torch_cuda):
+    """restore guess + compacted mixed solve captured once, replayed 4 times"""
+    torch = torch_cuda
+    n = 1 << 18
+    p, b, rhs, mask = make_problem(capi, torch, n, False)
+    prim = b["guess"].clone()
+    l2 = torch.zeros(n, dtype=torch.float64, device="cuda")
+    it = torch.zeros(n, dtype=torch.int32, device="cuda")
+    ws = torch.empty(capi.workspace_bytes(n), dtype=torch.uint8, device="cuda")
+    batch = capi.make_batch(prim, b["geom"], rhs, residual_mask=mask)
+    o = capi.newton_opts(capi.JAC_ANALYTIC, capi.LU_FP32, compaction=1, fused=1, phase1_iters=2,
+                         phase2_group=4, workspace=ws)
+
+    def step():
+        prim[:5].copy_(b["guess"][:5])
+        capi.batched_newton(p, batch, iterations=it, final_l2=l2, opts=o)
+
+    step()
+    torch.cuda.synchronize()
+    eager = (prim.clone(), it.clone(), l2.clone())
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step()  # warm-up on the capture stream
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        step()
+    for r in range(4):
+        prim[:5].fill_(0.0)  # the step restores the fluid rows from the guess
+        it.fill_(-1)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(prim, eager[0]) and torch.equal(it, eager[1]) and torch.equal(l2, eager[2]), r
+        # eager work between replays
+        step()
+        torch.cuda.synchronize()
+        assert torch.equal(prim, eager[0])
