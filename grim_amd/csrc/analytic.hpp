@@ -1,5 +1,5 @@
-// Analytic Jacobian of the per-zone residual (ideal MHD) and the fast FP64
-// residual of the mixed-precision Newton path.
+// Analytic Jacobian of the per-zone residual (ideal MHD and EMHD) and the
+// fast FP64 residual of the mixed-precision Newton path.
 //
 // Identities used (normal-observer split, beta^i = alpha^2 g^0i,
 // beta_j = g_0j, gamma_ij = g_ij):
@@ -29,6 +29,19 @@
 // Floors: set() uses max(P, floor); the derivative follows the one-sided
 // forward difference of grim's FD rule (1 when P >= floor, else 0).
 //
+// EMHD adds to T^0_nu the terms E1_nu = q/|b| (u^0 b_nu + b^0 u_nu) and
+// E2_nu = -dP (b^0 b_nu / b^2 - (d_0nu + u^0 u_nu)/3), with q = qTilde K_q
+// and dP = dPTilde K_d depending on (rho, u) only:
+//   K_q = T sqrt(rho alpha_c c_s^2),  K_d = sqrt(T rho alpha_v c_s^2),
+//   c_s^2 = Gamma(Gamma-1) u/(rho + Gamma u),  T = (Gamma-1) u/rho,
+//   dln K_q/drho = -1/rho + (1/rho - 1/(rho+Gamma u))/2,
+//   dln K_q/du   =  1/u + rho/(2 u (rho+Gamma u)),
+//   dln K_d/drho = (-1/rho + 1/rho - 1/(rho+Gamma u))/2,
+//   dln K_d/du   = (1/u + rho/(u (rho+Gamma u)))/2
+// (the 1/rho and 1/u terms of T drop out where T sits on its floor).  The
+// guess element's own closure (tau, chi, nu) never enters the residual:
+// only the coefficient element's does, and it is fixed during the solve.
+//
 // The primal quantities are recomputed in the tangent's precision T (float
 // for the mixed path, double for the analytic FP64 path).
 #pragma once
@@ -38,11 +51,58 @@
 
 namespace grim {
 
+// Per-zone constant weights of the EMHD time-derivative sources.  With the
+// coefficient element c (elemOld on the half step, elemHalfStep on the full
+// step) fixed during the solve, computeTimeDerivSources (physics.cpp:219-318)
+// is affine in the new element's u_mu and T:
+//   dS_DP/du_mu = [-(g/tau_c) K_DP (-rho_c nu_c g^0mu + 3 rho_c nu_c b^0_c b^mu_c / b^2_c)
+//                  - 1/2 g dPTilde_c g^0mu] / dt
+//   dS_Q /du_nu = [-(g/tau_c) K_Q (-rho_c chi_c T_c b^nu_c u^0_c / |b|_c)
+//                  - 1/2 g qTilde_c g^0nu] / dt
+//   dS_Q /dT    = [-(g/tau_c) K_Q (-rho_c chi_c b^0_c / |b|_c)] / dt
+// with K_DP = sqrt(tau/rho/nu/T)_c and K_Q = sqrt(tau/rho/chi)_c / T_c (high-
+// order terms; 1 otherwise).  Rows Q and DP also carry the normalisation
+// (residual.cpp:61-93) and the mask.
+struct EmhdWeights {
+  double wQ[4], wQT, wDP[4];
+  double sQ, sDP;      // mask * normQ, mask * normDP
+  double gOverTau;     // g / tau_c
+};
+
+template <class L>
+__device__ __forceinline__ void emhdWeights(const ZoneGeom& G, const ZoneConst<L>& c,
+                                            const DevParams& prm, double dt, EmhdWeights& w) {
+  const double invDt = 1. / dt;
+  w.gOverTau = G.g / c.cTau;
+  w.sQ = c.mask * c.normQ;
+  w.sDP = c.mask * c.normDP;
+  if constexpr (L::VISC_ON) {
+    const double K = prm.highOrderVisc ? sqrt(c.cTau / c.cRho / c.cNu / c.cT) : 1.;
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) {
+      double a = -w.gOverTau * K *
+                 (-c.cRho * c.cNu * G.gcon0[mu] + 3. * c.cRho * c.cNu * c.cBCon[0] * c.cBCon[mu] / c.cBSqr);
+      if (prm.highOrderVisc) a -= 0.5 * G.g * c.cDPTilde * G.gcon0[mu];
+      w.wDP[mu] = a * invDt;
+    }
+  }
+  if constexpr (L::COND_ON) {
+    const double K = prm.highOrderCond ? sqrt(c.cTau / c.cRho / c.cChi) / c.cT : 1.;
+#pragma unroll
+    for (int nu = 0; nu < 4; nu++) {
+      double a = -w.gOverTau * K * (-c.cRho * c.cChi * c.cT * c.cBCon[nu] * c.cUCon0 / c.cBNorm);
+      if (prm.highOrderCond) a -= 0.5 * G.g * c.cQTilde * G.gcon0[nu];
+      w.wQ[nu] = a * invDt;
+    }
+    w.wQT = -w.gOverTau * K * (-c.cRho * c.cChi * c.cBCon[0] / c.cBNorm) * invDt;
+  }
+}
+
 template <class T, class L>
-__device__ __forceinline__ void analyticJacobianIdeal(const double* Pd, const ZoneGeom& Gd,
-                                                      const DevParams& prm, double scale,
-                                                      T (&A)[L::NF][L::NF]) {
-  static_assert(!L::EMHD, "ideal-MHD tangent");
+__device__ __forceinline__ void analyticJacobian(const double* Pd, const ZoneGeom& Gd,
+                                                 const DevParams& prm, double scale,
+                                                 const ZoneConst<L>& c, const EmhdWeights& ew,
+                                                 double dt, T (&A)[L::NF][L::NF]) {
   T gc[4][4];
 #pragma unroll
   for (int m = 0; m < 4; m++)
@@ -86,18 +146,74 @@ __device__ __forceinline__ void analyticJacobianIdeal(const double* Pd, const Zo
   const T s = (T)scale;
   const T gs = g * s;
 
+  // EMHD primal: q = qTilde K_q(rho,u), dP = dPTilde K_d(rho,u) (set(),
+  // physics.cpp:118-148) and the logarithmic derivatives of K_q, K_d
+  T qt = T(0), dpt = T(0), Kq = T(1), Kd = T(1), q = T(0), dP = T(0);
+  T dlnKq_rho = T(0), dlnKq_u = T(0), dlnKd_rho = T(0), dlnKd_u = T(0);
+  T ib = T(0), temp = T(0), sT = T(0);
+  if constexpr (L::EMHD) {
+    const T rawT = p / rho;
+    sT = (rawT > (T)prm.tFloor) ? T(1) : T(0);
+    temp = (rawT > (T)prm.tFloor) ? rawT : (T)prm.tFloor;
+    const T cs2 = gam * p / (rho + gam * u);
+    const T invRho = T(1) / rho;
+    const T invRG = T(1) / (rho + gam * u);
+    const T invU = T(1) / u;
+    ib = T(1) / sqrt(bsq);
+    if constexpr (L::COND_ON) {
+      const T aC = (T)((prm.closure == GRIM_CLOSURE_PER_ZONE) ? c.alphaC : prm.condAlpha);
+      qt = (T)Pd[L::Q];
+      if (prm.highOrderCond) {
+        Kq = temp * sqrt(rho * aC * cs2);
+        dlnKq_rho = -sT * invRho + T(0.5) * (invRho - invRG);
+        dlnKq_u = sT * invU + T(0.5) * rho * invU * invRG;
+      }
+      q = qt * Kq;
+    }
+    if constexpr (L::VISC_ON) {
+      const T aV = (T)((prm.closure == GRIM_CLOSURE_PER_ZONE) ? c.alphaV : prm.viscAlpha);
+      dpt = (T)Pd[L::DP];
+      if (prm.highOrderVisc) {
+        Kd = sqrt(temp * rho * aV * cs2);
+        dlnKd_rho = T(0.5) * (-sT * invRho + invRho - invRG);
+        dlnKd_u = T(0.5) * (sT * invU + rho * invU * invRG);
+      }
+      dP = dpt * Kd;
+    }
+  }
+  // shapes of the EMHD stress terms: E1_nu = q ib (u^0 b_nu + b^0 u_nu),
+  // E2_nu = -dP (b^0 b_nu / b^2 - (d_0nu + u^0 u_nu)/3)
+  T f1[4], f2[4];
+#pragma unroll
+  for (int nu = 0; nu < 4; nu++) {
+    f1[nu] = ib * (u0 * bCov[nu] + b0 * uCov[nu]);
+    f2[nu] = -(b0 * bCov[nu] / bsq - T(1. / 3.) * ((nu == 0 ? T(1) : T(0)) + u0 * uCov[nu]));
+  }
+
   // rho and u columns
   {
     const T a0 = sRho * u0;
+    T dT[4];
+#pragma unroll
+    for (int nu = 0; nu < 4; nu++) {
+      dT[nu] = a0 * uCov[nu];
+      if constexpr (L::EMHD) dT[nu] += sRho * (q * dlnKq_rho * f1[nu] + dP * dlnKd_rho * f2[nu]);
+    }
     A[RHO][RHO] = gs * a0;
-    A[UU][RHO] = gs * (a0 * uCov[0] + a0);
+    A[UU][RHO] = gs * (dT[0] + a0);
 #pragma unroll
-    for (int i = 0; i < 3; i++) A[U1 + i][RHO] = gs * (a0 * uCov[i + 1]);
+    for (int i = 0; i < 3; i++) A[U1 + i][RHO] = gs * dT[i + 1];
     const T a1 = gam * sU * u0;
-    A[RHO][UU] = T(0);
-    A[UU][UU] = gs * (a1 * uCov[0] + (gam - T(1)) * sU);
 #pragma unroll
-    for (int i = 0; i < 3; i++) A[U1 + i][UU] = gs * (a1 * uCov[i + 1]);
+    for (int nu = 0; nu < 4; nu++) {
+      dT[nu] = a1 * uCov[nu];
+      if constexpr (L::EMHD) dT[nu] += sU * (q * dlnKq_u * f1[nu] + dP * dlnKd_u * f2[nu]);
+    }
+    dT[0] = dT[0] + (gam - T(1)) * sU;
+    A[RHO][UU] = T(0);
+    A[UU][UU] = gs * dT[0];
+#pragma unroll
+    for (int i = 0; i < 3; i++) A[U1 + i][UU] = gs * dT[i + 1];
   }
   // velocity columns
   const T c2 = w * u0 - b0 * b0 * invU0;
@@ -115,12 +231,83 @@ __device__ __forceinline__ void analyticJacobianIdeal(const double* Pd, const Zo
 #pragma unroll
     for (int nu = 0; nu < 4; nu++) dT[nu] = c1 * uCov[nu] + c2 * gc[nu][i + 1] + c3 * bCov[nu];
     dT[0] = dT[0] - c2 * alpha * gi + T(0.5) * dbsq;
+    if constexpr (L::EMHD) {
+      // d(E1 + E2) with du_nu = g_nu,i - alpha gamma_i d_0nu and
+      // db_nu = (db^0 u_nu + b^0 du_nu - b_nu du^0) / u^0
+      const T dib = T(-0.5) * dbsq * ib * ib * ib;
+#pragma unroll
+      for (int nu = 0; nu < 4; nu++) {
+        const T duCov = gc[nu][i + 1] - (nu == 0 ? alpha * gi : T(0));
+        const T dbCov = (db0 * uCov[nu] + b0 * duCov - bCov[nu] * du0) * invU0;
+        const T dE1 = q * (dib * (u0 * bCov[nu] + b0 * uCov[nu]) +
+                           ib * (du0 * bCov[nu] + u0 * dbCov + db0 * uCov[nu] + b0 * duCov));
+        const T dE2 = -dP * ((db0 * bCov[nu] + b0 * dbCov) / bsq - b0 * bCov[nu] * dbsq / (bsq * bsq) -
+                             T(1. / 3.) * (du0 * uCov[nu] + u0 * duCov));
+        dT[nu] += dE1 + dE2;
+      }
+    }
     const T dUrho = gRho * du0;
     A[RHO][U1 + i] = s * dUrho;
     A[UU][U1 + i] = s * (g * dT[0] + dUrho);
 #pragma unroll
     for (int k = 0; k < 3; k++) A[U1 + k][U1 + i] = gs * dT[k + 1];
+    if constexpr (L::COND_ON) {
+      const T dS = (T)ew.wQ[0] * (gc[0][i + 1] - alpha * gi) + (T)ew.wQ[1] * gc[1][i + 1] +
+                   (T)ew.wQ[2] * gc[2][i + 1] + (T)ew.wQ[3] * gc[3][i + 1];
+      A[L::Q][U1 + i] = (T)ew.sQ * (g * qt * du0 / (T)dt + dS);
+    }
+    if constexpr (L::VISC_ON) {
+      const T dS = (T)ew.wDP[0] * (gc[0][i + 1] - alpha * gi) + (T)ew.wDP[1] * gc[1][i + 1] +
+                   (T)ew.wDP[2] * gc[2][i + 1] + (T)ew.wDP[3] * gc[3][i + 1];
+      A[L::DP][U1 + i] = (T)ew.sDP * (g * dpt * du0 / (T)dt + dS);
+    }
   }
+  if constexpr (L::EMHD) {
+    // EMHD columns of the ideal rows: dT^0_nu/dqTilde = K_q f1_nu,
+    // dT^0_nu/ddPTilde = K_d f2_nu
+    if constexpr (L::COND_ON) {
+      A[RHO][L::Q] = T(0);
+      A[UU][L::Q] = gs * Kq * f1[0];
+#pragma unroll
+      for (int k = 0; k < 3; k++) A[U1 + k][L::Q] = gs * Kq * f1[k + 1];
+    }
+    if constexpr (L::VISC_ON) {
+      A[RHO][L::DP] = T(0);
+      A[UU][L::DP] = gs * Kd * f2[0];
+#pragma unroll
+      for (int k = 0; k < 3; k++) A[U1 + k][L::DP] = gs * Kd * f2[k + 1];
+    }
+    // EMHD rows: U_x = g u^0 x, S_impl = g x / tau_c, S_td affine in (u_mu, T)
+    const T invDt = T(1) / (T)dt;
+    const T diag = g * u0 * invDt + T(0.5) * (T)ew.gOverTau;
+    if constexpr (L::COND_ON) {
+      const T sQ = (T)ew.sQ;
+      const T dTdrho = -sT * temp / rho * sRho;
+      const T dTdu = sT * temp / u * sU;
+      A[L::Q][RHO] = sQ * (T)ew.wQT * dTdrho;
+      A[L::Q][UU] = sQ * (T)ew.wQT * dTdu;
+      A[L::Q][L::Q] = sQ * diag;
+      if constexpr (L::VISC_ON) A[L::Q][L::DP] = T(0);
+    }
+    if constexpr (L::VISC_ON) {
+      const T sDP = (T)ew.sDP;
+      A[L::DP][RHO] = T(0);
+      A[L::DP][UU] = T(0);
+      A[L::DP][L::DP] = sDP * diag;
+      if constexpr (L::COND_ON) A[L::DP][L::Q] = T(0);
+    }
+  }
+}
+
+// Ideal-MHD entry kept for the kernels that carry no EMHD constants.
+template <class T, class L>
+__device__ __forceinline__ void analyticJacobianIdeal(const double* Pd, const ZoneGeom& Gd,
+                                                      const DevParams& prm, double scale,
+                                                      T (&A)[L::NF][L::NF]) {
+  static_assert(!L::EMHD, "ideal-MHD tangent");
+  ZoneConst<L> c;
+  EmhdWeights ew;
+  analyticJacobian<T, L>(Pd, Gd, prm, scale, c, ew, 1., A);
 }
 
 // Residual of the mixed-precision path (ideal MHD): FP64 throughout, the same

@@ -76,7 +76,8 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonGlobalUpdate(DevParams pr
   for (int v = 0; v < L::NF; v++) R[v] = Rbuf[v * n + z];
   int evals = 0;
   ZoneScale zs{};
-  newtonUpdate<L, MODE_FD>(P, R, l2buf[z], G, c, prm, b.dt_prime, zs, Rt, evals);
+  EmhdWeights ew;
+  newtonUpdate<L, MODE_FD>(P, R, l2buf[z], G, c, prm, b.dt_prime, zs, ew, Rt, evals);
 #pragma unroll
   for (int v = 0; v < L::NF; v++) b.prim[v * n + z] = P[v];
   evalsBuf[z] += evals;
@@ -170,11 +171,14 @@ __global__ __launch_bounds__(kDefaultBlock) void analyticJacobianKernel(DevParam
   const int64_t n = b.n;
   ZoneGeom G;
   loadGeom(b.geom, n, z, G);
-  const double mask = b.residual_mask ? b.residual_mask[z] : 1.;
+  ZoneConst<L> c;
+  loadZoneConst<L>(b, prm, G, z, c);
+  EmhdWeights ew;
+  if constexpr (L::EMHD) emhdWeights<L>(G, c, prm, b.dt_prime, ew);
   double P[L::DOF], A[L::NF][L::NF];
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) P[v] = b.prim[v * n + z];
-  analyticJacobianIdeal<double, L>(P, G, prm, mask / b.dt_prime, A);
+  analyticJacobian<double, L>(P, G, prm, c.mask / b.dt_prime, c, ew, b.dt_prime, A);
 #pragma unroll
   for (int i = 0; i < L::NF; i++)
 #pragma unroll
@@ -271,8 +275,6 @@ int grim_jacobian(const grim_params* p, const grim_zone_batch* b, int jacobian, 
   GRIM_REQUIRE(validBatch(p, b) && (b->n == 0 || J), "grim_jacobian: bad arguments");
   GRIM_REQUIRE(jacobian == GRIM_JAC_FD || jacobian == GRIM_JAC_ANALYTIC,
                "grim_jacobian: bad Jacobian kind");
-  GRIM_REQUIRE(jacobian == GRIM_JAC_FD || !(p->conduction || p->viscosity),
-               "grim_jacobian: the analytic Jacobian is built for ideal MHD only");
   if (b->n == 0) return GRIM_OK;
   if (int rc = checkDevice()) return rc;
   hipStream_t s = (hipStream_t)stream;
@@ -282,8 +284,7 @@ int grim_jacobian(const grim_params* p, const grim_zone_batch* b, int jacobian, 
     if (jacobian == GRIM_JAC_FD) {
       fdJacobianKernel<L><<<gridFor(bb.n, kDefaultBlock), kDefaultBlock, 0, s>>>(prm, bb, J);
     } else {
-      if constexpr (!L::EMHD)
-        analyticJacobianKernel<L><<<gridFor(bb.n, kDefaultBlock), kDefaultBlock, 0, s>>>(prm, bb, J);
+      analyticJacobianKernel<L><<<gridFor(bb.n, kDefaultBlock), kDefaultBlock, 0, s>>>(prm, bb, J);
     }
     return checkLaunch("grim_jacobian");
   });
@@ -344,8 +345,6 @@ int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
   if (o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP64) mode = MODE_ANALYTIC64;
   if (o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP32) mode = MODE_MIXED;
   GRIM_REQUIRE(mode >= 0, "grim_batched_newton: unsupported Jacobian / LU precision pair");
-  GRIM_REQUIRE(mode == MODE_FD || mode == MODE_FD_FAST || !(p->conduction || p->viscosity),
-               "grim_batched_newton: the analytic Jacobian is built for ideal MHD only");
   GRIM_REQUIRE(mode == MODE_FD || o.semantics == GRIM_NEWTON_PER_ZONE,
                "grim_batched_newton: lock-step semantics run grim's exact FD-FP64 algorithm only");
   GRIM_REQUIRE(o.semantics == GRIM_NEWTON_PER_ZONE || !o.compaction,
@@ -384,11 +383,10 @@ int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
     rc = launchNewtonFast(mode, *p, bb, st, cfg, s);
   } else {
     rc = dispatchLayout(*p, [&]<class L>() -> int {
-      if (mode == MODE_FD) {
+      if (mode == MODE_FD)
         launchSolve<L, MODE_FD>(prm, bb, st, cfg, s);
-      } else if constexpr (!L::EMHD) {
+      else
         launchSolve<L, MODE_ANALYTIC64>(prm, bb, st, cfg, s);
-      }
       return checkLaunch("grim_batched_newton");
     });
   }

@@ -16,14 +16,10 @@ int launchNewtonFast(int mode, const grim_params& p, const grim_zone_batch& b,
                      const grim_newton_stats& st, const LaunchCfg& cfg, hipStream_t s) {
   const DevParams prm = makeDevParams(p);
   return dispatchLayout(p, [&]<class L>() -> int {
-    if (mode == MODE_FD_FAST) {
+    if (mode == MODE_FD_FAST)
       launchSolve<L, MODE_FD_FAST>(prm, b, st, cfg, s);
-    } else if constexpr (!L::EMHD) {
+    else
       launchSolve<L, MODE_MIXED>(prm, b, st, cfg, s);
-    } else {
-      setError("launchNewtonFast: the mixed path is built for ideal MHD only");
-      return GRIM_ERR_BAD_ARG;
-    }
     return checkLaunch("grim_batched_newton (fused)");
   });
 }

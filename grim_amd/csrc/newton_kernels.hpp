@@ -96,7 +96,7 @@ template <class L, int MODE>
 __device__ __forceinline__ bool newtonUpdate(double* P, const double* R, double l2,
                                              const ZoneGeom& G, const ZoneConst<L>& c,
                                              const DevParams& prm, double dt, const ZoneScale& zs,
-                                             double* Rt, int& evals) {
+                                             const EmhdWeights& ew, double* Rt, int& evals) {
   double d[L::NF];
   if constexpr (MODE == MODE_FD) {
     double A[L::NF][L::NF];
@@ -114,13 +114,13 @@ __device__ __forceinline__ bool newtonUpdate(double* P, const double* R, double 
     luSolve<double, L::NF>(A, d);
   } else if constexpr (MODE == MODE_ANALYTIC64) {
     double A[L::NF][L::NF];
-    analyticJacobianIdeal<double, L>(P, G, prm, c.mask / dt, A);
+    analyticJacobian<double, L>(P, G, prm, c.mask / dt, c, ew, dt, A);
 #pragma unroll
     for (int v = 0; v < L::NF; v++) d[v] = -R[v];
     luSolve<double, L::NF>(A, d);
   } else {
     float A[L::NF][L::NF], df[L::NF];
-    analyticJacobianIdeal<float, L>(P, G, prm, zs.scale, A);
+    analyticJacobian<float, L>(P, G, prm, c.mask / dt, c, ew, dt, A);
 #pragma unroll
     for (int v = 0; v < L::NF; v++) df[v] = (float)(-R[v]);
     equilibratedSolve<float, L::NF>(A, df, P);
@@ -176,6 +176,9 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
     zs.invAlpha = 1. / G.alpha;
     zs.scale = c.mask / dt;
   }
+  EmhdWeights ew;
+  if constexpr (L::EMHD && (MODE == MODE_ANALYTIC64 || MODE == MODE_MIXED))
+    emhdWeights<L>(G, c, prm, dt, ew);
   double P[L::DOF];
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) P[v] = b.prim[v * n + z];
@@ -202,7 +205,7 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
       break;
     }
     double Rt[L::NF];
-    haveR = newtonUpdate<L, MODE>(P, R, l2, G, c, prm, dt, zs, Rt, evals);
+    haveR = newtonUpdate<L, MODE>(P, R, l2, G, c, prm, dt, zs, ew, Rt, evals);
     iters++;
     if (haveR) {
 #pragma unroll

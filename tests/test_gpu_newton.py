@@ -220,57 +220,77 @@ def test_full_size_fd_fp64_converges(capi, torch_cuda):
 
 
 # ------------------------------------------------------- analytic Jacobian
-def test_analytic_jacobian_matches_dual_numbers(capi, torch_cuda, oracle):
+@pytest.mark.parametrize("emhd", [False, True])
+def test_analytic_jacobian_matches_dual_numbers(capi, torch_cuda, oracle, emhd):
     """Hand-written tangent vs the oracle's forward-mode dual numbers (exact)."""
     torch = torch_cuda
     n = 16384
-    p, b, rhs, mask = make_problem(capi, torch, n, False)
+    p, b, rhs, mask = make_problem(capi, torch, n, emhd)
     J = torch.empty((p.nf * p.nf, n), dtype=torch.float64, device="cuda")
-    batch = capi.make_batch(b["guess"], b["geom"], rhs, residual_mask=mask)
+    batch = capi.make_batch(b["guess"], b["geom"], rhs, b["prim_old"], None, b["closure"], residual_mask=mask)
     capi.jacobian(p, batch, capi.JAC_ANALYTIC, J)
     torch.cuda.synchronize()
     q = oracle_params(oracle, p)
-    ref = oracle.dual_jacobian(q, oracle.Batch(host(b["guess"]), host(b["geom"]), host(rhs),
-                                               residualMask=host(mask))).reshape(p.nf, p.nf, n)
+    ref = oracle.dual_jacobian(q, oracle.Batch(host(b["guess"]), host(b["geom"]), host(rhs), host(b["prim_old"]),
+                                               None, host(b["closure"]), None, 1.0,
+                                               host(mask))).reshape(p.nf, p.nf, n)
     got = host(J).reshape(p.nf, p.nf, n)
     colscale = np.abs(ref).max(axis=0, keepdims=True) + 1e-300
     err = np.abs(got - ref) / colscale
-    print("analytic vs dual: median %.2e  p99 %.2e  max %.2e" % (
-        np.median(err.max(axis=(0, 1))), np.percentile(err.max(axis=(0, 1)), 99), err.max()))
-    assert err.max() < 1e-9
+    worst = err.max(axis=(0, 1))
+    print("analytic vs dual (emhd=%s): median %.2e  p99 %.2e  max %.2e" % (
+        emhd, np.median(worst), np.percentile(worst, 99), err.max()))
+    if not emhd:
+        assert err.max() < 1e-9
+    else:
+        # near-floor zones (rho ~ 1e-8, u ~ 1e-11) lose digits to cancellation
+        # in either evaluation; everywhere else the two agree to rounding
+        assert np.median(worst) < 1e-12
+        assert np.percentile(worst, 99) < 1e-7
 
 
+@pytest.mark.parametrize("emhd", [False, True])
 @pytest.mark.parametrize("mode", ["analytic64", "mixed"])
-def test_analytic_newton_converges_to_fd_solution(capi, torch_cuda, oracle, mode):
+def test_analytic_newton_converges_to_fd_solution(capi, torch_cuda, oracle, mode, emhd):
     torch = torch_cuda
     n = 65536
-    p, b, rhs, mask = make_problem(capi, torch, n, False)
+    p, b, rhs, mask = make_problem(capi, torch, n, emhd)
     results = {}
     for name, jac, lu in (("fd", capi.JAC_FD, capi.LU_FP64),
                           (mode, capi.JAC_ANALYTIC, capi.LU_FP64 if mode == "analytic64" else capi.LU_FP32)):
         prim = b["guess"].clone()
         it = torch.zeros(n, dtype=torch.int32, device="cuda")
         l2 = torch.zeros(n, dtype=torch.float64, device="cuda")
-        capi.batched_newton(p, capi.make_batch(prim, b["geom"], rhs, residual_mask=mask), jac, lu,
-                            iterations=it, final_l2=l2)
+        capi.batched_newton(p, capi.make_batch(prim, b["geom"], rhs, b["prim_old"], None, b["closure"],
+                                               residual_mask=mask), jac, lu, iterations=it, final_l2=l2,
+                            fused=1 if lu == capi.LU_FP32 else 0)
         torch.cuda.synchronize()
         results[name] = (host(prim), host(it), host(l2))
     fd, an = results["fd"], results[mode]
     star = host(b["prim_star"])
-    assert np.all(an[2] <= p.nonlinearsolve_atol)
-    rel_fd = (np.abs(an[0][:5] - fd[0][:5]) / np.abs(fd[0][:5])).max(axis=0)
-    rel_star = (np.abs(an[0][:5] - star[:5]) / np.abs(star[:5])).max(axis=0)
-    rel_fd_star = (np.abs(fd[0][:5] - star[:5]) / np.abs(star[:5])).max(axis=0)
-    print("%s: iterations %s (fd %s); |P-P_fd|/|P_fd| max %.2e p99 %.2e median %.2e; "
-          "|P-P*|/|P*| p99 %.2e (fd %.2e)" % (
-              mode, np.bincount(an[1]).tolist(), np.bincount(fd[1]).tolist(), rel_fd.max(),
-              np.percentile(rel_fd, 99), np.median(rel_fd), np.percentile(rel_star, 99),
-              np.percentile(rel_fd_star, 99)))
-    assert np.all(an[0][5:] == host(b["guess"])[5:])
-    # both satisfy the relative-residual test; their distance is bounded by
-    # the conditioning of each zone times the tolerance
-    assert np.median(rel_fd) < 1e-10
-    assert np.percentile(rel_star, 99) < 10 * np.percentile(rel_fd_star, 99) + 1e-12
+    nf = p.nf
+    ok_fd = fd[2] <= p.nonlinearsolve_atol
+    ok = an[2] <= p.nonlinearsolve_atol
+    rel_fd = (np.abs(an[0][:nf] - fd[0][:nf]) / np.maximum(np.abs(fd[0][:nf]), 1e-300)).max(axis=0)
+    rel_star = (np.abs(an[0][:nf] - star[:nf]) / np.maximum(np.abs(star[:nf]), 1e-300)).max(axis=0)
+    rel_fd_star = (np.abs(fd[0][:nf] - star[:nf]) / np.maximum(np.abs(star[:nf]), 1e-300)).max(axis=0)
+    print("%s emhd=%s: converged %.4f (fd %.4f); iterations %s (fd %s); |P-P_fd|/|P_fd| p99 %.2e median %.2e; "
+          "|P-P*|/|P*| p99 %.2e median %.2e (fd %.2e, %.2e)" % (
+              mode, emhd, ok.mean(), ok_fd.mean(), np.bincount(an[1]).tolist(), np.bincount(fd[1]).tolist(),
+              np.percentile(rel_fd[ok & ok_fd], 99), np.median(rel_fd[ok & ok_fd]),
+              np.percentile(rel_star[ok], 99), np.median(rel_star[ok]),
+              np.percentile(rel_fd_star[ok_fd], 99), np.median(rel_fd_star[ok_fd])))
+    assert np.all(an[0][nf:] == host(b["guess"])[nf:])
+    if not emhd:
+        assert np.all(ok)
+        # both satisfy the relative-residual test; their distance is bounded
+        # by the conditioning of each zone times the tolerance
+        assert np.median(rel_fd) < 1e-10
+        assert np.percentile(rel_star, 99) < 10 * np.percentile(rel_fd_star, 99) + 1e-12
+    else:
+        # the exact Jacobian converges at least where grim's FD Jacobian does
+        assert ok.mean() >= ok_fd.mean()
+        assert np.median(rel_star[ok]) < 1e-9
 
 
 # ------------------------------------------------ compaction, autotune, graphs
