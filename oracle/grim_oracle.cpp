@@ -213,6 +213,23 @@ void grim_oracle_conserved(const grim_oracle_params* p, int64_t n,
   }
 }
 
+void grim_oracle_fluxes(const grim_oracle_params* p, int64_t n, const double* prim,
+                        const double* geom, const double* closure, const double* radius,
+                        int dir, double* flux) {
+  const Vars vars(*p);
+#pragma omp parallel for schedule(static)
+  for (int64_t z = 0; z < n; z++) {
+    ZoneGeom<double> G;
+    loadGeom(geom, radius, n, z, &G);
+    double P[GRIM_ORACLE_MAX_VARS], F[GRIM_ORACLE_MAX_VARS];
+    for (int v = 0; v < vars.dof; v++) P[v] = prim[v * n + z];
+    FluidElement<double> e;
+    e.set(P, G, *p, loadClosure(closure, n, z));
+    e.computeFluxes(dir, F, *p);
+    for (int v = 0; v < vars.dof; v++) flux[v * n + z] = F[v];
+  }
+}
+
 void grim_oracle_element(const grim_oracle_params* p, int64_t n, const double* prim,
                          const double* geom, const double* closure, const double* radius,
                          double* out) {
@@ -397,6 +414,10 @@ int64_t grim_oracle_newton(const grim_oracle_params* p,
     }
 #pragma omp parallel for schedule(static)
     for (int64_t z = 0; z < n; z++) {
+      // Zones outside the bulk (mask 0: zero residual, zero Jacobian) take
+      // no update; grim solves their singular systems and leaves non-finite
+      // values there until the boundary conditions overwrite them.
+      if (b->residualMask && b->residualMask[z] == 0.) continue;
       ZoneState<double> s;
       s.load(*p, *b, z);
       double r[GRIM_ORACLE_MAX_VARS];

@@ -99,6 +99,11 @@ void grim_oracle_conserved(const grim_oracle_params* p, int64_t n,
                            const double* closure, const double* radius,
                            double* cons);
 
+/* computeFluxes(dir) for each zone (dof planes); dir 0 gives U. */
+void grim_oracle_fluxes(const grim_oracle_params* p, int64_t n, const double* prim,
+                        const double* geom, const double* closure, const double* radius,
+                        int dir, double* flux);
+
 /* fluidElement::set internals of every zone (GRIM_ORACLE_ELEMENT_PLANES
  * planes): uCon[4], uCov[4], bCon[4], bCov[4], bSqr, q, deltaP, pressure,
  * rho, u, tau, chi, nu, TUpDown[4][4] (row-major). */
@@ -151,6 +156,129 @@ int grim_oracle_lu_solve(int n, double* A, double* b);
 void grim_oracle_count_ops(const grim_oracle_params* p,
                            const grim_oracle_batch* b, int64_t* residualOps,
                            int64_t* luOps);
+
+/* ------------------------------------------------------------------------
+ * Full time step (grim_step.cpp).
+ *
+ * Grid arrays are grim's `grid::vars` planes: N1T*N2T*N3T doubles per
+ * variable, index i + N1T*(j + N2T*k), with numGhost ghost zones on each
+ * side of every active dimension (none in X3 when dim < 3, none in X2 when
+ * dim < 2).
+ *
+ * Geometry at one location: GRIM_ORACLE_LOC_PLANES planes
+ *   0..9   gCov 00 01 02 03 11 12 13 22 23 33
+ *   10..13 gCon 00 01 02 03
+ *   14     alpha
+ *   15     g
+ *   16..21 gCon 11 12 13 22 23 33
+ *   22     r (xCoords[X1])
+ * (planes 0..15 are the batched-zone layout above).  The connection at the
+ * zone centre has 64 planes, Gamma^l_{k n} at plane l*16 + k*4 + n.
+ * ------------------------------------------------------------------------ */
+#define GRIM_ORACLE_LOC_PLANES 23
+
+#define GRIM_ORACLE_METRIC_MINKOWSKI 0
+#define GRIM_ORACLE_METRIC_MKS 1
+#define GRIM_ORACLE_BC_PERIODIC 0
+#define GRIM_ORACLE_BC_OUTFLOW 1
+#define GRIM_ORACLE_BC_MIRROR 2
+#define GRIM_ORACLE_RECON_MINMOD 0
+#define GRIM_ORACLE_RECON_WENO5 1
+#define GRIM_ORACLE_RECON_PPM 2
+#define GRIM_ORACLE_RIEMANN_HLL 0
+#define GRIM_ORACLE_RIEMANN_LLF 1
+
+typedef struct grim_oracle_grid {
+  int N1, N2, N3, dim, numGhost;
+  double X1Start, X1End, X2Start, X2End, X3Start, X3End;
+  int metric;
+  double blackHoleSpin, hSlope;
+  int DerefineThetaHorizon, DoCylindrify;
+  double X1cyl, X2cyl;
+  /* boundaries (src/params.hpp:48-54 names) */
+  int boundaryLeft, boundaryRight, boundaryBottom, boundaryTop,
+      boundaryBack, boundaryFront;
+  int reconstruction, riemannSolver;
+  double slopeLimTheta;
+  double CourantFactor, maxDtIncrement;
+} grim_oracle_grid;
+
+/* Geometry of every zone of the grid.  center, faceX1 (LEFT), faceX2
+ * (BOTTOM) hold GRIM_ORACLE_LOC_PLANES planes each; gamma 64 planes (may be
+ * NULL).  X3 faces use the centre geometry (grim passes geomCenter for X3,
+ * src/timestepper/fvmfluxes.cpp:147-151). */
+typedef struct grim_oracle_geometry {
+  double* center;
+  double* faceX1;
+  double* faceX2;
+  double* gamma;
+} grim_oracle_geometry;
+
+int64_t grim_oracle_grid_zones(const grim_oracle_grid* c);
+/* xCoords (r, theta, phi for MKS) at the CENTER of every zone: 3 planes. */
+void grim_oracle_grid_xcoords(const grim_oracle_grid* c, double* x3);
+void grim_oracle_build_geometry(const grim_oracle_grid* c, grim_oracle_geometry* gm);
+
+/* grid::communicate for one rank (periodic ghost fill) followed by
+ * boundaries::applyBoundaryConditions, in place on nvars planes. */
+void grim_oracle_apply_boundaries(const grim_oracle_grid* c, int nvars, double* prim);
+
+/* reconstruction::reconstruct along dir (0,1,2) for nvars planes: left =
+ * value at the i-1/2 face of zone i, right = at i+1/2.  Written for the
+ * zones whose stencil lies inside the grid; other entries untouched. */
+void grim_oracle_reconstruct(const grim_oracle_grid* c, int nvars, const double* prim,
+                             int dir, double* left, double* right);
+
+/* Reconstruction + riemannSolver::solve: flux planes (dof) at the lower
+ * face of every zone along dir, for faces whose stencil lies in the grid. */
+void grim_oracle_riemann_flux(const grim_oracle_grid* c, const grim_oracle_params* p,
+                              const grim_oracle_geometry* gm, const double* prim,
+                              int dir, double* flux);
+
+/* computeDivOfFluxes: fluxes along each active direction, flux-CT for
+ * dim >= 2, divergence (dof planes, bulk zones only).  fluxes (dim*dof
+ * planes) may be NULL. */
+void grim_oracle_div_fluxes(const grim_oracle_grid* c, const grim_oracle_params* p,
+                            const grim_oracle_geometry* gm, const double* prim,
+                            double* divF, double* fluxes);
+
+/* fluidElement::computeExplicitSources of the element set at prim (dof
+ * planes, bulk zones only). */
+void grim_oracle_explicit_sources(const grim_oracle_grid* c, const grim_oracle_params* p,
+                                  const grim_oracle_geometry* gm, const double* prim,
+                                  double* sources);
+
+/* timeStepper::computeDt: returns min(Courant/maxInvDt, maxDtIncrement*dt)
+ * with maxInvDt the maximum over every zone of the element set at prim. */
+double grim_oracle_compute_dt(const grim_oracle_grid* c, const grim_oracle_params* p,
+                              const grim_oracle_geometry* gm, const double* prim,
+                              double dt);
+
+typedef struct grim_oracle_step_stats {
+  double dt;
+  int64_t unconverged[2];   /* bulk zones above atol after each stage */
+  int32_t maxIterations[2]; /* per stage */
+} grim_oracle_step_stats;
+
+/* timeStepper::timeStep (src/timestepper/timestep.cpp:3-334): computeDt from
+ * primDt (the state elemOld was last set from), then the half and full
+ * stages.  primOld (dof planes) is advanced in place, primDt is replaced by
+ * the boundary-filled primOld of this step, *dt and *time advance.
+ * semantics: GRIM_ORACLE_NEWTON_PER_ZONE or GLOBAL. */
+void grim_oracle_time_step(const grim_oracle_grid* c, const grim_oracle_params* p,
+                           const grim_oracle_geometry* gm, double* primOld,
+                           double* primDt, double* dt, double* time, int semantics,
+                           grim_oracle_step_stats* stats);
+
+/* The two stages as separate calls, for stage-level parity tests.
+ * stage 0 (half): boundaries on primOld, consOld, sources, divF, induction,
+ * Newton; writes primHalf (dof planes) and consOld (dof planes).
+ * stage 1 (full): boundaries on primHalf, sources, divF, induction, Newton
+ * from the guess primHalf; writes primNew. */
+void grim_oracle_stage(const grim_oracle_grid* c, const grim_oracle_params* p,
+                       const grim_oracle_geometry* gm, int stage, double dt,
+                       double* primOld, double* primHalf, double* consOld,
+                       double* primNew, int semantics, grim_oracle_step_stats* stats);
 
 #ifdef __cplusplus
 }

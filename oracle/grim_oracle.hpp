@@ -296,6 +296,66 @@ struct FluidElement {
     if (p.viscosity) sources[vars.DP] = geom->g * (deltaPTilde) / tauDamp;
     if (p.conduction) sources[vars.Q] = geom->g * (qTilde) / tauDamp;
   }
+
+  // fluidElement::computeMinMaxCharSpeeds, src/physics/riemannsolver.cpp:54-200.
+  // dir = 0, 1, 2 for X1, X2, X3.  Needs the full gCon.
+  void computeMinMaxCharSpeeds(int dir, T* minSpeed, T* maxSpeed,
+                               const grim_oracle_params& p) const {
+    using std::sqrt;
+    T cAlvenSqr = bSqr / (rho + p.adiabaticIndex * u + bSqr);
+    T csSqr = soundSpeed * soundSpeed;
+    T cVisSqr = T(0.);
+    if (p.viscosity) cVisSqr = 4. / 3. / (rho + p.adiabaticIndex * u) * rho * nu_emhd / tau;
+    T cConSqr = T(0.);
+    if (p.conduction) cConSqr = (p.adiabaticIndex - 1.) * chi_emhd / tau;
+    cConSqr = 0.5 * (csSqr + cConSqr + sqrt(csSqr * csSqr + cConSqr * cConSqr));
+    csSqr = cConSqr + cVisSqr;
+    csSqr = csSqr + cAlvenSqr - csSqr * cAlvenSqr;
+    {
+      const double condition = oval(csSqr) > 1. ? 1. : 0.;
+      csSqr = csSqr * (1. - condition) + condition;
+    }
+    const int sdir = dir + 1;
+    T ACov[4], ACon[4], BCov[4], BCon[4];
+    for (int mu = 0; mu < 4; mu++) {
+      ACov[mu] = T(0.);
+      BCov[mu] = T(0.);
+    }
+    ACov[sdir] = T(1.);
+    BCov[0] = T(1.);
+    for (int mu = 0; mu < 4; mu++) {
+      ACon[mu] = T(0.);
+      BCon[mu] = T(0.);
+      for (int nu = 0; nu < 4; nu++) {
+        ACon[mu] += geom->gCon[mu][nu] * ACov[nu];
+        BCon[mu] += geom->gCon[mu][nu] * BCov[nu];
+      }
+    }
+    T ASqr = T(0.), BSqr = T(0.), ADotU = T(0.), BDotU = T(0.), ADotB = T(0.);
+    for (int mu = 0; mu < 4; mu++) {
+      ASqr += ACov[mu] * ACon[mu];
+      BSqr += BCov[mu] * BCon[mu];
+      ADotU += ACov[mu] * uCon[mu];
+      BDotU += BCov[mu] * uCon[mu];
+      ADotB += ACov[mu] * BCon[mu];
+    }
+    T A = (BDotU * BDotU) - (BSqr + BDotU * BDotU) * csSqr;
+    T B = 2. * (ADotU * BDotU - (ADotB + ADotU * BDotU) * csSqr);
+    T C = ADotU * ADotU - (ASqr + ADotU * ADotU) * csSqr;
+    T discr = sqrt(amax(B * B - 4. * A * C, 1.e-16));
+    T mn = -(-B + discr) / 2. / A;
+    T mx = -(-B - discr) / 2. / A;
+    {
+      const double condition = oval(mn) > -1.e-15 ? 1. : 0.;
+      mn = mn * (1 - condition) - 1.e-15 * condition;
+    }
+    {
+      const double condition = oval(mx) < 1.e-15 ? 1. : 0.;
+      mx = mx * (1 - condition) + 1.e-15 * condition;
+    }
+    *minSpeed = mn;
+    *maxSpeed = mx;
+  }
 };
 
 // ---------------------------------------------------------------------------

@@ -76,6 +76,65 @@ class _Batch(C.Structure):
     ]
 
 
+METRIC_MINKOWSKI = 0
+METRIC_MKS = 1
+BC_PERIODIC = 0
+BC_OUTFLOW = 1
+BC_MIRROR = 2
+RECON_MINMOD = 0
+RECON_WENO5 = 1
+RECON_PPM = 2
+RIEMANN_HLL = 0
+RIEMANN_LLF = 1
+LOC_PLANES = 23
+
+
+class Grid(C.Structure):
+    """Mirror of ``grim_oracle_grid`` (grid, geometry and scheme switches)."""
+
+    _fields_ = [
+        ("N1", C.c_int), ("N2", C.c_int), ("N3", C.c_int), ("dim", C.c_int), ("numGhost", C.c_int),
+        ("X1Start", C.c_double), ("X1End", C.c_double), ("X2Start", C.c_double),
+        ("X2End", C.c_double), ("X3Start", C.c_double), ("X3End", C.c_double),
+        ("metric", C.c_int), ("blackHoleSpin", C.c_double), ("hSlope", C.c_double),
+        ("DerefineThetaHorizon", C.c_int), ("DoCylindrify", C.c_int),
+        ("X1cyl", C.c_double), ("X2cyl", C.c_double),
+        ("boundaryLeft", C.c_int), ("boundaryRight", C.c_int), ("boundaryBottom", C.c_int),
+        ("boundaryTop", C.c_int), ("boundaryBack", C.c_int), ("boundaryFront", C.c_int),
+        ("reconstruction", C.c_int), ("riemannSolver", C.c_int), ("slopeLimTheta", C.c_double),
+        ("CourantFactor", C.c_double), ("maxDtIncrement", C.c_double),
+    ]
+
+    @property
+    def shape(self):
+        """(N3T, N2T, N1T): numpy shape of one plane (X1 fastest)."""
+        g = self.numGhost
+        return (self.N3 + (2 * g if self.dim > 2 else 0), self.N2 + (2 * g if self.dim > 1 else 0),
+                self.N1 + 2 * g)
+
+    @property
+    def n(self) -> int:
+        a, b, c = self.shape
+        return a * b * c
+
+    def bulk(self):
+        """Index expression selecting the bulk of a (…, N3T, N2T, N1T) array."""
+        g = self.numGhost
+        sl = [slice(g, g + self.N1)]
+        sl.insert(0, slice(g, g + self.N2) if self.dim > 1 else slice(None))
+        sl.insert(0, slice(g, g + self.N3) if self.dim > 2 else slice(None))
+        return (Ellipsis, *sl)
+
+
+class _Geometry(C.Structure):
+    _fields_ = [("center", C.c_void_p), ("faceX1", C.c_void_p), ("faceX2", C.c_void_p),
+                ("gamma", C.c_void_p)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("dt", C.c_double), ("unconverged", C.c_int64 * 2), ("maxIterations", C.c_int32 * 2)]
+
+
 def build() -> str:
     """Compile the oracle with its Makefile (g++, -ffp-contract=off)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
@@ -101,6 +160,7 @@ def lib():
         L.grim_oracle_residual.argtypes = [P, B, vp]
         L.grim_oracle_known_root_rhs.argtypes = [P, B, vp]
         L.grim_oracle_element.argtypes = [P, i64, vp, vp, vp, vp, vp]
+        L.grim_oracle_fluxes.argtypes = [P, i64, vp, vp, vp, vp, C.c_int, vp]
         L.grim_oracle_residual_pieces.argtypes = [P, B, vp, vp, vp]
         L.grim_oracle_fd_jacobian.argtypes = [P, B, vp]
         L.grim_oracle_dual_jacobian.argtypes = [P, B, vp]
@@ -109,6 +169,21 @@ def lib():
         L.grim_oracle_lu_solve.argtypes = [C.c_int, vp, vp]
         L.grim_oracle_lu_solve.restype = C.c_int
         L.grim_oracle_count_ops.argtypes = [P, B, vp, vp]
+        Gp = C.POINTER(Grid)
+        Gm = C.POINTER(_Geometry)
+        L.grim_oracle_grid_zones.argtypes = [Gp]
+        L.grim_oracle_grid_zones.restype = i64
+        L.grim_oracle_grid_xcoords.argtypes = [Gp, vp]
+        L.grim_oracle_build_geometry.argtypes = [Gp, Gm]
+        L.grim_oracle_apply_boundaries.argtypes = [Gp, C.c_int, vp]
+        L.grim_oracle_reconstruct.argtypes = [Gp, C.c_int, vp, C.c_int, vp, vp]
+        L.grim_oracle_riemann_flux.argtypes = [Gp, P, Gm, vp, C.c_int, vp]
+        L.grim_oracle_div_fluxes.argtypes = [Gp, P, Gm, vp, vp, vp]
+        L.grim_oracle_explicit_sources.argtypes = [Gp, P, Gm, vp, vp]
+        L.grim_oracle_compute_dt.argtypes = [Gp, P, Gm, vp, d]
+        L.grim_oracle_compute_dt.restype = d
+        L.grim_oracle_stage.argtypes = [Gp, P, Gm, C.c_int, d, vp, vp, vp, vp, C.c_int, C.POINTER(StepStats)]
+        L.grim_oracle_time_step.argtypes = [Gp, P, Gm, vp, vp, vp, vp, C.c_int, C.POINTER(StepStats)]
         _lib = L
     return _lib
 
@@ -199,6 +274,16 @@ ELEMENT_FIELDS = (["uCon%d" % i for i in range(4)] + ["uCov%d" % i for i in rang
                   + ["T%d%d" % (m, k) for m in range(4) for k in range(4)])
 
 
+def fluxes(p: Params, prim, geom, d: int, closure=None, radius=None):
+    """computeFluxes(d) per zone (d = 0: conserved variables)."""
+    prim, geom = _f64(prim), _f64(geom)
+    n = prim.shape[-1]
+    out = np.zeros((p.dof, n))
+    lib().grim_oracle_fluxes(C.byref(p), n, _ptr(prim), _ptr(geom), _ptr(_f64(closure)), _ptr(_f64(radius)), d,
+                             _ptr(out))
+    return out
+
+
 def element(p: Params, prim, geom, closure=None, radius=None):
     """fluidElement::set internals, dict name -> (n,) array (see ELEMENT_FIELDS)."""
     prim, geom, closure, radius = map(_f64, (prim, geom, closure, radius))
@@ -271,3 +356,112 @@ def count_ops(p: Params, b: Batch):
     cb = b._c()
     lib().grim_oracle_count_ops(C.byref(p), C.byref(cb), _ptr(r), _ptr(l))
     return r, l
+
+
+# ---------------------------------------------------------------------------
+# Full time step
+# ---------------------------------------------------------------------------
+def grid(**kw) -> Grid:
+    """A Grid with grim's defaults (Minkowski, periodic, MINMOD+HLL) overridden by kw."""
+    g = Grid(N1=64, N2=64, N3=1, dim=2, numGhost=3, X1Start=0., X1End=1., X2Start=0., X2End=1.,
+             X3Start=0., X3End=1., metric=METRIC_MINKOWSKI, blackHoleSpin=0.9375, hSlope=0.3,
+             DerefineThetaHorizon=0, DoCylindrify=0, X1cyl=0., X2cyl=0.,
+             boundaryLeft=BC_PERIODIC, boundaryRight=BC_PERIODIC, boundaryBottom=BC_PERIODIC,
+             boundaryTop=BC_PERIODIC, boundaryBack=BC_PERIODIC, boundaryFront=BC_PERIODIC,
+             reconstruction=RECON_MINMOD, riemannSolver=RIEMANN_HLL, slopeLimTheta=2.,
+             CourantFactor=0.9, maxDtIncrement=1.3)
+    for k, v in kw.items():
+        setattr(g, k, v)
+    return g
+
+
+class Geometry:
+    """Geometry arrays of a grid: centre/faceX1/faceX2 (23 planes), gamma (64)."""
+
+    def __init__(self, g: Grid, center=None, faceX1=None, faceX2=None, gamma=None, build=True):
+        n = g.n
+        self.center = _f64(center) if center is not None else np.zeros((LOC_PLANES, n))
+        self.faceX1 = _f64(faceX1) if faceX1 is not None else np.zeros((LOC_PLANES, n))
+        self.faceX2 = _f64(faceX2) if faceX2 is not None else np.zeros((LOC_PLANES, n))
+        self.gamma = _f64(gamma) if gamma is not None else np.zeros((64, n))
+        if build and center is None:
+            lib().grim_oracle_build_geometry(C.byref(g), C.byref(self._c()))
+
+    def _c(self):
+        return _Geometry(_ptr(self.center), _ptr(self.faceX1), _ptr(self.faceX2), _ptr(self.gamma))
+
+
+def xcoords(g: Grid):
+    out = np.zeros((3, g.n))
+    lib().grim_oracle_grid_xcoords(C.byref(g), _ptr(out))
+    return out
+
+
+def apply_boundaries(g: Grid, prim):
+    prim = _f64(prim).copy()
+    lib().grim_oracle_apply_boundaries(C.byref(g), prim.shape[0], _ptr(prim))
+    return prim
+
+
+def reconstruct(g: Grid, prim, d: int):
+    prim = _f64(prim)
+    left = np.full_like(prim, np.nan)
+    right = np.full_like(prim, np.nan)
+    lib().grim_oracle_reconstruct(C.byref(g), prim.shape[0], _ptr(prim), d, _ptr(left), _ptr(right))
+    return left, right
+
+
+def riemann_flux(g: Grid, p: Params, gm: Geometry, prim, d: int):
+    prim = _f64(prim)
+    out = np.full((p.dof, g.n), np.nan)
+    lib().grim_oracle_riemann_flux(C.byref(g), C.byref(p), C.byref(gm._c()), _ptr(prim), d, _ptr(out))
+    return out
+
+
+def div_fluxes(g: Grid, p: Params, gm: Geometry, prim, want_fluxes=False):
+    prim = _f64(prim)
+    div = np.full((p.dof, g.n), np.nan)
+    fl = np.zeros((g.dim * p.dof, g.n)) if want_fluxes else None
+    lib().grim_oracle_div_fluxes(C.byref(g), C.byref(p), C.byref(gm._c()), _ptr(prim), _ptr(div), _ptr(fl))
+    return (div, fl) if want_fluxes else div
+
+
+def explicit_sources(g: Grid, p: Params, gm: Geometry, prim):
+    prim = _f64(prim)
+    out = np.full((p.dof, g.n), np.nan)
+    lib().grim_oracle_explicit_sources(C.byref(g), C.byref(p), C.byref(gm._c()), _ptr(prim), _ptr(out))
+    return out
+
+
+def compute_dt(g: Grid, p: Params, gm: Geometry, prim, dt: float) -> float:
+    return float(lib().grim_oracle_compute_dt(C.byref(g), C.byref(p), C.byref(gm._c()), _ptr(_f64(prim)), dt))
+
+
+def stage(g: Grid, p: Params, gm: Geometry, st: int, dt: float, prim_old, prim_half=None, cons_old=None,
+          semantics=NEWTON_PER_ZONE):
+    """One stage; returns (prim_new, cons_old, stats); prim_old/prim_half get their boundaries applied."""
+    cons_old = np.zeros((p.dof, g.n)) if cons_old is None else cons_old
+    new = np.zeros((p.dof, g.n))
+    st_ = StepStats()
+    lib().grim_oracle_stage(C.byref(g), C.byref(p), C.byref(gm._c()), st, dt, _ptr(prim_old), _ptr(prim_half),
+                            _ptr(cons_old), _ptr(new), semantics, C.byref(st_))
+    return new, cons_old, st_
+
+
+class Simulation:
+    """timeStepper state: primOld, the state elemOld was set from, dt, time."""
+
+    def __init__(self, g: Grid, p: Params, prim, dt: float, gm: Geometry | None = None, time: float = 0.):
+        self.g, self.p = g, p
+        self.gm = gm if gm is not None else Geometry(g)
+        self.prim = _f64(prim).copy()
+        self.prim_dt = self.prim.copy()
+        self.dt = C.c_double(dt)
+        self.time = C.c_double(time)
+
+    def step(self, semantics=NEWTON_PER_ZONE) -> StepStats:
+        st = StepStats()
+        lib().grim_oracle_time_step(C.byref(self.g), C.byref(self.p), C.byref(self.gm._c()), _ptr(self.prim),
+                                    _ptr(self.prim_dt), C.byref(self.dt), C.byref(self.time), semantics,
+                                    C.byref(st))
+        return st
