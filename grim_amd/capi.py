@@ -112,10 +112,58 @@ class NewtonStats(C.Structure):
     ]
 
 
+METRIC_MINKOWSKI = 0
+METRIC_MKS = 1
+BC_PERIODIC = 0
+BC_OUTFLOW = 1
+BC_MIRROR = 2
+RECON_MINMOD = 0
+RECON_WENO5 = 1
+RECON_PPM = 2
+RIEMANN_HLL = 0
+RIEMANN_LLF = 1
+LOC_PLANES = 23
+
+
+class GridConfig(C.Structure):
+    """grim_grid_config: grid, geometry, boundary and scheme params."""
+
+    _fields_ = [
+        ("N1", C.c_int), ("N2", C.c_int), ("N3", C.c_int), ("dim", C.c_int), ("numGhost", C.c_int),
+        ("X1Start", C.c_double), ("X1End", C.c_double), ("X2Start", C.c_double),
+        ("X2End", C.c_double), ("X3Start", C.c_double), ("X3End", C.c_double),
+        ("metric", C.c_int), ("blackHoleSpin", C.c_double), ("hSlope", C.c_double),
+        ("DerefineThetaHorizon", C.c_int), ("DoCylindrify", C.c_int),
+        ("X1cyl", C.c_double), ("X2cyl", C.c_double),
+        ("boundaryLeft", C.c_int), ("boundaryRight", C.c_int), ("boundaryBottom", C.c_int),
+        ("boundaryTop", C.c_int), ("boundaryBack", C.c_int), ("boundaryFront", C.c_int),
+        ("reconstruction", C.c_int), ("riemannSolver", C.c_int), ("slopeLimTheta", C.c_double),
+        ("CourantFactor", C.c_double), ("maxDtIncrement", C.c_double),
+    ]
+
+    @property
+    def shape(self):
+        """(N3T, N2T, N1T): shape of one plane, X1 fastest."""
+        g = self.numGhost
+        return (self.N3 + (2 * g if self.dim > 2 else 0), self.N2 + (2 * g if self.dim > 1 else 0),
+                self.N1 + 2 * g)
+
+    @property
+    def n(self) -> int:
+        a, b, c = self.shape
+        return a * b * c
+
+
+class StepStats(C.Structure):
+    _fields_ = [("dt", C.c_double), ("unconverged", C.c_int64 * 2), ("max_iterations", C.c_int32 * 2)]
+
+
 # Every symbol include/grim_hip.h declares, with its ctypes signature.
 _vp, _i64, _d, _i = C.c_void_p, C.c_int64, C.c_double, C.c_int
 _P = C.POINTER(Params)
 _B = C.POINTER(ZoneBatch)
+_G = C.POINTER(GridConfig)
+_pvp = C.POINTER(C.c_void_p)
 SIGNATURES = {
     "grim_default_params": (None, [_P, _i]),
     "grim_last_error": (C.c_char_p, []),
@@ -130,6 +178,25 @@ SIGNATURES = {
     "grim_known_root_rhs": (_i, [_P, _B, _vp, _vp, _vp]),
     "grim_newton_workspace_bytes": (C.c_int64, [_i64]),
     "grim_autotune_newton": (_i, [_P, _B, C.POINTER(NewtonOpts), _vp]),
+    "grim_grid_zones": (_i64, [_G]),
+    "grim_create": (_i, [_P, _G, C.POINTER(NewtonOpts), _vp, _pvp]),
+    "grim_destroy": (_i, [_vp]),
+    "grim_prim": (_i, [_vp, _pvp]),
+    "grim_geometry": (_i, [_vp, _i, _pvp]),
+    "grim_set_prim": (_i, [_vp, _vp, _i]),
+    "grim_get_prim": (_i, [_vp, _vp, _i]),
+    "grim_get_geometry": (_i, [_vp, _i, _vp, _i]),
+    "grim_set_time": (_i, [_vp, _d, _d]),
+    "grim_get_time": (_i, [_vp, C.POINTER(_d), C.POINTER(_d)]),
+    "grim_time_step": (_i, [_vp, C.POINTER(StepStats)]),
+    "grim_apply_boundaries": (_i, [_vp, _vp, _i]),
+    "grim_communicate": (_i, [_vp, _vp, _i]),
+    "grim_reconstruct": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
+    "grim_riemann_flux": (_i, [_vp, _vp, _i, _vp]),
+    "grim_div_fluxes": (_i, [_vp, _vp, _vp, _vp]),
+    "grim_explicit_sources": (_i, [_vp, _vp, _vp]),
+    "grim_compute_dt": (_i, [_vp, _vp, _d, C.POINTER(_d)]),
+    "grim_init_linear_mode": (_i, [_vp, _i, _d]),
 }
 
 _lib = None

@@ -64,6 +64,10 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonGlobalUpdate(DevParams pr
   if (counts[k] == 0) return;
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= b.n) return;
+  // Zones outside the bulk (mask 0) have a zero residual and a zero
+  // Jacobian: they take no update (grim would write the non-finite result
+  // of the singular solve there, later overwritten by the boundaries).
+  if (b.residual_mask && b.residual_mask[z] == 0.) return;
   const int64_t n = b.n;
   ZoneGeom G;
   loadGeom(b.geom, n, z, G);

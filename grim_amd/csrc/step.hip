@@ -1,0 +1,1247 @@
+// The full time step on a grid: grim's timeStepper on one gfx950 device.
+//
+// One context owns the grid arrays.  A stage is a short chain of kernels on
+// the context's stream:
+//   boundaries   -> ghost slabs of the stage state
+//   [EMHD] uT    -> u_mu and T of every zone (the arrays grim's slopes read)
+//   sources      -> S_expl, S_implOld and (half stage) consOld, bulk zones
+//   face fluxes  -> one kernel per direction: reconstruction of both sides,
+//                   two elements, fluxes, conserved variables, speeds, HLL/LLF
+//   EMF          -> edge EMFs for flux-CT (dim >= 2)
+//   fold         -> divergence (CT fluxes formed on the fly), the folded
+//                   right-hand side of the Newton residual and the induction
+//                   update of B, bulk zones
+//   Newton       -> grim_batched_newton on the whole grid (mask 0 outside the
+//                   bulk: those zones are not solved)
+//   communicate  -> periodic ghost fill of the result
+// Every kernel evaluates grim's expressions in grim's order (built with
+// -ffp-contract=off), so a stage agrees bit for bit with the CPU FP64
+// restatement given the same geometry.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "grim_hip.h"
+#include "runtime.hpp"
+#include "step_physics.hpp"
+#include "zone_physics.hpp"
+
+struct grim_ctx;
+
+namespace grim {
+namespace {
+
+constexpr int kBlock = 256;
+
+struct StepArgs {
+  Dims D;
+  DevParams prm;
+  double dX[3];
+  double theta;
+  int riemann;
+  int metric;
+};
+
+__device__ __forceinline__ int64_t flatIndex(const Dims& D, int i, int j, int k) {
+  return i + (int64_t)D.NT[0] * (j + (int64_t)D.NT[1] * k);
+}
+
+inline unsigned blocksFor(int64_t count) { return (unsigned)((count + kBlock - 1) / kBlock); }
+
+// ---------------------------------------------------------------------------
+// Geometry (set-up): coordinates (grid.cpp:168-249), metric, inverse, alpha,
+// g and r at one location; the connection at the centre (geometry.cpp:61-121,
+// 631-680).
+// ---------------------------------------------------------------------------
+struct CoordGrid {
+  double start[3], dX[3];
+};
+
+__device__ __forceinline__ void XCoordsAt(const Dims& D, const CoordGrid& cg, int i, int j, int k,
+                                          int loc, double* X) {
+  const double i1 = (double)i - D.g[0], i2 = (double)j - D.g[1], i3 = (double)k - D.g[2];
+  X[0] = cg.start[0] + (loc == 1 ? (i1)*cg.dX[0] : (i1 + 0.5) * cg.dX[0]);
+  X[1] = cg.start[1] + (loc == 2 ? (i2)*cg.dX[1] : (i2 + 0.5) * cg.dX[1]);
+  X[2] = cg.start[2] + (i3 + 0.5) * cg.dX[2];
+}
+
+__global__ __launch_bounds__(kBlock) void geometryKernel(Dims D, CoordMap cm, CoordGrid cg, int loc,
+                                                         double* __restrict__ out) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= D.n) return;
+  const int i = (int)(z % D.NT[0]), j = (int)((z / D.NT[0]) % D.NT[1]),
+            k = (int)(z / ((int64_t)D.NT[0] * D.NT[1]));
+  double X[3], x[3], G[4][4], C[4][4], det;
+  XCoordsAt(D, cg, i, j, k, loc, X);
+  cm.toX(X, x);
+  cm.gCov(X, G);
+  gDetAndgCon(G, det, C);
+  const int64_t n = D.n;
+  const int cov[10][2] = {{0, 0}, {0, 1}, {0, 2}, {0, 3}, {1, 1}, {1, 2}, {1, 3}, {2, 2}, {2, 3}, {3, 3}};
+#pragma unroll
+  for (int q = 0; q < 10; q++) out[q * n + z] = G[cov[q][0]][cov[q][1]];
+#pragma unroll
+  for (int q = 0; q < 4; q++) out[(10 + q) * n + z] = C[0][q];
+  out[14 * n + z] = 1. / sqrt(-C[0][0]);
+  out[15 * n + z] = sqrt(-det);
+#pragma unroll
+  for (int q = 0; q < 6; q++) out[(16 + q) * n + z] = C[cov[4 + q][0]][cov[4 + q][1]];
+  out[LOC_R * n + z] = x[0];
+}
+
+__global__ __launch_bounds__(kBlock) void connectionKernel(Dims D, CoordMap cm, CoordGrid cg,
+                                                           double* __restrict__ gamma) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= D.n) return;
+  const int i = (int)(z % D.NT[0]), j = (int)((z / D.NT[0]) % D.NT[1]),
+            k = (int)(z / ((int64_t)D.NT[0] * D.NT[1]));
+  const double EPS = 1.e-5;
+  double X[3], G[4][4], C[4][4], det;
+  XCoordsAt(D, cg, i, j, k, 0, X);
+  cm.gCov(X, G);
+  gDetAndgCon(G, det, C);
+  double Gp[3][4][4], Gm[3][4][4];
+  for (int d = 0; d < 3; d++) {
+    double Xp[3] = {X[0], X[1], X[2]}, Xm[3] = {X[0], X[1], X[2]};
+    Xp[d] += EPS;
+    Xm[d] -= EPS;
+    cm.gCov(Xp, Gp[d]);
+    cm.gCov(Xm, Gm[d]);
+  }
+  const int64_t n = D.n;
+  for (int mu = 0; mu < 4; mu++)
+    for (int nu = 0; nu < 4; nu++)
+      for (int la = 0; la < 4; la++) {
+        double s = 0.;
+        for (int eta = 0; eta < 4; eta++) {
+          double out = 0.;
+          if (la > 0) {
+            out += 0.5 * Gp[la - 1][eta][nu] / (2. * EPS);
+            out -= 0.5 * Gm[la - 1][eta][nu] / (2. * EPS);
+          }
+          if (nu > 0) {
+            out += 0.5 * Gp[nu - 1][eta][la] / (2. * EPS);
+            out -= 0.5 * Gm[nu - 1][eta][la] / (2. * EPS);
+          }
+          if (eta > 0) {
+            out -= 0.5 * Gp[eta - 1][nu][la] / (2. * EPS);
+            out += 0.5 * Gm[eta - 1][nu][la] / (2. * EPS);
+          }
+          s += C[mu][eta] * out;
+        }
+        gamma[(mu * 16 + nu * 4 + la) * n + z] = s;
+      }
+}
+
+__global__ __launch_bounds__(kBlock) void maskKernel(Dims D, double* __restrict__ mask) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= D.n) return;
+  const int i = (int)(z % D.NT[0]), j = (int)((z / D.NT[0]) % D.NT[1]),
+            k = (int)(z / ((int64_t)D.NT[0] * D.NT[1]));
+  const bool bulk = i >= D.g[0] && i < D.g[0] + D.N[0] && j >= D.g[1] && j < D.g[1] + D.N[1] &&
+                    k >= D.g[2] && k < D.g[2] + D.N[2];
+  mask[z] = bulk ? 1. : 0.;
+}
+
+// ---------------------------------------------------------------------------
+// Boundaries
+// ---------------------------------------------------------------------------
+struct BcTypes {
+  int lo[3], hi[3];
+};
+
+// grid::communicate on one rank: ghost zones whose out-of-range coordinates
+// all lie along periodic directions take the periodic image (DMDA box
+// stencil, grid.cpp:88-121, 306-315).
+__global__ __launch_bounds__(kBlock) void communicateKernel(Dims D, BcTypes bc, int nvars,
+                                                            double* __restrict__ prim) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= D.n) return;
+  int I[3] = {(int)(z % D.NT[0]), (int)((z / D.NT[0]) % D.NT[1]),
+              (int)(z / ((int64_t)D.NT[0] * D.NT[1]))};
+  bool ghost = false;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (D.g[d] == 0) continue;
+    if (I[d] < D.g[d]) {
+      if (bc.lo[d] != GRIM_BC_PERIODIC) return;
+      I[d] += D.N[d];
+      ghost = true;
+    } else if (I[d] >= D.g[d] + D.N[d]) {
+      if (bc.hi[d] != GRIM_BC_PERIODIC) return;
+      I[d] -= D.N[d];
+      ghost = true;
+    }
+  }
+  if (!ghost) return;
+  const int64_t w = flatIndex(D, I[0], I[1], I[2]);
+  for (int v = 0; v < nvars; v++) prim[v * D.n + z] = prim[v * D.n + w];
+}
+
+// One ghost slab of boundaries::applyBoundaryConditions (boundary.cpp:32-239):
+// outflow copies the first interior layer, mirror the reflected layer.
+__global__ __launch_bounds__(kBlock) void boundaryKernel(Dims D, Box box, int d, int side, int type,
+                                                         int nvars, double* __restrict__ prim) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= box.count()) return;
+  int I[3];
+  box.at(t, I[0], I[1], I[2]);
+  const int g = D.g[d], N = D.N[d], a = I[d];
+  const int64_t z = flatIndex(D, I[0], I[1], I[2]);
+  if (type == GRIM_BC_MIRROR)
+    I[d] = side == 0 ? 2 * g - 1 - a : 2 * (N + g) - 1 - a;
+  else
+    I[d] = side == 0 ? g : N + g - 1;
+  const int64_t w = flatIndex(D, I[0], I[1], I[2]);
+  for (int v = 0; v < nvars; v++) prim[v * D.n + z] = prim[v * D.n + w];
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction and face fluxes
+// ---------------------------------------------------------------------------
+template <int RECON>
+__global__ __launch_bounds__(kBlock) void reconstructKernel(Dims D, Box box, int64_t s, double theta,
+                                                            int nvars, const double* __restrict__ prim,
+                                                            double* __restrict__ left,
+                                                            double* __restrict__ right) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= box.count()) return;
+  int i, j, k;
+  box.at(t, i, j, k);
+  const int64_t z = flatIndex(D, i, j, k);
+  for (int v = 0; v < nvars; v++) {
+    double y[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) y[q] = prim[v * D.n + z + (q - 2) * s];
+    double l, r;
+    reconstructZone<RECON>(y, theta, l, r);
+    left[v * D.n + z] = l;
+    right[v * D.n + z] = r;
+  }
+}
+
+// Flux at the lower face (i - 1/2 along DIR) of zone z: riemannSolver::solve
+// (riemannsolver.cpp:202-328) for one face.  The left state is the right
+// face value of zone z - s, the right state the left face value of zone z;
+// both sit at the face geometry.
+template <class L, int DIR, int RECON>
+__global__ __launch_bounds__(kBlock) void faceFluxKernel(StepArgs a, Box box,
+                                                         const double* __restrict__ prim,
+                                                         const double* __restrict__ geomFace,
+                                                         double* __restrict__ flux) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= box.count()) return;
+  int i, j, k;
+  box.at(t, i, j, k);
+  const Dims& D = a.D;
+  const int64_t n = D.n, z = flatIndex(D, i, j, k), s = D.s[DIR];
+  double PL[L::DOF], PR[L::DOF];
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) {
+    double y[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) y[q] = prim[v * n + z + (q - 3) * s];
+    double l, r;
+    reconstructZone<RECON>(y, a.theta, l, r);
+    PL[v] = r;
+    reconstructZone<RECON>(y + 1, a.theta, l, r);
+    PR[v] = l;
+  }
+  GeomFull G;
+  loadGeomFull(geomFace, n, z, G);
+  double FL[L::DOF], UL[L::DOF], minL, maxL;
+  {
+    Kinematics kk;
+    Closure cc;
+    kinematics<L>(PL, G.zg, a.prm, kk);
+    elementClosure<L>(PL, kk, a.prm, G.r, cc);
+    fluxes<L, DIR + 1>(PL, kk, cc, G.zg.g, FL);
+    fluxes<L, 0>(PL, kk, cc, G.zg.g, UL);
+    charSpeeds<L, DIR>(kk, cc, G, a.prm, minL, maxL);
+  }
+  double FR[L::DOF], UR[L::DOF], minR, maxR;
+  {
+    Kinematics kk;
+    Closure cc;
+    kinematics<L>(PR, G.zg, a.prm, kk);
+    elementClosure<L>(PR, kk, a.prm, G.r, cc);
+    fluxes<L, DIR + 1>(PR, kk, cc, G.zg.g, FR);
+    fluxes<L, 0>(PR, kk, cc, G.zg.g, UR);
+    charSpeeds<L, DIR>(kk, cc, G, a.prm, minR, maxR);
+  }
+  const double minSpeed = stdmin(minL, minR);
+  const double maxSpeed = stdmax(maxL, maxR);
+  if (a.riemann == GRIM_RIEMANN_HLL) {
+#pragma unroll
+    for (int v = 0; v < L::DOF; v++)
+      flux[v * n + z] = (maxSpeed * FL[v] - minSpeed * FR[v] + minSpeed * maxSpeed * (UR[v] - UL[v])) /
+                        (maxSpeed - minSpeed);
+  } else {
+    const double c = stdmax(maxSpeed, -minSpeed);
+#pragma unroll
+    for (int v = 0; v < L::DOF; v++) flux[v * n + z] = 0.5 * (FL[v] + FR[v] - c * (UR[v] - UL[v]));
+  }
+}
+
+// timeStepper::computeEMF (constrainedtransport.cpp:49-83) on the edges the
+// bulk divergence reads.
+template <class L>
+__global__ __launch_bounds__(kBlock) void emfKernel(Dims D, Box box, const double* __restrict__ F,
+                                                    double* __restrict__ emf) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= box.count()) return;
+  int i, j, k;
+  box.at(t, i, j, k);
+  const int64_t n = D.n, z = flatIndex(D, i, j, k), s1 = D.s[0], s2 = D.s[1], s3 = D.s[2];
+  const double* F1 = F;
+  const double* F2 = F + (size_t)L::DOF * n;
+  const double* F3 = F + (size_t)2 * L::DOF * n;
+  const double* F1B2 = F1 + L::B2 * n;
+  const double* F2B1 = F2 + L::B1 * n;
+  emf[2 * n + z] = 0.25 * (F1B2[z] + F1B2[z - s2] - F2B1[z] - F2B1[z - s1]);
+  if (D.dim == 3) {
+    const double* F2B3 = F2 + L::B3 * n;
+    const double* F3B2 = F3 + L::B2 * n;
+    const double* F3B1 = F3 + L::B1 * n;
+    const double* F1B3 = F1 + L::B3 * n;
+    emf[0 * n + z] = 0.25 * (F2B3[z] + F2B3[z - s3] - F3B2[z] - F3B2[z - s2]);
+    emf[1 * n + z] = 0.25 * (F3B1[z] + F3B1[z - s1] - F1B3[z] - F1B3[z - s3]);
+  }
+}
+
+// Flux of variable v through the lower face of zone z along d, after
+// timeStepper::fluxCT (constrainedtransport.cpp:3-47) for dim >= 2.
+template <class L>
+__device__ __forceinline__ double ctFlux(const Dims& D, const double* __restrict__ F,
+                                         const double* __restrict__ emf, int d, int v, int64_t z) {
+  const int64_t n = D.n;
+  if (D.dim >= 2 && v >= L::B1) {
+    const double* e1 = emf;
+    const double* e2 = emf + n;
+    const double* e3 = emf + 2 * n;
+    const int64_t s1 = D.s[0], s2 = D.s[1], s3 = D.s[2];
+    const int b = v - L::B1;
+    if (d == 0) {
+      if (b == 0) return 0.;
+      if (b == 1) return 0.5 * (e3[z] + e3[z + s2]);
+      if (D.dim == 3) return -0.5 * (e2[z] + e2[z + s3]);
+    } else if (d == 1) {
+      if (b == 0) return -0.5 * (e3[z] + e3[z + s1]);
+      if (b == 1) return 0.;
+      if (D.dim == 3) return 0.5 * (e1[z] + e1[z + s3]);
+    } else {
+      if (b == 0) return 0.5 * (e2[z] + e2[z + s1]);
+      if (b == 1) return -0.5 * (e1[z] + e1[z + s2]);
+      return 0.;
+    }
+  }
+  return F[((size_t)d * L::DOF + v) * n + z];
+}
+
+// Divergence of the fluxes of zone z, variable v (fvmfluxes.cpp:36-46,
+// 91-103, 164-180): dim < 3 is af::convolve with {1, -1, 0}/dX,
+// dim 3 is (shift(F, -1) - F)/dX.
+template <class L>
+__device__ __forceinline__ double divergence(const Dims& D, const double* dX,
+                                             const double* __restrict__ F,
+                                             const double* __restrict__ emf, int v, int64_t z) {
+  double acc = 0.;
+  for (int d = 0; d < D.dim; d++) {
+    const double f0 = ctFlux<L>(D, F, emf, d, v, z);
+    const double f1 = ctFlux<L>(D, F, emf, d, v, z + D.s[d]);
+    double term;
+    if (D.dim < 3) {
+      const double w = 1. / dX[d];
+      term = f1 * w + f0 * (-w);
+    } else {
+      term = (f1 - f0) / dX[d];
+    }
+    acc = (d == 0) ? term : acc + term;
+  }
+  return acc;
+}
+
+// Bulk zones: divF (optional output), and the stage fold
+//   rhs = consOld - dt'((divF + S_expl) + 1/2 S_implOld)      (fluid rows)
+//   B   = (consOld_B - dt' divF_B) / g                        (timestep.cpp:80-89)
+template <class L>
+__global__ __launch_bounds__(kBlock) void foldKernel(StepArgs a, Box box, const double* __restrict__ F,
+                                                     const double* __restrict__ emf,
+                                                     const double* __restrict__ geomC,
+                                                     const double* __restrict__ consOld,
+                                                     const double* __restrict__ sExp,
+                                                     const double* __restrict__ sImp, double dtPrime,
+                                                     double* __restrict__ divOut,
+                                                     double* __restrict__ rhs,
+                                                     double* __restrict__ primNew) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= box.count()) return;
+  int i, j, k;
+  box.at(t, i, j, k);
+  const Dims& D = a.D;
+  const int64_t n = D.n, z = flatIndex(D, i, j, k);
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) {
+    const double div = divergence<L>(D, a.dX, F, emf, v, z);
+    if (divOut) divOut[v * n + z] = div;
+    if (!rhs) continue;
+    const double U = consOld[v * n + z];
+    if (v < L::NF) {
+      const double si = L::EMHD ? sImp[v * n + z] : 0.;
+      rhs[v * n + z] = U - dtPrime * ((div + sExp[v * n + z]) + 0.5 * si);
+    } else {
+      const double consB = U - dtPrime * div;
+      primNew[v * n + z] = consB / geomC[GEOM_G * n + z];
+    }
+  }
+}
+
+// Post-CT flux arrays for the faces of the bulk zones (API output only).
+template <class L>
+__global__ __launch_bounds__(kBlock) void ctWriteKernel(Dims D, Box box, const double* __restrict__ F,
+                                                        const double* __restrict__ emf,
+                                                        double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= box.count()) return;
+  int i, j, k;
+  box.at(t, i, j, k);
+  const int64_t z = flatIndex(D, i, j, k);
+  for (int d = 0; d < D.dim; d++)
+    for (int v = L::B1; v < L::DOF; v++) out[((size_t)d * L::DOF + v) * D.n + z] = ctFlux<L>(D, F, emf, d, v, z);
+}
+
+// ---------------------------------------------------------------------------
+// Sources
+// ---------------------------------------------------------------------------
+// u_mu and T of every zone of the stage state (EMHD gradients read them).
+template <class L>
+__global__ __launch_bounds__(kBlock) void uTKernel(StepArgs a, const double* __restrict__ prim,
+                                                   const double* __restrict__ geomC,
+                                                   double* __restrict__ uT) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = a.D.n;
+  if (z >= n) return;
+  double P[L::DOF];
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
+  ZoneGeom G;
+  loadGeom(geomC, n, z, G);
+  Kinematics kk;
+  kinematics<L>(P, G, a.prm, kk);
+#pragma unroll
+  for (int m = 0; m < 4; m++) uT[m * n + z] = kk.uCov[m];
+  uT[4 * n + z] = kk.temperature;
+}
+
+// Bulk zones: fluidElement::computeExplicitSources with computeEMHDGradients
+// (physics.cpp:360-586) of the stage element; computeImplicitSources of
+// elemOld with the stage element's tau (timestep.cpp:56-60, 211-215); on the
+// half stage also consOld = computeFluxes(0) of elemOld (timestep.cpp:33-36).
+template <class L, int RECON>
+__global__ __launch_bounds__(kBlock) void sourcesKernel(StepArgs a, Box box,
+                                                        const double* __restrict__ prim,
+                                                        const double* __restrict__ primOld,
+                                                        const double* __restrict__ geomC,
+                                                        const double* __restrict__ gamma,
+                                                        const double* __restrict__ uT,
+                                                        double* __restrict__ sExp,
+                                                        double* __restrict__ sImp,
+                                                        double* __restrict__ consOld) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= box.count()) return;
+  int i, j, k;
+  box.at(t, i, j, k);
+  const Dims& D = a.D;
+  const int64_t n = D.n, z = flatIndex(D, i, j, k);
+  double P[L::DOF];
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
+  GeomFull G;
+  loadGeomFull(geomC, n, z, G);
+  Kinematics kk;
+  Closure cc;
+  kinematics<L>(P, G.zg, a.prm, kk);
+  elementClosure<L>(P, kk, a.prm, G.r, cc);
+  const double g = G.zg.g;
+  if (consOld) {
+    double U[L::DOF];
+    fluxes<L, 0>(P, kk, cc, g, U);
+#pragma unroll
+    for (int v = 0; v < L::DOF; v++) consOld[v * n + z] = U[v];
+  }
+  double src[L::NF];
+#pragma unroll
+  for (int v = 0; v < L::NF; v++) src[v] = 0.;
+  if (a.metric != GRIM_METRIC_MINKOWSKI) {
+    double T[4][4];
+    stressRow<L, 0>(kk, cc, T[0]);
+    stressRow<L, 1>(kk, cc, T[1]);
+    stressRow<L, 2>(kk, cc, T[2]);
+    stressRow<L, 3>(kk, cc, T[3]);
+#pragma unroll
+    for (int nu = 0; nu < 4; nu++)
+#pragma unroll
+      for (int kappa = 0; kappa < 4; kappa++)
+#pragma unroll
+        for (int lamda = 0; lamda < 4; lamda++)
+          src[UU + nu] -= g * T[kappa][lamda] * gamma[(lamda * 16 + kappa * 4 + nu) * n + z];
+  }
+  if constexpr (L::EMHD) {
+    auto gam = [&](int l, int p, int q) { return gamma[(l * 16 + p * 4 + q) * n + z]; };
+    auto slopeOf = [&](const double* arr, int d) {
+      double y[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) y[q] = arr[z + (q - 2) * D.s[d]];
+      return slopeZone<RECON>(y, a.theta, a.dX[d]);
+    };
+    double graduCov[4][4];
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) {
+      const double* um = uT + mu * n;
+      graduCov[0][mu] = 0.;
+      graduCov[1][mu] = slopeOf(um, 0);
+      graduCov[2][mu] = 0.;
+      if (D.dim > 1) graduCov[2][mu] = slopeOf(um, 1);
+      graduCov[3][mu] = 0.;
+      if (D.dim > 2) graduCov[3][mu] = slopeOf(um, 2);
+#pragma unroll
+      for (int nu = 0; nu < 4; nu++)
+#pragma unroll
+        for (int lambda = 0; lambda < 4; lambda++) graduCov[nu][mu] -= gam(lambda, nu, mu) * kk.uCov[lambda];
+    }
+    double divuCov = 0.;
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++)
+#pragma unroll
+      for (int nu = 0; nu < 4; nu++) divuCov += G.con(mu, nu) * graduCov[mu][nu];
+    if constexpr (L::VISC_ON) {
+      double deltaP0 = -divuCov * kk.rho * cc.nu;
+#pragma unroll
+      for (int mu = 0; mu < 4; mu++)
+#pragma unroll
+        for (int nu = 0; nu < 4; nu++)
+          deltaP0 += 3. * kk.rho * cc.nu * kk.bCon[mu] * kk.bCon[nu] / kk.bSqr * graduCov[mu][nu];
+      double deltaP0Tilde = deltaP0;
+      if (a.prm.highOrderVisc == 1)
+        deltaP0Tilde = deltaP0 * sqrt(cc.tau / kk.rho / cc.nu / kk.temperature);
+      double sd = -g * (deltaP0Tilde) / cc.tau;
+      if (a.prm.highOrderVisc == 1) sd -= 0.5 * g * divuCov * P[L::DP];
+      src[L::DP] = sd;
+    }
+    if constexpr (L::COND_ON) {
+      const double* Tp = uT + 4 * n;
+      double gradT[4];
+      gradT[0] = 0.;
+      gradT[1] = slopeOf(Tp, 0);
+      gradT[2] = 0.;
+      if (D.dim > 1) gradT[2] = slopeOf(Tp, 1);
+      gradT[3] = 0.;
+      if (D.dim > 2) gradT[3] = slopeOf(Tp, 2);
+      double q0 = 0.;
+#pragma unroll
+      for (int mu = 0; mu < 4; mu++) {
+        q0 -= kk.rho * cc.chi * kk.bCon[mu] / cc.bNorm * gradT[mu];
+#pragma unroll
+        for (int nu = 0; nu < 4; nu++)
+          q0 -= kk.rho * cc.chi * kk.temperature * kk.bCon[nu] / cc.bNorm * kk.uCon[mu] * graduCov[mu][nu];
+      }
+      double q0Tilde = q0;
+      if (a.prm.highOrderCond == 1) q0Tilde = q0 * sqrt(cc.tau / kk.rho / cc.chi) / kk.temperature;
+      double sq = -g * (q0Tilde) / cc.tau;
+      if (a.prm.highOrderCond == 1) sq -= 0.5 * g * divuCov * P[L::Q];
+      src[L::Q] = sq;
+    }
+    if (sImp) {
+      // computeImplicitSources of elemOld with tauDamp = the stage element's tau
+#pragma unroll
+      for (int v = 0; v < L::NF; v++) sImp[v * n + z] = 0.;
+      if constexpr (L::VISC_ON) sImp[L::DP * n + z] = g * (primOld[L::DP * n + z]) / cc.tau;
+      if constexpr (L::COND_ON) sImp[L::Q * n + z] = g * (primOld[L::Q * n + z]) / cc.tau;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < L::NF; v++) sExp[v * n + z] = src[v];
+}
+
+// ---------------------------------------------------------------------------
+// dt and stats
+// ---------------------------------------------------------------------------
+template <class L>
+__global__ __launch_bounds__(kBlock) void dtKernel(StepArgs a, const double* __restrict__ prim,
+                                                   const double* __restrict__ geomC,
+                                                   double* __restrict__ blockMax) {
+  __shared__ double red[kBlock];
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = a.D.n;
+  double m = -INFINITY;
+  if (z < n) {
+    double P[L::DOF];
+#pragma unroll
+    for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
+    GeomFull G;
+    loadGeomFull(geomC, n, z, G);
+    Kinematics kk;
+    Closure cc;
+    kinematics<L>(P, G.zg, a.prm, kk);
+    elementClosure<L>(P, kk, a.prm, G.r, cc);
+    double maxSpeed = 0.;
+    for (int d = 0; d < a.D.dim; d++) {
+      double mn, mx;
+      if (d == 0) charSpeeds<L, 0>(kk, cc, G, a.prm, mn, mx);
+      else if (d == 1) charSpeeds<L, 1>(kk, cc, G, a.prm, mn, mx);
+      else charSpeeds<L, 2>(kk, cc, G, a.prm, mn, mx);
+      mn = mn / a.dX[d];
+      mx = mx / a.dX[d];
+      const double s = stdmax(mx, fabs(mn));
+      maxSpeed = (d == 0) ? s : maxSpeed + s;
+    }
+    m = maxSpeed;
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = stdmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blockMax[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kBlock) void maxReduceKernel(int64_t count, const double* __restrict__ in,
+                                                          double* __restrict__ out) {
+  __shared__ double red[kBlock];
+  double m = -INFINITY;
+  for (int64_t t = threadIdx.x; t < count; t += kBlock) m = stdmax(m, in[t]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = stdmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// Bulk-zone Newton statistics: count above atol and the largest iteration
+// count (vector-memory atomics).
+__global__ __launch_bounds__(kBlock) void stageStatsKernel(int64_t n, const double* __restrict__ mask,
+                                                           const double* __restrict__ l2,
+                                                           const int* __restrict__ iters, double atol,
+                                                           unsigned long long* __restrict__ out) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= n || mask[z] == 0.) return;
+  if (l2[z] > atol) atomicAdd(&out[0], 1ull);
+  atomicMax(&out[1], (unsigned long long)iters[z]);
+}
+
+// ---------------------------------------------------------------------------
+// Linear-mode initial conditions (src/problem/linear_modes/problem.cpp:10-111)
+// ---------------------------------------------------------------------------
+// Ideal fast magnetosonic eigenvector on rho = 1, u = 2, B = (0.1, 0.3, 0),
+// Gamma = 4/3, k = (2 pi, 2 pi), unit norm; derived from the linearised
+// equations (tests/golden/fast_mode.json holds the derivation's output).
+__constant__ double kFastMode[8] = {0.3454194280099405,  0.9211184746935905,  0.12567391680338838,
+                                    0.11701881004980452, 0.,                  -0.03700566889854888,
+                                    0.037005668898594656, 0.};
+
+template <class L>
+__global__ __launch_bounds__(kBlock) void linearModeKernel(StepArgs a, int mode, double amp,
+                                                           const double* __restrict__ geomC,
+                                                           CoordGrid cg, double* __restrict__ prim) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const Dims& D = a.D;
+  const int64_t n = D.n;
+  if (z >= n) return;
+  const int i = (int)(z % D.NT[0]), j = (int)((z / D.NT[0]) % D.NT[1]),
+            k = (int)(z / ((int64_t)D.NT[0] * D.NT[1]));
+  double X[3];
+  XCoordsAt(D, cg, i, j, k, 0, X);  // Minkowski: xCoords = XCoords
+  double P[L::DOF];
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) P[v] = 0.;
+  if (mode == 0) {
+    const double cphi = cos(2. * M_PI * X[0] + 2. * M_PI * X[1]);
+    const double bg[8] = {1., 2., 0., 0., 0., 0.1, 0.3, 0.};
+#pragma unroll
+    for (int v = 0; v < 5; v++) P[v] = bg[v] + amp * kFastMode[v] * cphi;
+#pragma unroll
+    for (int b = 0; b < 3; b++) P[L::B1 + b] = bg[5 + b] + amp * kFastMode[5 + b] * cphi;
+  } else {
+    const double Aw = amp;
+    const double k1 = 2. * M_PI, k2 = 4. * M_PI;
+    const double cphi = cos(k1 * X[0] + k2 * X[1]);
+    const double sphi = sin(k1 * X[0] + k2 * X[1]);
+    P[RHO] = 1.;
+    P[UU] = 2.;
+    P[L::B1] = 0.1;
+    P[L::B2] = 0.3;
+    P[RHO] += Aw * cphi * (-0.518522524082246) + Aw * sphi * 0.1792647678001878;
+    P[UU] += Aw * cphi * 0.5516170736393813;
+    P[U1] += Aw * cphi * 0.008463122479547856 + Aw * sphi * (-0.011862022608466367);
+    P[U2] += Aw * cphi * (-0.16175466371870734) + Aw * sphi * (0.034828080823603294);
+    P[L::B1] += Aw * cphi * (-0.05973794979640743) + Aw * sphi * 0.03351707506150924;
+    P[L::B2] += Aw * cphi * 0.02986897489820372 - Aw * sphi * 0.016758537530754618;
+    if constexpr (L::EMHD) {
+      double Q = Aw * cphi * 0.5233486841539436 - Aw * sphi * 0.04767672501939603;
+      double DP = Aw * cphi * 0.2909106062057657 - Aw * sphi * 0.02159452055336572;
+      const double gam = a.prm.gam;
+      const double pressure = (gam - 1.) * P[UU];
+      const double temperature = pressure / P[RHO];
+      const double soundSpeed = sqrt(gam * pressure / (P[RHO] + gam * P[UU]));
+      if (a.prm.highOrderCond)
+        Q = Q / temperature / sqrt(P[RHO] * a.prm.condAlpha * soundSpeed * soundSpeed);
+      if (a.prm.highOrderVisc)
+        DP = DP / sqrt(temperature * P[RHO] * a.prm.viscAlpha * soundSpeed * soundSpeed);
+      if constexpr (L::COND_ON) P[L::Q] = Q;
+      if constexpr (L::VISC_ON) P[L::DP] = DP;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) prim[v * n + z] = P[v];
+}
+
+}  // namespace
+}  // namespace grim
+
+using namespace grim;
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct grim_ctx {
+  grim_params p;
+  grim_grid_config c;
+  grim_newton_opts opts;
+  hipStream_t s;
+  Dims D;
+  StepArgs a;
+  CoordGrid cg;
+  CoordMap cm;
+  int dof, nf;
+  double time, dt;
+  double *primOld, *primHalf, *primNew, *primDt, *consOld;
+  double *fluxes, *emf, *sExp, *sImp, *uT, *rhs, *mask;
+  double *geomC, *geomX1, *geomX2, *gamma;
+  double* red;
+  int* iters;
+  double* l2;
+  unsigned long long* counters;
+  double* hostBuf;  // pinned: dt / stats readback
+};
+
+namespace grim {
+namespace {
+
+template <class F>
+int withLayout(const grim_ctx* x, F&& f) {
+  return dispatchLayout(x->p, std::forward<F>(f));
+}
+
+Box bulkBox(const Dims& D) {
+  Box b;
+  for (int d = 0; d < 3; d++) {
+    b.lo[d] = D.g[d];
+    b.hi[d] = D.g[d] + D.N[d];
+  }
+  return b;
+}
+
+// Faces whose 6-point stencil lies in the grid (the oracle's face range).
+Box faceBox(const Dims& D, int dir) {
+  Box b;
+  for (int d = 0; d < 3; d++) {
+    b.lo[d] = 0;
+    b.hi[d] = D.NT[d];
+  }
+  b.lo[dir] = 3;
+  b.hi[dir] = D.NT[dir] - 2;
+  return b;
+}
+
+// Edges read by the bulk divergence: [g, N + g] in each active dimension.
+Box edgeBox(const Dims& D) {
+  Box b;
+  for (int d = 0; d < 3; d++) {
+    b.lo[d] = D.g[d];
+    b.hi[d] = D.g[d] + D.N[d] + (d < D.dim ? 1 : 0);
+  }
+  return b;
+}
+
+int applyBoundaries(grim_ctx* x, double* prim, int nvars, bool periodicFill) {
+  const Dims& D = x->D;
+  BcTypes bc;
+  const int lo[3] = {x->c.boundaryLeft, x->c.boundaryBottom, x->c.boundaryBack};
+  const int hi[3] = {x->c.boundaryRight, x->c.boundaryTop, x->c.boundaryFront};
+  for (int d = 0; d < 3; d++) {
+    bc.lo[d] = lo[d];
+    bc.hi[d] = hi[d];
+  }
+  if (periodicFill) communicateKernel<<<blocksFor(D.n), kBlock, 0, x->s>>>(D, bc, nvars, prim);
+  const int types[6] = {lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]};
+  for (int d = 0; d < D.dim; d++)
+    for (int side = 0; side < 2; side++) {
+      const int type = types[2 * d + side];
+      if (type != GRIM_BC_OUTFLOW && type != GRIM_BC_MIRROR) continue;
+      Box b;
+      for (int e = 0; e < 3; e++) {
+        b.lo[e] = 0;
+        b.hi[e] = D.NT[e];
+      }
+      b.lo[d] = side == 0 ? 0 : D.N[d] + D.g[d];
+      b.hi[d] = side == 0 ? D.g[d] : D.N[d] + 2 * D.g[d];
+      boundaryKernel<<<blocksFor(b.count()), kBlock, 0, x->s>>>(D, b, d, side, type, nvars, prim);
+    }
+  return checkLaunch("boundaries");
+}
+
+int communicate(grim_ctx* x, double* prim, int nvars) {
+  BcTypes bc;
+  const int lo[3] = {x->c.boundaryLeft, x->c.boundaryBottom, x->c.boundaryBack};
+  const int hi[3] = {x->c.boundaryRight, x->c.boundaryTop, x->c.boundaryFront};
+  for (int d = 0; d < 3; d++) {
+    bc.lo[d] = lo[d];
+    bc.hi[d] = hi[d];
+  }
+  communicateKernel<<<blocksFor(x->D.n), kBlock, 0, x->s>>>(x->D, bc, nvars, prim);
+  return checkLaunch("communicate");
+}
+
+template <class L, int DIR>
+void launchFace(grim_ctx* x, const double* prim, double* flux) {
+  const Box b = faceBox(x->D, DIR);
+  const double* gf = DIR == 0 ? x->geomX1 : (DIR == 1 ? x->geomX2 : x->geomC);
+  const unsigned nb = blocksFor(b.count());
+  switch (x->c.reconstruction) {
+    case GRIM_RECON_WENO5:
+      faceFluxKernel<L, DIR, GRIM_RECON_WENO5><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);
+      break;
+    case GRIM_RECON_PPM:
+      faceFluxKernel<L, DIR, GRIM_RECON_PPM><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);
+      break;
+    default:
+      faceFluxKernel<L, DIR, GRIM_RECON_MINMOD><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);
+  }
+}
+
+// Riemann fluxes of every active direction into x->fluxes, then the EMFs.
+template <class L>
+int fluxesAndEmf(grim_ctx* x, const double* prim) {
+  const size_t plane = (size_t)L::DOF * x->D.n;
+  launchFace<L, 0>(x, prim, x->fluxes);
+  if (x->D.dim > 1) launchFace<L, 1>(x, prim, x->fluxes + plane);
+  if (x->D.dim > 2) launchFace<L, 2>(x, prim, x->fluxes + 2 * plane);
+  if (x->D.dim > 1) {
+    const Box e = edgeBox(x->D);
+    emfKernel<L><<<blocksFor(e.count()), kBlock, 0, x->s>>>(x->D, e, x->fluxes, x->emf);
+  }
+  return checkLaunch("face fluxes");
+}
+
+template <class L>
+int explicitSources(grim_ctx* x, const double* prim, const double* primOld, double* sExp, double* sImp,
+                    double* consOld) {
+  const Box b = bulkBox(x->D);
+  if constexpr (L::EMHD)
+    uTKernel<L><<<blocksFor(x->D.n), kBlock, 0, x->s>>>(x->a, prim, x->geomC, x->uT);
+  const unsigned nb = blocksFor(b.count());
+  switch (x->c.reconstruction) {
+    case GRIM_RECON_WENO5:
+      sourcesKernel<L, GRIM_RECON_WENO5><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, primOld, x->geomC, x->gamma,
+                                                                  x->uT, sExp, sImp, consOld);
+      break;
+    case GRIM_RECON_PPM:
+      sourcesKernel<L, GRIM_RECON_PPM><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, primOld, x->geomC, x->gamma,
+                                                                x->uT, sExp, sImp, consOld);
+      break;
+    default:
+      sourcesKernel<L, GRIM_RECON_MINMOD><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, primOld, x->geomC,
+                                                                   x->gamma, x->uT, sExp, sImp, consOld);
+  }
+  return checkLaunch("sources");
+}
+
+template <class L>
+int computeDt(grim_ctx* x, const double* prim, double dt, double* out) {
+  const unsigned nb = blocksFor(x->D.n);
+  dtKernel<L><<<nb, kBlock, 0, x->s>>>(x->a, prim, x->geomC, x->red);
+  maxReduceKernel<<<1, kBlock, 0, x->s>>>(nb, x->red, x->red + nb);
+  GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf, x->red + nb, sizeof(double), hipMemcpyDeviceToHost, x->s));
+  GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+  const double maxInvDt = x->hostBuf[0];
+  double newDt = x->c.CourantFactor / maxInvDt;
+  if (newDt > x->c.maxDtIncrement * dt) newDt = x->c.maxDtIncrement * dt;
+  *out = newDt;
+  return GRIM_OK;
+}
+
+// One stage of timeStep (timestep.cpp:14-107 half, 183-255 full).
+template <class L>
+int stage(grim_ctx* x, int st, double dt, grim_step_stats* stats) {
+  const Dims& D = x->D;
+  const size_t bytes = sizeof(double) * (size_t)L::DOF * D.n;
+  double* primStage = st == 0 ? x->primOld : x->primHalf;
+  const double dtPrime = st == 0 ? dt / 2. : dt;
+  if (int rc = applyBoundaries(x, primStage, L::DOF, false)) return rc;
+  if (st == 0) GRIM_HIP_TRY(hipMemcpyAsync(x->primDt, x->primOld, bytes, hipMemcpyDeviceToDevice, x->s));
+  if (int rc = explicitSources<L>(x, primStage, x->primOld, x->sExp, L::EMHD ? x->sImp : nullptr,
+                                  st == 0 ? x->consOld : nullptr))
+    return rc;
+  if (int rc = fluxesAndEmf<L>(x, primStage)) return rc;
+  GRIM_HIP_TRY(hipMemcpyAsync(x->primNew, primStage, bytes, hipMemcpyDeviceToDevice, x->s));
+  const Box b = bulkBox(D);
+  foldKernel<L><<<blocksFor(b.count()), kBlock, 0, x->s>>>(x->a, b, x->fluxes, x->emf, x->geomC, x->consOld,
+                                                           x->sExp, x->sImp, dtPrime, nullptr, x->rhs,
+                                                           x->primNew);
+  if (int rc = checkLaunch("fold")) return rc;
+
+  grim_zone_batch zb = {};
+  zb.n = D.n;
+  zb.prim = x->primNew;
+  zb.geom = x->geomC;
+  zb.rhs = x->rhs;
+  zb.prim_old = x->primOld;
+  zb.prim_coef = st == 0 ? x->primOld : x->primHalf;
+  zb.closure = nullptr;
+  zb.radius = x->geomC + (size_t)LOC_R * D.n;
+  zb.residual_mask = x->mask;
+  zb.dt_prime = dtPrime;
+  grim_newton_stats ns = {x->iters, nullptr, x->l2};
+  if (int rc = grim_batched_newton(&x->p, &zb, &x->opts, &ns, x->s)) return rc;
+  if (stats) {
+    GRIM_HIP_TRY(hipMemsetAsync(x->counters, 0, 2 * sizeof(unsigned long long), x->s));
+    stageStatsKernel<<<blocksFor(D.n), kBlock, 0, x->s>>>(D.n, x->mask, x->l2, x->iters,
+                                                          x->p.nonlinearsolve_atol, x->counters);
+    GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf, x->counters, 2 * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, x->s));
+    GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+    unsigned long long c[2];
+    std::memcpy(c, x->hostBuf, sizeof(c));
+    stats->unconverged[st] = (int64_t)c[0];
+    stats->max_iterations[st] = (int32_t)c[1];
+  }
+  if (int rc = communicate(x, x->primNew, L::DOF)) return rc;
+  if (st == 0)
+    std::swap(x->primHalf, x->primNew);
+  else
+    std::swap(x->primOld, x->primNew);
+  return GRIM_OK;
+}
+
+int validConfig(const grim_params* p, const grim_grid_config* c) {
+  if (!validParams(p) || !c) return 0;
+  if (c->dim < 1 || c->dim > 3 || c->numGhost != 3) return 0;
+  if (c->N1 < 1 || c->N2 < 1 || c->N3 < 1) return 0;
+  if (c->dim < 2 && c->N2 != 1) return 0;
+  if (c->dim < 3 && c->N3 != 1) return 0;
+  if (c->metric != GRIM_METRIC_MINKOWSKI && c->metric != GRIM_METRIC_MKS) return 0;
+  const int b[6] = {c->boundaryLeft, c->boundaryRight, c->boundaryBottom, c->boundaryTop, c->boundaryBack,
+                    c->boundaryFront};
+  for (int q = 0; q < 6; q++)
+    if (b[q] < GRIM_BC_PERIODIC || b[q] > GRIM_BC_MIRROR) return 0;
+  if (c->reconstruction < 0 || c->reconstruction > 2 || c->riemannSolver < 0 || c->riemannSolver > 1)
+    return 0;
+  if ((p->conduction || p->viscosity) && p->closure == GRIM_CLOSURE_PER_ZONE) return 0;
+  return 1;
+}
+
+}  // namespace
+}  // namespace grim
+
+extern "C" {
+
+int64_t grim_grid_zones(const grim_grid_config* c) {
+  if (!c) return 0;
+  return makeDims(*c).n;
+}
+
+int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newton_opts* opts,
+                grim_stream_t stream, grim_ctx** out) {
+  GRIM_REQUIRE(out, "grim_create: null output");
+  *out = nullptr;
+  GRIM_REQUIRE(validConfig(p, c), "grim_create: bad params or grid config");
+  if (int rc = checkDevice()) return rc;
+  grim_ctx* x = new (std::nothrow) grim_ctx();
+  GRIM_REQUIRE(x, "grim_create: host allocation failed");
+  x->p = *p;
+  x->c = *c;
+  x->opts = grim_newton_opts{};
+  x->opts.jacobian = GRIM_JAC_FD;
+  x->opts.lu_precision = GRIM_LU_FP64;
+  x->opts.semantics = GRIM_NEWTON_PER_ZONE;
+  if (opts) x->opts = *opts;
+  x->opts.workspace = nullptr;
+  x->opts.workspace_bytes = 0;
+  x->s = (hipStream_t)stream;
+  x->D = makeDims(*c);
+  x->dof = 8 + p->conduction + p->viscosity;
+  x->nf = 5 + p->conduction + p->viscosity;
+  x->a.D = x->D;
+  x->a.prm = makeDevParams(*p);
+  x->a.dX[0] = (c->X1End - c->X1Start) / c->N1;
+  x->a.dX[1] = (c->X2End - c->X2Start) / c->N2;
+  x->a.dX[2] = (c->X3End - c->X3Start) / c->N3;
+  x->a.theta = c->slopeLimTheta;
+  x->a.riemann = c->riemannSolver;
+  x->a.metric = c->metric;
+  x->cg.start[0] = c->X1Start;
+  x->cg.start[1] = c->X2Start;
+  x->cg.start[2] = c->X3Start;
+  for (int d = 0; d < 3; d++) x->cg.dX[d] = x->a.dX[d];
+  x->cm.metric = c->metric;
+  x->cm.derefine = c->DerefineThetaHorizon;
+  x->cm.cylindrify = c->DoCylindrify;
+  x->cm.a = c->blackHoleSpin;
+  x->cm.hSlope = c->hSlope;
+  x->cm.X1Start = c->X1Start;
+  x->cm.X1cyl = c->X1cyl;
+  x->cm.X2cyl = c->X2cyl;
+  x->time = 0.;
+  x->dt = 0.;
+  const int64_t n = x->D.n;
+  const size_t dofB = sizeof(double) * (size_t)x->dof * n;
+  const size_t nfB = sizeof(double) * (size_t)x->nf * n;
+  const size_t locB = sizeof(double) * (size_t)GRIM_LOC_PLANES * n;
+  const unsigned nb = blocksFor(n);
+  struct Req {
+    void** ptr;
+    size_t bytes;
+  } reqs[] = {
+      {(void**)&x->primOld, dofB},
+      {(void**)&x->primHalf, dofB},
+      {(void**)&x->primNew, dofB},
+      {(void**)&x->primDt, dofB},
+      {(void**)&x->consOld, dofB},
+      {(void**)&x->fluxes, dofB * (size_t)c->dim},
+      {(void**)&x->emf, sizeof(double) * 3 * (size_t)n},
+      {(void**)&x->sExp, nfB},
+      {(void**)&x->sImp, nfB},
+      {(void**)&x->uT, sizeof(double) * 5 * (size_t)n},
+      {(void**)&x->rhs, nfB},
+      {(void**)&x->mask, sizeof(double) * (size_t)n},
+      {(void**)&x->geomC, locB},
+      {(void**)&x->geomX1, locB},
+      {(void**)&x->geomX2, locB},
+      {(void**)&x->gamma, sizeof(double) * 64 * (size_t)n},
+      {(void**)&x->red, sizeof(double) * ((size_t)nb + 1)},
+      {(void**)&x->iters, sizeof(int) * (size_t)n},
+      {(void**)&x->l2, sizeof(double) * (size_t)n},
+      {(void**)&x->counters, 2 * sizeof(unsigned long long)},
+  };
+  for (auto& r : reqs) {
+    hipError_t e = hipMalloc(r.ptr, r.bytes);
+    if (e != hipSuccess) {
+      setError(std::string("grim_create: hipMalloc: ") + hipGetErrorString(e));
+      grim_destroy(x);
+      return GRIM_ERR_OOM;
+    }
+  }
+  if (hipHostMalloc((void**)&x->hostBuf, 64) != hipSuccess) {
+    setError("grim_create: hipHostMalloc failed");
+    grim_destroy(x);
+    return GRIM_ERR_OOM;
+  }
+  const struct {
+    void* ptr;
+    size_t bytes;
+  } zeros[] = {{x->primOld, dofB}, {x->fluxes, dofB * (size_t)c->dim}, {x->emf, sizeof(double) * 3 * (size_t)n},
+               {x->sImp, nfB},     {x->uT, sizeof(double) * 5 * (size_t)n}, {x->rhs, nfB}};
+  for (auto& zr : zeros) {
+    hipError_t e = hipMemsetAsync(zr.ptr, 0, zr.bytes, x->s);
+    if (e != hipSuccess) {
+      setError(std::string("grim_create: hipMemsetAsync: ") + hipGetErrorString(e));
+      grim_destroy(x);
+      return GRIM_ERR_HIP;
+    }
+  }
+  geometryKernel<<<nb, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, 0, x->geomC);
+  geometryKernel<<<nb, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, 1, x->geomX1);
+  geometryKernel<<<nb, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, 2, x->geomX2);
+  connectionKernel<<<nb, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, x->gamma);
+  maskKernel<<<nb, kBlock, 0, x->s>>>(x->D, x->mask);
+  if (int rc = checkLaunch("grim_create")) {
+    grim_destroy(x);
+    return rc;
+  }
+  hipError_t e = hipStreamSynchronize(x->s);
+  if (e != hipSuccess) {
+    setError(std::string("grim_create: ") + hipGetErrorString(e));
+    grim_destroy(x);
+    return GRIM_ERR_HIP;
+  }
+  *out = x;
+  return GRIM_OK;
+}
+
+int grim_destroy(grim_ctx* x) {
+  if (!x) return GRIM_OK;
+  if (x->s) (void)hipStreamSynchronize(x->s);
+  void* ptrs[] = {x->primOld, x->primHalf, x->primNew, x->primDt, x->consOld, x->fluxes, x->emf,
+                  x->sExp,    x->sImp,     x->uT,      x->rhs,    x->mask,    x->geomC,  x->geomX1,
+                  x->geomX2,  x->gamma,    x->red,     x->iters,  x->l2,      x->counters};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (x->hostBuf) (void)hipHostFree(x->hostBuf);
+  delete x;
+  return GRIM_OK;
+}
+
+int grim_prim(grim_ctx* x, double** prim_old) {
+  GRIM_REQUIRE(x && prim_old, "grim_prim: null argument");
+  *prim_old = x->primOld;
+  return GRIM_OK;
+}
+
+int grim_geometry(grim_ctx* x, int which, double** geom) {
+  GRIM_REQUIRE(x && geom && which >= 0 && which <= 3, "grim_geometry: bad argument");
+  double* g[4] = {x->geomC, x->geomX1, x->geomX2, x->gamma};
+  *geom = g[which];
+  return GRIM_OK;
+}
+
+int grim_set_prim(grim_ctx* x, const double* src, int on_device) {
+  GRIM_REQUIRE(x && src, "grim_set_prim: null argument");
+  GRIM_HIP_TRY(hipMemcpyAsync(x->primOld, src, sizeof(double) * (size_t)x->dof * x->D.n,
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, x->s));
+  GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+  return GRIM_OK;
+}
+
+int grim_get_prim(grim_ctx* x, double* dst, int on_device) {
+  GRIM_REQUIRE(x && dst, "grim_get_prim: null argument");
+  GRIM_HIP_TRY(hipMemcpyAsync(dst, x->primOld, sizeof(double) * (size_t)x->dof * x->D.n,
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, x->s));
+  GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+  return GRIM_OK;
+}
+
+int grim_get_geometry(grim_ctx* x, int which, double* dst, int on_device) {
+  GRIM_REQUIRE(x && dst && which >= 0 && which <= 3, "grim_get_geometry: bad argument");
+  double* g[4] = {x->geomC, x->geomX1, x->geomX2, x->gamma};
+  const size_t planes = which == 3 ? 64 : GRIM_LOC_PLANES;
+  GRIM_HIP_TRY(hipMemcpyAsync(dst, g[which], sizeof(double) * planes * x->D.n,
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, x->s));
+  GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+  return GRIM_OK;
+}
+
+int grim_set_time(grim_ctx* x, double time, double dt) {
+  GRIM_REQUIRE(x && dt > 0., "grim_set_time: bad argument");
+  x->time = time;
+  x->dt = dt;
+  // primDt = primOld: computeDt of the first step reads the initial state.
+  GRIM_HIP_TRY(hipMemcpyAsync(x->primDt, x->primOld, sizeof(double) * (size_t)x->dof * x->D.n,
+                              hipMemcpyDeviceToDevice, x->s));
+  return GRIM_OK;
+}
+
+int grim_get_time(grim_ctx* x, double* time, double* dt) {
+  GRIM_REQUIRE(x && time && dt, "grim_get_time: null argument");
+  *time = x->time;
+  *dt = x->dt;
+  return GRIM_OK;
+}
+
+int grim_time_step(grim_ctx* x, grim_step_stats* stats) {
+  GRIM_REQUIRE(x && x->dt > 0., "grim_time_step: no context or dt not set (grim_set_time)");
+  return withLayout(x, [&]<class L>() -> int {
+    double dt;
+    if (int rc = computeDt<L>(x, x->primDt, x->dt, &dt)) return rc;
+    if (stats) stats->dt = dt;
+    if (int rc = stage<L>(x, 0, dt, stats)) return rc;
+    if (int rc = stage<L>(x, 1, dt, stats)) return rc;
+    x->dt = dt;
+    x->time += dt;
+    return GRIM_OK;
+  });
+}
+
+int grim_apply_boundaries(grim_ctx* x, double* prim, int nvars) {
+  GRIM_REQUIRE(x && prim && nvars > 0, "grim_apply_boundaries: bad argument");
+  return applyBoundaries(x, prim, nvars, true);
+}
+
+int grim_communicate(grim_ctx* x, double* prim, int nvars) {
+  GRIM_REQUIRE(x && prim && nvars > 0, "grim_communicate: bad argument");
+  return communicate(x, prim, nvars);
+}
+
+int grim_reconstruct(grim_ctx* x, const double* prim, int nvars, int dir, double* left, double* right) {
+  GRIM_REQUIRE(x && prim && left && right && nvars > 0 && dir >= 0 && dir < x->D.dim,
+               "grim_reconstruct: bad argument");
+  Box b;
+  for (int d = 0; d < 3; d++) {
+    b.lo[d] = 0;
+    b.hi[d] = x->D.NT[d];
+  }
+  b.lo[dir] = 2;
+  b.hi[dir] = x->D.NT[dir] - 2;
+  const unsigned nb = blocksFor(b.count());
+  const int64_t s = x->D.s[dir];
+  switch (x->c.reconstruction) {
+    case GRIM_RECON_WENO5:
+      reconstructKernel<GRIM_RECON_WENO5><<<nb, kBlock, 0, x->s>>>(x->D, b, s, x->a.theta, nvars, prim, left, right);
+      break;
+    case GRIM_RECON_PPM:
+      reconstructKernel<GRIM_RECON_PPM><<<nb, kBlock, 0, x->s>>>(x->D, b, s, x->a.theta, nvars, prim, left, right);
+      break;
+    default:
+      reconstructKernel<GRIM_RECON_MINMOD><<<nb, kBlock, 0, x->s>>>(x->D, b, s, x->a.theta, nvars, prim, left, right);
+  }
+  return checkLaunch("grim_reconstruct");
+}
+
+int grim_riemann_flux(grim_ctx* x, const double* prim, int dir, double* flux) {
+  GRIM_REQUIRE(x && prim && flux && dir >= 0 && dir < x->D.dim, "grim_riemann_flux: bad argument");
+  return withLayout(x, [&]<class L>() -> int {
+    if (dir == 0) launchFace<L, 0>(x, prim, flux);
+    else if (dir == 1) launchFace<L, 1>(x, prim, flux);
+    else launchFace<L, 2>(x, prim, flux);
+    return checkLaunch("grim_riemann_flux");
+  });
+}
+
+int grim_div_fluxes(grim_ctx* x, const double* prim, double* div_f, double* fluxes) {
+  GRIM_REQUIRE(x && prim && div_f, "grim_div_fluxes: bad argument");
+  return withLayout(x, [&]<class L>() -> int {
+    if (int rc = fluxesAndEmf<L>(x, prim)) return rc;
+    const Box b = bulkBox(x->D);
+    foldKernel<L><<<blocksFor(b.count()), kBlock, 0, x->s>>>(x->a, b, x->fluxes, x->emf, x->geomC, nullptr,
+                                                             nullptr, nullptr, 0., div_f, nullptr, nullptr);
+    if (fluxes) {
+      GRIM_HIP_TRY(hipMemcpyAsync(fluxes, x->fluxes, sizeof(double) * (size_t)L::DOF * x->D.dim * x->D.n,
+                                  hipMemcpyDeviceToDevice, x->s));
+      if (x->D.dim > 1) {
+        const Box e = bulkBox(x->D);
+        ctWriteKernel<L><<<blocksFor(e.count()), kBlock, 0, x->s>>>(x->D, e, x->fluxes, x->emf, fluxes);
+      }
+    }
+    return checkLaunch("grim_div_fluxes");
+  });
+}
+
+int grim_explicit_sources(grim_ctx* x, const double* prim, double* sources) {
+  GRIM_REQUIRE(x && prim && sources, "grim_explicit_sources: bad argument");
+  return withLayout(x, [&]<class L>() -> int {
+    return explicitSources<L>(x, prim, prim, sources, nullptr, nullptr);
+  });
+}
+
+int grim_compute_dt(grim_ctx* x, const double* prim, double dt, double* new_dt) {
+  GRIM_REQUIRE(x && prim && new_dt && dt > 0., "grim_compute_dt: bad argument");
+  return withLayout(x, [&]<class L>() -> int { return computeDt<L>(x, prim, dt, new_dt); });
+}
+
+int grim_init_linear_mode(grim_ctx* x, int mode, double amplitude) {
+  GRIM_REQUIRE(x && (mode == 0 || mode == 1), "grim_init_linear_mode: bad argument");
+  GRIM_REQUIRE(x->c.metric == GRIM_METRIC_MINKOWSKI, "grim_init_linear_mode: Minkowski grids only");
+  GRIM_REQUIRE(mode == 1 || !(x->p.conduction || x->p.viscosity),
+               "grim_init_linear_mode: the fast mode is an ideal-MHD eigenvector");
+  return withLayout(x, [&]<class L>() -> int {
+    linearModeKernel<L><<<blocksFor(x->D.n), kBlock, 0, x->s>>>(x->a, mode, amplitude, x->geomC, x->cg,
+                                                                 x->primOld);
+    return checkLaunch("grim_init_linear_mode");
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+"""Host-side mirror of grim's ``timeStepper`` (src/timestepper/timestepper.hpp:19-132)
+over the C ABI: one object owns one ``grim_ctx`` (all grid arrays live on the
+device), ``timeStep()`` advances primOld by one predictor/corrector step.
+
+Names follow the reference: ``timeStep``, ``computeDt``, ``primOld``,
+``time``, ``dt``; the stage-level pieces (``applyBoundaryConditions``,
+``reconstruct``, ``riemannFlux``, ``computeDivOfFluxes``,
+``computeExplicitSources``) take and return torch device tensors of shape
+(vars, N3T, N2T, N1T).  There is no CPU path: every call goes to
+``libgrim_hip.so`` and raises ``GrimError`` when it fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+
+
+def grid_config(**kw) -> capi.GridConfig:
+    """A grid config with grim's linear-modes defaults (2D Minkowski, periodic,
+    MINMOD + HLL, Courant 0.9; src/problem/linear_modes/params.cpp:8-64)
+    overridden by keyword."""
+    g = capi.GridConfig(N1=64, N2=64, N3=1, dim=2, numGhost=3, X1Start=0., X1End=1., X2Start=0., X2End=1.,
+                        X3Start=0., X3End=1., metric=capi.METRIC_MINKOWSKI, blackHoleSpin=0.9375, hSlope=0.3,
+                        DerefineThetaHorizon=0, DoCylindrify=0, X1cyl=0., X2cyl=0.,
+                        boundaryLeft=capi.BC_PERIODIC, boundaryRight=capi.BC_PERIODIC,
+                        boundaryBottom=capi.BC_PERIODIC, boundaryTop=capi.BC_PERIODIC,
+                        boundaryBack=capi.BC_PERIODIC, boundaryFront=capi.BC_PERIODIC,
+                        reconstruction=capi.RECON_MINMOD, riemannSolver=capi.RIEMANN_HLL, slopeLimTheta=2.,
+                        CourantFactor=0.9, maxDtIncrement=1.3)
+    for k, v in kw.items():
+        if not hasattr(g, k):
+            raise AttributeError(f"grim_grid_config has no field {k}")
+        setattr(g, k, v)
+    return g
+
+
+class TimeStepper:
+    def __init__(self, params: capi.Params, grid: capi.GridConfig, opts: capi.NewtonOpts | None = None,
+                 stream=None):
+        """Allocate the grid and build its geometry.  Set primOld (or call
+        initLinearMode), then set_time(time, dt) before the first timeStep."""
+        import torch
+
+        self.params, self.grid = params, grid
+        self.dof = params.dof
+        self.shape = grid.shape
+        self.n = grid.n
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        self._stream = stream
+        ctx = C.c_void_p()
+        o = C.byref(opts) if opts is not None else None
+        capi.check(capi.lib().grim_create(C.byref(params), C.byref(grid), o, capi._stream(stream), C.byref(ctx)),
+                   "grim_create")
+        self._ctx = ctx
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_ctx", None) is not None and self._ctx.value:
+            capi.lib().grim_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    # -- state ------------------------------------------------------------
+    @property
+    def primOld(self) -> np.ndarray:
+        out = np.empty((self.dof, self.n))
+        capi.check(capi.lib().grim_get_prim(self._ctx, out.ctypes.data, 0), "grim_get_prim")
+        return out
+
+    @primOld.setter
+    def primOld(self, prim):
+        prim = np.ascontiguousarray(prim, dtype=np.float64).reshape(self.dof, self.n)
+        capi.check(capi.lib().grim_set_prim(self._ctx, prim.ctypes.data, 0), "grim_set_prim")
+
+    def geometry(self, which: int = 0) -> np.ndarray:
+        planes = 64 if which == 3 else capi.LOC_PLANES
+        out = np.empty((planes, self.n))
+        capi.check(capi.lib().grim_get_geometry(self._ctx, which, out.ctypes.data, 0), "grim_get_geometry")
+        return out
+
+    def set_time(self, time: float, dt: float):
+        """time and the previous dt (timeStepper(..., Time, InitialDt, ...));
+        also marks the current primOld as the state computeDt reads."""
+        capi.check(capi.lib().grim_set_time(self._ctx, time, dt), "grim_set_time")
+
+    @property
+    def time(self) -> float:
+        t, d = C.c_double(), C.c_double()
+        capi.check(capi.lib().grim_get_time(self._ctx, C.byref(t), C.byref(d)), "grim_get_time")
+        return t.value
+
+    @property
+    def dt(self) -> float:
+        t, d = C.c_double(), C.c_double()
+        capi.check(capi.lib().grim_get_time(self._ctx, C.byref(t), C.byref(d)), "grim_get_time")
+        return d.value
+
+    def initLinearMode(self, mode: int, amplitude: float):
+        capi.check(capi.lib().grim_init_linear_mode(self._ctx, mode, amplitude), "grim_init_linear_mode")
+
+    # -- the step -----------------------------------------------------------
+    def timeStep(self) -> capi.StepStats:
+        st = capi.StepStats()
+        capi.check(capi.lib().grim_time_step(self._ctx, C.byref(st)), "grim_time_step")
+        return st
+
+    # -- stage-level pieces on torch device tensors ------------------------
+    def _empty(self, planes):
+        import torch
+
+        return torch.full((planes, self.n), float("nan"), dtype=torch.float64, device=self.device)
+
+    def applyBoundaryConditions(self, prim):
+        capi.check(capi.lib().grim_apply_boundaries(self._ctx, capi._ptr(prim), prim.shape[0]),
+                   "grim_apply_boundaries")
+        return prim
+
+    def communicate(self, prim):
+        capi.check(capi.lib().grim_communicate(self._ctx, capi._ptr(prim), prim.shape[0]), "grim_communicate")
+        return prim
+
+    def reconstruct(self, prim, dir: int):
+        left, right = self._empty(prim.shape[0]), self._empty(prim.shape[0])
+        capi.check(capi.lib().grim_reconstruct(self._ctx, capi._ptr(prim), prim.shape[0], dir, capi._ptr(left),
+                                               capi._ptr(right)), "grim_reconstruct")
+        return left, right
+
+    def riemannFlux(self, prim, dir: int):
+        flux = self._empty(self.dof)
+        capi.check(capi.lib().grim_riemann_flux(self._ctx, capi._ptr(prim), dir, capi._ptr(flux)),
+                   "grim_riemann_flux")
+        return flux
+
+    def computeDivOfFluxes(self, prim, want_fluxes: bool = False):
+        div = self._empty(self.dof)
+        fl = self._empty(self.grid.dim * self.dof) if want_fluxes else None
+        capi.check(capi.lib().grim_div_fluxes(self._ctx, capi._ptr(prim), capi._ptr(div), capi._ptr(fl)),
+                   "grim_div_fluxes")
+        return (div, fl) if want_fluxes else div
+
+    def computeExplicitSources(self, prim):
+        src = self._empty(self.params.nf)
+        capi.check(capi.lib().grim_explicit_sources(self._ctx, capi._ptr(prim), capi._ptr(src)),
+                   "grim_explicit_sources")
+        return src
+
+    def computeDt(self, prim, dt: float) -> float:
+        out = C.c_double()
+        capi.check(capi.lib().grim_compute_dt(self._ctx, capi._ptr(prim), dt, C.byref(out)), "grim_compute_dt")
+        return out.value

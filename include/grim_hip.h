@@ -190,6 +190,121 @@ int grim_generate_kerr_schild_batch(const grim_params* p, int64_t n,
 int grim_known_root_rhs(const grim_params* p, const grim_zone_batch* b,
                         double* rhs, double* mask, grim_stream_t stream);
 
+/* ------------------------------------------------------------------------
+ * The full time step on a grid (grim's timeStepper).
+ *
+ * A context owns every device array of one grid: the primitive variables
+ * (grim's grid::vars planes, N1T*N2T*N3T doubles each, index
+ * i + N1T*(j + N2T*k), numGhost ghost zones per side of each active
+ * dimension), the geometry and the work arrays.  All work is enqueued on
+ * the context's stream.
+ *
+ * Geometry of one location: GRIM_LOC_PLANES planes
+ *   0..9   gCov 00 01 02 03 11 12 13 22 23 33
+ *   10..13 gCon 00 01 02 03,  14 alpha,  15 g = sqrt(-det gCov)
+ *   16..21 gCon 11 12 13 22 23 33,  22 r (xCoords[X1])
+ * (planes 0..15 are the batched-zone layout).  The connection at the zone
+ * centre has 64 planes, Gamma^l_{k n} at plane l*16 + k*4 + n.
+ * ------------------------------------------------------------------------ */
+#define GRIM_LOC_PLANES 23
+
+typedef enum grim_metric { GRIM_METRIC_MINKOWSKI = 0, GRIM_METRIC_MKS = 1 } grim_metric;
+/* src/params.hpp:48-54 (DIRICHLET is not supported) */
+typedef enum grim_boundary {
+  GRIM_BC_PERIODIC = 0, GRIM_BC_OUTFLOW = 1, GRIM_BC_MIRROR = 2
+} grim_boundary;
+/* src/params.hpp:72-86 */
+typedef enum grim_reconstruction {
+  GRIM_RECON_MINMOD = 0, GRIM_RECON_WENO5 = 1, GRIM_RECON_PPM = 2
+} grim_reconstruction;
+typedef enum grim_riemann { GRIM_RIEMANN_HLL = 0, GRIM_RIEMANN_LLF = 1 } grim_riemann;
+
+/* Grid, geometry, boundary and scheme members of grim's `namespace params`
+ * (e.g. src/problem/torus/params.cpp:8-105). */
+typedef struct grim_grid_config {
+  int N1, N2, N3, dim, numGhost;
+  double X1Start, X1End, X2Start, X2End, X3Start, X3End;
+  int metric; /* grim_metric */
+  double blackHoleSpin, hSlope;
+  int DerefineThetaHorizon, DoCylindrify;
+  double X1cyl, X2cyl;
+  int boundaryLeft, boundaryRight, boundaryBottom, boundaryTop, boundaryBack,
+      boundaryFront; /* grim_boundary */
+  int reconstruction; /* grim_reconstruction */
+  int riemannSolver;  /* grim_riemann */
+  double slopeLimTheta;
+  double CourantFactor, maxDtIncrement;
+} grim_grid_config;
+
+typedef struct grim_ctx grim_ctx;
+
+typedef struct grim_step_stats {
+  double dt;                  /* dt taken by the step */
+  int64_t unconverged[2];     /* bulk zones above nonlinearsolve_atol after the half / full stage */
+  int32_t max_iterations[2];  /* largest per-zone Newton iteration count of each stage */
+} grim_step_stats;
+
+/* Number of zones (N1T*N2T*N3T) of a grid. */
+int64_t grim_grid_zones(const grim_grid_config* c);
+
+/* timeStepper::timeStepper (src/timestepper/timestepper.cpp:3-386 without
+ * the initial conditions): allocates the grid arrays, builds the geometry at
+ * the zone centres and lower faces and the connection (geometry.cpp:4-121).
+ * opts selects the Newton path of the two solves (NULL: grim's FD-FP64,
+ * per-zone convergence).  The primitive variables start at zero. */
+int grim_create(const grim_params* p, const grim_grid_config* c,
+                const grim_newton_opts* opts, grim_stream_t stream, grim_ctx** ctx);
+int grim_destroy(grim_ctx* ctx);
+
+/* Device pointer of primOld (dof planes of grim_grid_zones doubles), and
+ * of a geometry array: which = 0 centre, 1 X1 faces (LEFT), 2 X2 faces
+ * (BOTTOM), 3 connection (64 planes). */
+int grim_prim(grim_ctx* ctx, double** prim_old);
+int grim_geometry(grim_ctx* ctx, int which, double** geom);
+/* Copies of primOld and of a geometry array to / from host (on_device = 0)
+ * or device (on_device = 1) memory; synchronise the stream (grim's
+ * grid::setVars / getVars, src/grid/gridPy.pyx:185-230). */
+int grim_set_prim(grim_ctx* ctx, const double* src, int on_device);
+int grim_get_prim(grim_ctx* ctx, double* dst, int on_device);
+int grim_get_geometry(grim_ctx* ctx, int which, double* dst, int on_device);
+/* time and dt of the timeStepper (dt is the previous step's, which bounds
+ * the next one through maxDtIncrement). */
+int grim_set_time(grim_ctx* ctx, double time, double dt);
+int grim_get_time(grim_ctx* ctx, double* time, double* dt);
+
+/* timeStepper::timeStep (src/timestepper/timestep.cpp:3-334): computeDt,
+ * then the half and full stages (boundaries, conserved variables, sources,
+ * divergence of fluxes with flux-CT, induction, Newton solve). */
+int grim_time_step(grim_ctx* ctx, grim_step_stats* stats);
+
+/* The pieces of the step on caller-provided device arrays (dof planes each):
+ *  boundaries::applyBoundaryConditions after grid::communicate
+ *    (src/boundary/boundary.cpp:5-243, src/grid/grid.cpp:306-315);
+ *  grid::communicate alone (periodic ghost fill);
+ *  reconstruction::reconstruct (src/reconstruction/reconstruction.cpp:3-38);
+ *  riemannSolver::solve with reconstruction (src/physics/riemannsolver.cpp:202-328):
+ *    flux at the lower face of each zone along dir;
+ *  timeStepper::computeDivOfFluxes with fluxCT (src/timestepper/fvmfluxes.cpp:3-184,
+ *    constrainedtransport.cpp:3-83): divF over the bulk, fluxes (dim*dof planes) may be NULL;
+ *  fluidElement::computeExplicitSources (src/physics/physics.cpp:360-586) over the bulk;
+ *  timeStepper::computeDt (src/timestepper/timestep.cpp:336-400). */
+int grim_apply_boundaries(grim_ctx* ctx, double* prim, int nvars);
+int grim_communicate(grim_ctx* ctx, double* prim, int nvars);
+int grim_reconstruct(grim_ctx* ctx, const double* prim, int nvars, int dir,
+                     double* left, double* right);
+int grim_riemann_flux(grim_ctx* ctx, const double* prim, int dir, double* flux);
+int grim_div_fluxes(grim_ctx* ctx, const double* prim, double* div_f, double* fluxes);
+int grim_explicit_sources(grim_ctx* ctx, const double* prim, double* sources);
+int grim_compute_dt(grim_ctx* ctx, const double* prim, double dt, double* new_dt);
+
+/* Initial conditions of the linear-mode problem
+ * (src/problem/linear_modes/problem.cpp:10-111) into primOld:
+ * mode 0 = ideal fast magnetosonic wave, k = (2 pi, 2 pi), on rho = 1,
+ * u = 2, B = (0.1, 0.3, 0) (eigenvector derived from the linearised
+ * equations; not in the reference); mode 1 = the reference's EMHD mode,
+ * k = (2 pi, 4 pi).  amplitude multiplies the eigenvector. */
+int grim_init_linear_mode(grim_ctx* ctx, int mode, double amplitude);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,297 @@
+"""GPU parity of the full time step (grim_amd/csrc/step.hip) against the
+oracle's restatement (oracle/grim_step.cpp), through the C ABI.
+
+Bar: bit-identical FP64 results for every piece (boundaries, reconstruction,
+Riemann fluxes, flux-CT divergence, explicit sources, computeDt) and for
+whole FD-FP64 time steps, given the same geometry and the same initial state.
+The geometry itself calls transcendental functions (exp, sin, acos, ...) whose
+device and host implementations may differ by an ulp: it is compared to a
+tolerance, and for Kerr-Schild grids the GPU geometry is handed to the oracle
+so that the step comparison stays bitwise.  The torus closure (pow, exp in
+every residual) makes EMHD torus steps agree to a tolerance instead.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FAST = json.load(open(os.path.join(HERE, "golden", "fast_mode.json")))
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+
+    assert torch.cuda.is_available()
+    return torch
+
+
+def make(cfg_kw, emhd=False, closure=None, opts=None, **pkw):
+    from grim_amd import capi
+    from grim_amd.timestepper import TimeStepper, grid_config
+
+    cfg = grid_config(**cfg_kw)
+    p = capi.default_params(emhd)
+    po = O.default_params(emhd)
+    for k, v in pkw.items():
+        setattr(p, k, v)
+        setattr(po, k, v)
+    if closure is not None:
+        p.closure = closure
+        po.closure = closure
+    ts = TimeStepper(p, cfg, opts=opts)
+    g = O.grid(**{f: getattr(cfg, f) for f, _ in cfg._fields_})
+    return ts, g, p, po
+
+
+def oracle_geometry(ts, g):
+    return O.Geometry(g, center=ts.geometry(0), faceX1=ts.geometry(1), faceX2=ts.geometry(2),
+                      gamma=ts.geometry(3))
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def bulk(g, a):
+    return a.reshape(a.shape[0], *g.shape)[g.bulk()]
+
+
+def assert_bitwise(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    same = (a == b) | (np.isnan(a) & np.isnan(b))
+    if not same.all():
+        idx = np.argwhere(~same)[:5]
+        raise AssertionError(f"{what}: {(~same).sum()} of {same.size} differ, first {idx.tolist()}: "
+                             f"{[(a[tuple(i)], b[tuple(i)]) for i in idx]}")
+
+
+MKS3D = dict(N1=16, N2=12, N3=8, dim=3, metric=1, X1Start=np.log(1.5), X1End=np.log(20.), X2Start=0.1,
+             X2End=0.9, X3Start=0., X3End=2 * np.pi, boundaryLeft=1, boundaryRight=1, boundaryBottom=2,
+             boundaryTop=2, boundaryBack=0, boundaryFront=0)
+TORUS_GEOM = dict(N1=96, N2=32, N3=4, dim=3, metric=1, X1Start=np.log(0.85 * (1 + np.sqrt(1 - 0.9375 ** 2))),
+                  X1End=np.log(3000.), X2Start=1e-8, X2End=1 - 1e-8, X3Start=0., X3End=2 * np.pi,
+                  DerefineThetaHorizon=1, DoCylindrify=1)
+
+
+def mks_state(g, p, seed=0):
+    """A smooth, bound, magnetised state on a Kerr-Schild grid."""
+    rng = np.random.default_rng(seed)
+    x = O.xcoords(g)
+    r, th, ph = x
+    P = np.zeros((p.dof, g.n))
+    P[0] = r ** -1.5 * (1 + 0.05 * np.sin(3 * th + ph))
+    P[1] = 0.1 * r ** -2.5 * (1 + 0.05 * np.cos(2 * th - ph))
+    P[2] = 0.02 * np.sin(th) * np.cos(ph)
+    P[3] = 0.01 * np.cos(2 * th)
+    P[4] = 0.05 * r ** -1.5
+    if p.nf > 5:
+        P[5:p.nf] = 1e-3 * rng.standard_normal((p.nf - 5, 1)) * np.sin(th + ph)
+    P[p.nf] = 0.1 * r ** -2
+    P[p.nf + 1] = 0.01 * np.sin(th) / r ** 2
+    P[p.nf + 2] = 0.02 * r ** -1.5
+    return P
+
+
+# ---------------------------------------------------------------------------
+def test_minkowski_geometry_bitwise(torch):
+    ts, g, p, po = make(dict(N1=12, N2=10))
+    gm = O.Geometry(g)
+    assert_bitwise(ts.geometry(0), gm.center, "centre")
+    assert_bitwise(ts.geometry(1), gm.faceX1, "faceX1")
+    assert_bitwise(ts.geometry(3), gm.gamma, "gamma")
+
+
+@pytest.mark.parametrize("kw", [MKS3D, TORUS_GEOM], ids=["mks", "torus"])
+def test_kerr_schild_geometry_matches_oracle(torch, kw):
+    ts, g, p, po = make(kw)
+    gm = O.Geometry(g)
+    for which, ref in ((0, gm.center), (1, gm.faceX1), (2, gm.faceX2)):
+        got = ts.geometry(which)
+        scale = np.abs(ref).max(axis=1, keepdims=True) + 1e-300
+        err = np.abs(got - ref) / scale
+        print(f"geometry {which}: max scaled diff {err.max():.2e}")
+        assert err.max() < 1e-12
+    got = ts.geometry(3)
+    err = np.abs(got - gm.gamma).max() / np.abs(gm.gamma).max()
+    print(f"connection: max scaled diff {err:.2e}")
+    assert err < 1e-8
+
+
+def test_boundaries_bitwise(torch):
+    ts, g, p, po = make(MKS3D)
+    rng = np.random.default_rng(1)
+    prim = rng.standard_normal((p.dof, g.n))
+    got = ts.applyBoundaryConditions(dev(torch, prim)).cpu().numpy()
+    assert_bitwise(got, O.apply_boundaries(g, prim), "boundaries")
+
+
+@pytest.mark.parametrize("recon", [0, 1, 2])
+def test_reconstruction_bitwise(torch, recon):
+    ts, g, p, po = make(dict(MKS3D, reconstruction=recon))
+    prim = mks_state(g, po)
+    for d in range(3):
+        l, r = ts.reconstruct(dev(torch, prim), d)
+        lo, ro = O.reconstruct(g, prim, d)
+        assert_bitwise(l.cpu().numpy(), lo, f"left d={d}")
+        assert_bitwise(r.cpu().numpy(), ro, f"right d={d}")
+
+
+CASES = [
+    dict(cfg=dict(N1=16, N2=16, reconstruction=1, riemannSolver=1), emhd=False),
+    dict(cfg=dict(N1=16, N2=16, reconstruction=0, riemannSolver=0), emhd=True),
+    dict(cfg=dict(MKS3D, reconstruction=1, riemannSolver=1), emhd=False),
+    dict(cfg=dict(MKS3D, reconstruction=2, riemannSolver=0), emhd=True),
+    dict(cfg=dict(MKS3D, reconstruction=0, riemannSolver=1), emhd=True),
+]
+IDS = ["mink-weno-llf", "mink-emhd-mm-hll", "mks-weno-llf", "mks-emhd-ppm-hll", "mks-emhd-mm-llf"]
+
+
+def case_state(g, po, cfg):
+    if cfg.get("metric", 0) == 1:
+        return O.apply_boundaries(g, mks_state(g, po))
+    rng = np.random.default_rng(2)
+    x = O.xcoords(g)
+    P = np.zeros((po.dof, g.n))
+    base = [1., 2., 0.1, -0.05, 0.02] + [0.01] * (po.nf - 5) + [0.1, 0.3, 0.05]
+    for v in range(po.dof):
+        a, b = rng.standard_normal(2)
+        P[v] = base[v] + 0.05 * abs(base[v]) * np.sin(2 * np.pi * (x[0] + a)) * np.cos(2 * np.pi * (x[1] + b))
+    return O.apply_boundaries(g, P)
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_riemann_and_divergence_bitwise(torch, case):
+    ts, g, p, po = make(case["cfg"], emhd=case["emhd"])
+    gm = oracle_geometry(ts, g)
+    prim = case_state(g, po, case["cfg"])
+    for d in range(g.dim):
+        got = ts.riemannFlux(dev(torch, prim), d).cpu().numpy()
+        ref = O.riemann_flux(g, po, gm, prim, d)
+        ok = ~np.isnan(ref[0])
+        assert ok.sum() > 0
+        assert_bitwise(got[:, ok], ref[:, ok], f"flux d={d}")
+    div = ts.computeDivOfFluxes(dev(torch, prim)).cpu().numpy()
+    assert_bitwise(bulk(g, div), bulk(g, O.div_fluxes(g, po, gm, prim)), "divF")
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_explicit_sources_and_dt_bitwise(torch, case):
+    ts, g, p, po = make(case["cfg"], emhd=case["emhd"])
+    gm = oracle_geometry(ts, g)
+    prim = case_state(g, po, case["cfg"])
+    got = ts.computeExplicitSources(dev(torch, prim)).cpu().numpy()
+    ref = O.explicit_sources(g, po, gm, prim)[:po.nf]
+    assert_bitwise(bulk(g, got), bulk(g, ref), "explicit sources")
+    if case["cfg"].get("metric", 0) == 1:
+        assert np.abs(bulk(g, ref)[:5]).max() > 0  # the connection term is exercised
+    assert ts.computeDt(dev(torch, prim), 1e3) == O.compute_dt(g, po, gm, prim, 1e3)
+
+
+def test_linear_mode_initial_conditions(torch):
+    ts, g, p, po = make(dict(N1=32, N2=32))
+    ts.initLinearMode(0, 1e-4)
+    x = O.xcoords(g)
+    r = np.array(FAST["eigenvector"])
+    r[np.abs(r) < 1e-20] = 0.
+    want = np.array(FAST["background"])[:, None] + 1e-4 * r[:, None] * np.cos(
+        FAST["k"][0] * x[0] + FAST["k"][1] * x[1])[None, :]
+    np.testing.assert_allclose(ts.primOld, want, rtol=0, atol=1e-15)
+    ts2, g2, p2, po2 = make(dict(N1=32, N2=32), emhd=True)
+    ts2.initLinearMode(1, 1e-8)
+    from test_step_oracle import emhd_mode_ic
+
+    np.testing.assert_allclose(ts2.primOld, emhd_mode_ic(g2, po2), rtol=1e-15, atol=1e-22)
+
+
+def run_pair(ts, g, po, gm, steps, semantics=O.NEWTON_PER_ZONE, check_each=True, dt0=None):
+    """Steps the GPU context and the oracle from the same state, comparing the
+    bulk bitwise after every step.  Returns the last stats."""
+    prim0 = ts.primOld
+    dt0 = dt0 if dt0 is not None else 0.5 / g.N1
+    ts.set_time(0., dt0)
+    sim = O.Simulation(g, po, prim0, dt=dt0, gm=gm)
+    for s in range(steps):
+        st = ts.timeStep()
+        so = sim.step(semantics)
+        print(f"step {s}: dt {st.dt:.6e} unconverged {list(st.unconverged)} iters {list(st.max_iterations)}",
+              flush=True)
+        assert st.dt == so.dt
+        assert list(st.unconverged) == list(so.unconverged)
+        if check_each or s == steps - 1:
+            assert_bitwise(bulk(g, ts.primOld), bulk(g, sim.prim), f"primOld after step {s}")
+    assert ts.time == sim.time.value
+    return st
+
+
+@pytest.mark.parametrize("semantics", [0, 1], ids=["per_zone", "global"])
+def test_fast_mode_steps_bitwise(torch, semantics):
+    """The 64x64 linear-wave workload: WENO5 + LLF, FD-FP64 Newton."""
+    from grim_amd import capi
+
+    opts = capi.newton_opts(semantics=semantics)
+    ts, g, p, po = make(dict(N1=64, N2=64, reconstruction=1, riemannSolver=1), opts=opts,
+                        maxNonLinearIter=10)
+    ts.initLinearMode(0, 1e-4)
+    run_pair(ts, g, po, O.Geometry(g), 10, semantics=semantics)
+
+
+def test_emhd_mode_steps_bitwise(torch):
+    """The reference's EMHD linear mode (MINMOD + HLL, grim's default solver settings)."""
+    ts, g, p, po = make(dict(N1=32, N2=32), emhd=True)
+    ts.initLinearMode(1, 1e-8)
+    run_pair(ts, g, po, O.Geometry(g), 6, dt0=0.5 / 288)
+
+
+@pytest.mark.parametrize("emhd", [False, True])
+def test_kerr_schild_steps_bitwise(torch, emhd):
+    """3D MKS grid with outflow, mirror and periodic boundaries (LINEAR_MODES
+    closure for EMHD: no transcendental functions inside the residual)."""
+    ts, g, p, po = make(dict(MKS3D, reconstruction=1, riemannSolver=1), emhd=emhd, maxNonLinearIter=6)
+    gm = oracle_geometry(ts, g)
+    ts.primOld = mks_state(g, po, 3)
+    run_pair(ts, g, po, gm, 2, dt0=1e-3)
+
+
+def test_torus_closure_steps_match(torch):
+    """EMHD with the torus closure: pow/exp inside every residual evaluation
+    may differ by an ulp between device and host, so the bar is relative."""
+    ts, g, p, po = make(dict(MKS3D, reconstruction=1, riemannSolver=1), emhd=True, closure=2,
+                        maxNonLinearIter=6)
+    gm = oracle_geometry(ts, g)
+    ts.primOld = mks_state(g, po, 4)
+    ts.set_time(0., 1e-3)
+    sim = O.Simulation(g, po, ts.primOld, dt=1e-3, gm=gm)
+    for _ in range(2):
+        st = ts.timeStep()
+        sim.step()
+    a, b = bulk(g, ts.primOld), bulk(g, sim.prim)
+    rel = np.abs(a - b) / (np.abs(b) + 1e-30)
+    print(f"torus closure: max rel diff {rel.max():.2e}, unconverged {list(st.unconverged)}")
+    assert np.percentile(rel, 99) < 1e-12 and rel[:p.nf].max() < 1e-8
+
+
+def test_mixed_newton_step_close_to_fd(torch):
+    """The step with the analytic-Jacobian FP32-LU Newton path stays within the
+    solver tolerance of the FD-FP64 step."""
+    from grim_amd import capi
+
+    kw = dict(N1=64, N2=64, reconstruction=1, riemannSolver=1)
+    fd, g, p, po = make(kw, maxNonLinearIter=10)
+    mixed, *_ = make(kw, opts=capi.newton_opts(jacobian=capi.JAC_ANALYTIC, lu_precision=capi.LU_FP32),
+                     maxNonLinearIter=10)
+    for ts in (fd, mixed):
+        ts.initLinearMode(0, 1e-4)
+        ts.set_time(0., 0.5 / 64)
+    for _ in range(10):
+        a, b = fd.timeStep(), mixed.timeStep()
+        assert b.unconverged[0] == 0 and b.unconverged[1] == 0
+    d = np.abs(bulk(g, fd.primOld) - bulk(g, mixed.primOld)).max()
+    print(f"mixed vs FD after 10 steps: max |diff| {d:.2e}")
+    assert d < 1e-10
