@@ -122,7 +122,7 @@ RECON_WENO5 = 1
 RECON_PPM = 2
 RIEMANN_HLL = 0
 RIEMANN_LLF = 1
-LOC_PLANES = 23
+LOC_PLANES = 24
 
 
 class GridConfig(C.Structure):
@@ -156,6 +156,31 @@ class GridConfig(C.Structure):
 
 class StepStats(C.Structure):
     _fields_ = [("dt", C.c_double), ("unconverged", C.c_int64 * 2), ("max_iterations", C.c_int32 * 2)]
+
+
+class TorusParams(C.Structure):
+    """grim_torus_params (src/problem/torus/params.cpp:38-80)."""
+
+    _fields_ = [
+        ("InnerEdgeRadius", C.c_double), ("PressureMaxRadius", C.c_double), ("Adiabat", C.c_double),
+        ("MinPlasmaBeta", C.c_double), ("MagneticLoops", C.c_int), ("UseMADdisk", C.c_int),
+        ("InitialPerturbationAmplitude", C.c_double), ("MaxLorentzFactor", C.c_double),
+        ("RhoFloorAmpl", C.c_double), ("UFloorAmpl", C.c_double), ("RhoFloorSlope", C.c_double),
+        ("UFloorSlope", C.c_double), ("BsqrOverRhoMax", C.c_double), ("BsqrOverUMax", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+def torus_params(**kw) -> TorusParams:
+    """SANE single-loop torus: r_in = 6, r_max = 12, kappa = 1e-3, beta_min = 100,
+    4% internal-energy noise, grim's floors; noise seed 20261202."""
+    t = TorusParams(InnerEdgeRadius=6., PressureMaxRadius=12., Adiabat=1e-3, MinPlasmaBeta=100., MagneticLoops=1,
+                    UseMADdisk=0, InitialPerturbationAmplitude=4e-2, MaxLorentzFactor=10., RhoFloorAmpl=1e-3,
+                    UFloorAmpl=1e-5, RhoFloorSlope=-2.5, UFloorSlope=-2.5, BsqrOverRhoMax=10., BsqrOverUMax=500.,
+                    seed=20261202)
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
 
 
 # Every symbol include/grim_hip.h declares, with its ctypes signature.
@@ -197,6 +222,9 @@ SIGNATURES = {
     "grim_explicit_sources": (_i, [_vp, _vp, _vp]),
     "grim_compute_dt": (_i, [_vp, _vp, _d, C.POINTER(_d)]),
     "grim_init_linear_mode": (_i, [_vp, _i, _d]),
+    "grim_init_torus": (_i, [_vp, C.POINTER(TorusParams)]),
+    "grim_apply_floor": (_i, [_vp, _vp]),
+    "grim_torus_bcs": (_i, [_vp, _vp]),
 }
 
 _lib = None

@@ -25,7 +25,9 @@
 
 #include "grim_hip.h"
 #include "runtime.hpp"
+#include "batchgen.hpp"
 #include "step_physics.hpp"
+#include "torus.hpp"
 #include "zone_physics.hpp"
 
 struct grim_ctx;
@@ -42,6 +44,7 @@ struct StepArgs {
   double theta;
   int riemann;
   int metric;
+  int fluxFilter;  // torus: applyProblemSpecificFluxFilter
 };
 
 __device__ __forceinline__ int64_t flatIndex(const Dims& D, int i, int j, int k) {
@@ -89,6 +92,7 @@ __global__ __launch_bounds__(kBlock) void geometryKernel(Dims D, CoordMap cm, Co
 #pragma unroll
   for (int q = 0; q < 6; q++) out[(16 + q) * n + z] = C[cov[4 + q][0]][cov[4 + q][1]];
   out[LOC_R * n + z] = x[0];
+  out[LOC_THETA * n + z] = x[1];
 }
 
 __global__ __launch_bounds__(kBlock) void connectionKernel(Dims D, CoordMap cm, CoordGrid cg,
@@ -152,32 +156,34 @@ struct BcTypes {
   int lo[3], hi[3];
 };
 
-// grid::communicate on one rank: ghost zones whose out-of-range coordinates
-// all lie along periodic directions take the periodic image (DMDA box
-// stencil, grid.cpp:88-121, 306-315).
+// grid::communicate on one rank (grid.cpp:88-121, 306-360): ghost zones
+// whose out-of-range coordinates all lie along periodic directions take the
+// periodic image (DMDA box stencil); the others lie outside the global
+// domain, which DMGlobalToLocal never writes, and read back the local
+// vector's initial zeros.  Reads only bulk zones, so zones are independent.
 __global__ __launch_bounds__(kBlock) void communicateKernel(Dims D, BcTypes bc, int nvars,
                                                             double* __restrict__ prim) {
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= D.n) return;
   int I[3] = {(int)(z % D.NT[0]), (int)((z / D.NT[0]) % D.NT[1]),
               (int)(z / ((int64_t)D.NT[0] * D.NT[1]))};
-  bool ghost = false;
+  bool ghost = false, outside = false;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     if (D.g[d] == 0) continue;
     if (I[d] < D.g[d]) {
-      if (bc.lo[d] != GRIM_BC_PERIODIC) return;
+      if (bc.lo[d] != GRIM_BC_PERIODIC) outside = true;
       I[d] += D.N[d];
       ghost = true;
     } else if (I[d] >= D.g[d] + D.N[d]) {
-      if (bc.hi[d] != GRIM_BC_PERIODIC) return;
+      if (bc.hi[d] != GRIM_BC_PERIODIC) outside = true;
       I[d] -= D.N[d];
       ghost = true;
     }
   }
   if (!ghost) return;
   const int64_t w = flatIndex(D, I[0], I[1], I[2]);
-  for (int v = 0; v < nvars; v++) prim[v * D.n + z] = prim[v * D.n + w];
+  for (int v = 0; v < nvars; v++) prim[v * D.n + z] = outside ? 0. : prim[v * D.n + w];
 }
 
 // One ghost slab of boundaries::applyBoundaryConditions (boundary.cpp:32-239):
@@ -346,11 +352,25 @@ __device__ __forceinline__ double ctFlux(const Dims& D, const double* __restrict
 template <class L>
 __device__ __forceinline__ double divergence(const Dims& D, const double* dX,
                                              const double* __restrict__ F,
-                                             const double* __restrict__ emf, int v, int64_t z) {
+                                             const double* __restrict__ emf, int v, int64_t z,
+                                             int i, int j, int filter) {
   double acc = 0.;
   for (int d = 0; d < D.dim; d++) {
-    const double f0 = ctFlux<L>(D, F, emf, d, v, z);
-    const double f1 = ctFlux<L>(D, F, emf, d, v, z + D.s[d]);
+    double f0 = ctFlux<L>(D, F, emf, d, v, z);
+    double f1 = ctFlux<L>(D, F, emf, d, v, z + D.s[d]);
+    if (filter) {
+      // The torus's flux filter (problem.cpp:1243-1285), after flux-CT: no
+      // mass inflow through the radial boundary faces, nothing through the
+      // polar faces.
+      if (d == 0 && v == RHO) {
+        if (i == D.g[0]) f0 = stdmin(f0, 0.);
+        if (i + 1 == D.N[0] + D.g[0]) f1 = stdmax(f1, 0.);
+      }
+      if (d == 1) {
+        if (j == D.g[1]) f0 = 0.;
+        if (j + 1 == D.N[1] + D.g[1]) f1 = 0.;
+      }
+    }
     double term;
     if (D.dim < 3) {
       const double w = 1. / dX[d];
@@ -384,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void foldKernel(StepArgs a, Box box, const 
   const int64_t n = D.n, z = flatIndex(D, i, j, k);
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) {
-    const double div = divergence<L>(D, a.dX, F, emf, v, z);
+    const double div = divergence<L>(D, a.dX, F, emf, v, z, i, j, a.fluxFilter);
     if (divOut) divOut[v * n + z] = div;
     if (!rhs) continue;
     const double U = consOld[v * n + z];
@@ -700,6 +720,308 @@ __global__ __launch_bounds__(kBlock) void linearModeKernel(StepArgs a, int mode,
   for (int v = 0; v < L::DOF; v++) prim[v * n + z] = P[v];
 }
 
+// ---------------------------------------------------------------------------
+// Torus problem (src/problem/torus/problem.cpp)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void zoneIJK(const Dims& D, int64_t z, int& i, int& j, int& k) {
+  i = (int)(z % D.NT[0]);
+  j = (int)((z / D.NT[0]) % D.NT[1]);
+  k = (int)(z / ((int64_t)D.NT[0] * D.NT[1]));
+}
+
+// Floor values pow(r, slope) * ampl of every zone (applyFloor, :548-549).
+__global__ __launch_bounds__(kBlock) void floorPlanesKernel(Dims D, grim_torus_params t,
+                                                            const double* __restrict__ geomC,
+                                                            double* __restrict__ floors) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= D.n) return;
+  const double Radius = geomC[LOC_R * D.n + z];
+  floors[z] = pow(Radius, t.RhoFloorSlope) * t.RhoFloorAmpl;
+  floors[D.n + z] = pow(Radius, t.UFloorSlope) * t.UFloorAmpl;
+}
+
+// The fluid part of the initial conditions at every zone (:175-322).
+template <class L>
+__global__ __launch_bounds__(kBlock) void torusICKernel(StepArgs a, grim_torus_params t, double aBH,
+                                                        double hSlope, CoordGrid cg,
+                                                        const double* __restrict__ geomC,
+                                                        double* __restrict__ prim) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const Dims& D = a.D;
+  const int64_t n = D.n;
+  if (z >= n) return;
+  int i, j, k;
+  zoneIJK(D, z, i, j, k);
+  const double X2 = cg.start[1] + ((double)j - D.g[1] + 0.5) * cg.dX[1];
+  const double beta[4] = {0., geomC[11 * n + z], geomC[12 * n + z], geomC[13 * n + z]};
+  double P[L::DOF];
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) P[v] = 0.;
+  torusZone(t, aBH, hSlope, a.prm.gam, a.prm.rhoFloor, a.prm.uFloor, geomC[LOC_R * n + z],
+            geomC[LOC_THETA * n + z], X2, geomC[GEOM_ALPHA * n + z], beta, uniform01(t.seed, (uint64_t)z, 0), P);
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) prim[v * n + z] = P[v];
+}
+
+// Per-block maximum of sign * plane (a plane minimum with sign = -1).
+__global__ __launch_bounds__(kBlock) void planeMaxKernel(int64_t n, const double* __restrict__ in, double sign,
+                                                         double* __restrict__ blockMax) {
+  __shared__ double red[kBlock];
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  red[threadIdx.x] = z < n ? sign * in[z] : -INFINITY;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = stdmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blockMax[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kBlock) void normalizeKernel(int64_t n, const double* __restrict__ rhoMax,
+                                                          double* __restrict__ prim) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= n) return;
+  const double m = rhoMax[0];
+  prim[RHO * n + z] = prim[RHO * n + z] / m;
+  prim[UU * n + z] = prim[UU * n + z] / m;
+}
+
+// Vector potential from the neighbour-averaged density (af::shift wraps
+// around the grid), :355-380.
+__global__ __launch_bounds__(kBlock) void vectorPotentialKernel(Dims D, grim_torus_params t,
+                                                                const double* __restrict__ geomC,
+                                                                const double* __restrict__ rho,
+                                                                double* __restrict__ Avec) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= D.n) return;
+  int I[3];
+  zoneIJK(D, z, I[0], I[1], I[2]);
+  auto at = [&](int d, int off) {
+    int J[3] = {I[0], I[1], I[2]};
+    J[d] = (J[d] + off + D.NT[d]) % D.NT[d];
+    return rho[flatIndex(D, J[0], J[1], J[2])];
+  };
+  double rhoAvg = at(0, -1) + at(0, 1) + at(1, -1) + at(1, 1);
+  if (D.dim > 2)
+    rhoAvg = (rhoAvg + at(2, -1) + at(2, 1)) / 6.;
+  else
+    rhoAvg = rhoAvg / 4.;
+  const double zero = rhoAvg * 0.;
+  if (t.UseMADdisk)
+    Avec[z] = stdmax(zero, rhoAvg * rhoAvg * pow(geomC[LOC_R * D.n + z], 4));
+  else
+    Avec[z] = stdmax(rhoAvg - 0.2, zero) * cos(geomC[LOC_THETA * D.n + z] * (t.MagneticLoops - 1));
+}
+
+// B = curl A in the bulk, zero in ghost zones (:382-404).
+template <class L>
+__global__ __launch_bounds__(kBlock) void curlKernel(Dims D, double dX1, double dX2,
+                                                     const double* __restrict__ geomC,
+                                                     const double* __restrict__ Avec, double* __restrict__ prim) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = D.n;
+  if (z >= n) return;
+  int I[3];
+  zoneIJK(D, z, I[0], I[1], I[2]);
+  auto A = [&](int di, int dj) {
+    const int ii = (I[0] + di) % D.NT[0], jj = (I[1] + dj) % D.NT[1];
+    return Avec[flatIndex(D, ii, jj, I[2])];
+  };
+  const double g = geomC[GEOM_G * n + z];
+  const double A00 = A(0, 0), A01 = A(0, 1), A10 = A(1, 0), A11 = A(1, 1);
+  const double B1 = (A01 - A00 + A11 - A10) / (2. * dX2 * g);
+  const double B2 = (A00 - A10 + A01 - A11) / (2. * dX1 * g);
+  const bool bulk = I[0] >= D.g[0] && I[0] < D.g[0] + D.N[0] && I[1] >= D.g[1] && I[1] < D.g[1] + D.N[1] &&
+                    I[2] >= D.g[2] && I[2] < D.g[2] + D.N[2];
+  const double mask = bulk ? 1. : 0.;
+  prim[L::B1 * n + z] = B1 * mask;
+  prim[L::B2 * n + z] = B2 * mask;
+  prim[L::B3 * n + z] = 0.;
+}
+
+// Plasma-beta reductions for the field normalisation (:406-504): SANE needs
+// min beta = -max(-beta), MAD max Pmag and max Pgas.
+template <class L>
+__global__ __launch_bounds__(kBlock) void betaKernel(StepArgs a, int mad, const double* __restrict__ prim,
+                                                     const double* __restrict__ geomC,
+                                                     double* __restrict__ blockA, double* __restrict__ blockB) {
+  __shared__ double ra[kBlock], rb[kBlock];
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = a.D.n;
+  double va = -INFINITY, vb = -INFINITY;
+  if (z < n) {
+    double P[L::DOF];
+#pragma unroll
+    for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
+    ZoneGeom G;
+    loadGeom(geomC, n, z, G);
+    Kinematics k;
+    kinematics<L>(P, G, a.prm, k);
+    if (mad) {
+      va = k.bSqr / 2.;
+      vb = k.pressure;
+    } else {
+      va = -(2. * (k.pressure + 1.e-13) / (k.bSqr + 1.e-18));
+    }
+  }
+  ra[threadIdx.x] = va;
+  rb[threadIdx.x] = vb;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      ra[threadIdx.x] = stdmax(ra[threadIdx.x], ra[threadIdx.x + w]);
+      rb[threadIdx.x] = stdmax(rb[threadIdx.x], rb[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    blockA[blockIdx.x] = ra[0];
+    blockB[blockIdx.x] = rb[0];
+  }
+}
+
+template <class L>
+__global__ __launch_bounds__(kBlock) void scaleFieldKernel(int64_t n, grim_torus_params t,
+                                                           const double* __restrict__ redA,
+                                                           const double* __restrict__ redB,
+                                                           double* __restrict__ prim) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= n) return;
+  const double BFactor = t.UseMADdisk ? sqrt(redB[0] / redA[0] / t.MinPlasmaBeta)
+                                      : sqrt(-redA[0] / t.MinPlasmaBeta);
+#pragma unroll
+  for (int b = 0; b < 3; b++) prim[(L::B1 + b) * n + z] *= BFactor;
+}
+
+template <class L>
+__global__ __launch_bounds__(kBlock) void floorKernel(StepArgs a, grim_torus_params t,
+                                                      const double* __restrict__ geomC,
+                                                      const double* __restrict__ floors,
+                                                      double* __restrict__ prim) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = a.D.n;
+  if (z >= n) return;
+  double P[L::DOF];
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
+  GeomFull G;
+  loadGeomFull(geomC, n, z, G);
+  applyFloorZone<L>(a.prm, t, G, floors[z], floors[n + z], P);
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) prim[v * n + z] = P[v];
+}
+
+// fullStepDiagnostics' NaN scrub (:722-741): count NaNs in the bulk, then
+// zero every zone holding one when the count is positive.
+__global__ __launch_bounds__(kBlock) void nanCountKernel(Dims D, Box box, int nvars, const double* __restrict__ prim,
+                                                         unsigned long long* __restrict__ count) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= box.count()) return;
+  int i, j, k;
+  box.at(t, i, j, k);
+  const int64_t z = flatIndex(D, i, j, k);
+  unsigned long long c = 0;
+  for (int v = 0; v < nvars; v++) c += isnan(prim[v * D.n + z]) ? 1ull : 0ull;
+  if (c) atomicAdd(count, c);
+}
+
+__global__ __launch_bounds__(kBlock) void nanZeroKernel(int64_t n, int nvars, const unsigned long long* __restrict__ count,
+                                                        double* __restrict__ prim) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= n || count[0] == 0) return;
+  bool bad = false;
+  for (int v = 0; v < nvars; v++) bad = bad || isnan(prim[v * n + z]);
+  if (bad)
+    for (int v = 0; v < nvars; v++) prim[v * n + z] = 0.;
+}
+
+// inflowCheck (:941-1044) on the radial ghost slabs.
+template <class L>
+__global__ __launch_bounds__(kBlock) void inflowCheckKernel(StepArgs a, grim_torus_params t, Box box, int side,
+                                                            const double* __restrict__ geomC,
+                                                            double* __restrict__ prim) {
+  const int64_t tt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tt >= box.count()) return;
+  int i, j, k;
+  box.at(tt, i, j, k);
+  const Dims& D = a.D;
+  const int64_t n = D.n, z = flatIndex(D, i, j, k);
+  double P[L::DOF];
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
+  ZoneGeom G;
+  loadGeom(geomC, n, z, G);
+  Kinematics kk;
+  kinematics<L>(P, G, a.prm, kk);
+  double u[3] = {P[U1] / kk.gammaLF, P[U2] / kk.gammaLF, P[U3] / kk.gammaLF};
+  if (side == 1) {
+    const int64_t zl = flatIndex(D, D.N[0] + D.g[0] - 1, j, k);
+    const double minUr = geomC[(GEOM_GCON0 + 1) * n + zl] * geomC[GEOM_ALPHA * n + zl];
+    u[0] = stdmax(minUr, u[0]);
+  } else {
+    u[0] = stdmin(G.gcon0[1] * G.alpha, u[0]);
+  }
+  double vSqr = u[0] * 0.;
+#pragma unroll
+  for (int p = 0; p < 3; p++)
+#pragma unroll
+    for (int q = 0; q < 3; q++) vSqr += G.cov(p + 1, q + 1) * u[p] * u[q];
+  const double vSqrMax = 1. - 1. / t.MaxLorentzFactor / t.MaxLorentzFactor;
+  const double vSqrMin = 1.e-13;
+  vSqr = stdmax(stdmin(vSqr, vSqrMax), vSqrMin);
+  const double newLorentzFactor = 1. / sqrt(1. - vSqr);
+#pragma unroll
+  for (int p = 0; p < 3; p++) prim[(U1 + p) * n + z] = u[p] * newLorentzFactor;
+}
+
+// fixPoles (:1046-1215): one thread per (i, k) column and pole; the two
+// active rows next to the pole first, then the mirrored ghost rows.
+template <class L>
+__global__ __launch_bounds__(kBlock) void fixPolesKernel(Dims D, const double* __restrict__ geomC,
+                                                         double* __restrict__ prim) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t cols = (int64_t)D.NT[0] * D.NT[2];
+  if (t >= 2 * cols) return;
+  const int pole = (int)(t / cols);
+  const int64_t c = t % cols;
+  const int i = (int)(c % D.NT[0]), k = (int)(c / D.NT[0]);
+  const int64_t n = D.n;
+  const int g2 = D.g[1], N2 = D.N[1];
+  const int idx0 = pole == 0 ? g2 : N2 + g2 - 1;
+  const int idx1 = pole == 0 ? g2 + 1 : idx0 - 1;
+  const int idx2 = pole == 0 ? g2 + 2 : idx0 - 2;
+  auto at = [&](int v, int j) -> double& { return prim[v * n + flatIndex(D, i, j, k)]; };
+  auto Theta = [&](int j) {
+    const double th = geomC[LOC_THETA * n + flatIndex(D, i, j, k)];
+    return pole == 0 ? th : th * (-1.) + M_PI;
+  };
+  const int rows[2] = {idx0, idx1};
+  for (int r = 0; r < 2; r++) {
+    const int jj = rows[r];
+    at(RHO, jj) = at(RHO, idx2);
+    at(UU, jj) = at(UU, idx2);
+    at(U1, jj) = at(U1, idx2);
+    at(U2, jj) = at(U2, idx2) * Theta(jj) / Theta(idx2);
+    at(U3, jj) = at(U3, idx2);
+    if constexpr (L::COND_ON) at(L::Q, jj) = at(L::Q, idx2) * Theta(jj) / Theta(idx2);
+    if constexpr (L::VISC_ON) at(L::DP, jj) = at(L::DP, idx2) * Theta(jj) / Theta(idx2);
+  }
+  for (int q = 0; q < g2; q++) {
+    const int dst = pole == 0 ? g2 - 1 - q : N2 + g2 + q;
+    const int src = pole == 0 ? g2 + q : N2 + g2 - 1 - q;
+    at(RHO, dst) = at(RHO, src);
+    at(UU, dst) = at(UU, src);
+    at(U1, dst) = at(U1, src);
+    at(U3, dst) = at(U3, src);
+    at(U2, dst) = at(U2, src) * (-1.0);
+    at(L::B1, dst) = at(L::B1, src);
+    at(L::B3, dst) = at(L::B3, src);
+    at(L::B2, dst) = at(L::B2, src) * (-1.0);
+    if constexpr (L::COND_ON) at(L::Q, dst) = at(L::Q, src) * (-1.0);
+    if constexpr (L::VISC_ON) at(L::DP, dst) = at(L::DP, src) * (-1.0);
+  }
+}
+
 }  // namespace
 }  // namespace grim
 
@@ -727,6 +1049,10 @@ struct grim_ctx {
   double* l2;
   unsigned long long* counters;
   double* hostBuf;  // pinned: dt / stats readback
+  // torus problem hooks (grim_init_torus)
+  int torus;
+  grim_torus_params tp;
+  double* floors;  // 2 planes: rho and u floor values
 };
 
 namespace grim {
@@ -875,6 +1201,42 @@ int computeDt(grim_ctx* x, const double* prim, double dt, double* out) {
   return GRIM_OK;
 }
 
+template <class L>
+int applyFloor(grim_ctx* x, double* prim) {
+  floorKernel<L><<<blocksFor(x->D.n), kBlock, 0, x->s>>>(x->a, x->tp, x->geomC, x->floors, prim);
+  return checkLaunch("applyFloor");
+}
+
+int scrubNaNs(grim_ctx* x, double* prim, int nvars) {
+  const Box b = bulkBox(x->D);
+  GRIM_HIP_TRY(hipMemsetAsync(x->counters, 0, sizeof(unsigned long long), x->s));
+  nanCountKernel<<<blocksFor(b.count()), kBlock, 0, x->s>>>(x->D, b, nvars, prim, x->counters);
+  nanZeroKernel<<<blocksFor(x->D.n), kBlock, 0, x->s>>>(x->D.n, nvars, x->counters, prim);
+  return checkLaunch("NaN scrub");
+}
+
+// setProblemSpecificBCs of the torus: inflowCheck on both radial ghost
+// slabs, then fixPoles (problem.cpp:1217-1241).
+template <class L>
+int torusBCs(grim_ctx* x, double* prim) {
+  const Dims& D = x->D;
+  for (int side = 0; side < 2; side++) {
+    Box b;
+    for (int d = 0; d < 3; d++) {
+      b.lo[d] = 0;
+      b.hi[d] = D.NT[d];
+    }
+    b.lo[0] = side == 0 ? 0 : D.N[0] + D.g[0];
+    b.hi[0] = side == 0 ? D.g[0] : D.N[0] + 2 * D.g[0];
+    inflowCheckKernel<L><<<blocksFor(b.count()), kBlock, 0, x->s>>>(x->a, x->tp, b, side, x->geomC, prim);
+  }
+  if (D.dim >= 2) {
+    const int64_t cols = 2 * (int64_t)D.NT[0] * D.NT[2];
+    fixPolesKernel<L><<<blocksFor(cols), kBlock, 0, x->s>>>(D, x->geomC, prim);
+  }
+  return checkLaunch("torus boundaries");
+}
+
 // One stage of timeStep (timestep.cpp:14-107 half, 183-255 full).
 template <class L>
 int stage(grim_ctx* x, int st, double dt, grim_step_stats* stats) {
@@ -883,7 +1245,8 @@ int stage(grim_ctx* x, int st, double dt, grim_step_stats* stats) {
   double* primStage = st == 0 ? x->primOld : x->primHalf;
   const double dtPrime = st == 0 ? dt / 2. : dt;
   if (int rc = applyBoundaries(x, primStage, L::DOF, false)) return rc;
-  if (st == 0) GRIM_HIP_TRY(hipMemcpyAsync(x->primDt, x->primOld, bytes, hipMemcpyDeviceToDevice, x->s));
+  if (x->torus)
+    if (int rc = torusBCs<L>(x, primStage)) return rc;
   if (int rc = explicitSources<L>(x, primStage, x->primOld, x->sExp, L::EMHD ? x->sImp : nullptr,
                                   st == 0 ? x->consOld : nullptr))
     return rc;
@@ -921,6 +1284,12 @@ int stage(grim_ctx* x, int st, double dt, grim_step_stats* stats) {
     stats->max_iterations[st] = (int32_t)c[1];
   }
   if (int rc = communicate(x, x->primNew, L::DOF)) return rc;
+  // halfStepDiagnostics / fullStepDiagnostics of the torus (:714-741)
+  if (x->torus) {
+    if (st == 1)
+      if (int rc = scrubNaNs(x, x->primNew, L::DOF)) return rc;
+    if (int rc = applyFloor<L>(x, x->primNew)) return rc;
+  }
   if (st == 0)
     std::swap(x->primHalf, x->primNew);
   else
@@ -1082,6 +1451,7 @@ int grim_destroy(grim_ctx* x) {
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (x->hostBuf) (void)hipHostFree(x->hostBuf);
+  if (x->floors) (void)hipFree(x->floors);
   delete x;
   return GRIM_OK;
 }
@@ -1116,9 +1486,10 @@ int grim_get_prim(grim_ctx* x, double* dst, int on_device) {
 }
 
 int grim_get_geometry(grim_ctx* x, int which, double* dst, int on_device) {
-  GRIM_REQUIRE(x && dst && which >= 0 && which <= 3, "grim_get_geometry: bad argument");
-  double* g[4] = {x->geomC, x->geomX1, x->geomX2, x->gamma};
-  const size_t planes = which == 3 ? 64 : GRIM_LOC_PLANES;
+  GRIM_REQUIRE(x && dst && which >= 0 && which <= 4, "grim_get_geometry: bad argument");
+  GRIM_REQUIRE(which != 4 || x->floors, "grim_get_geometry: no floors before grim_init_torus");
+  double* g[5] = {x->geomC, x->geomX1, x->geomX2, x->gamma, x->floors};
+  const size_t planes = which == 3 ? 64 : (which == 4 ? 2 : GRIM_LOC_PLANES);
   GRIM_HIP_TRY(hipMemcpyAsync(dst, g[which], sizeof(double) * planes * x->D.n,
                               on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, x->s));
   GRIM_HIP_TRY(hipStreamSynchronize(x->s));
@@ -1150,6 +1521,10 @@ int grim_time_step(grim_ctx* x, grim_step_stats* stats) {
     if (stats) stats->dt = dt;
     if (int rc = stage<L>(x, 0, dt, stats)) return rc;
     if (int rc = stage<L>(x, 1, dt, stats)) return rc;
+    // elemOld of the next computeDt: the problem's fullStepDiagnostics set
+    // it from this step's final primOld.
+    GRIM_HIP_TRY(hipMemcpyAsync(x->primDt, x->primOld, sizeof(double) * (size_t)L::DOF * x->D.n,
+                                hipMemcpyDeviceToDevice, x->s));
     x->dt = dt;
     x->time += dt;
     return GRIM_OK;
@@ -1230,6 +1605,54 @@ int grim_explicit_sources(grim_ctx* x, const double* prim, double* sources) {
 int grim_compute_dt(grim_ctx* x, const double* prim, double dt, double* new_dt) {
   GRIM_REQUIRE(x && prim && new_dt && dt > 0., "grim_compute_dt: bad argument");
   return withLayout(x, [&]<class L>() -> int { return computeDt<L>(x, prim, dt, new_dt); });
+}
+
+int grim_init_torus(grim_ctx* x, const grim_torus_params* t) {
+  GRIM_REQUIRE(x && t, "grim_init_torus: null argument");
+  GRIM_REQUIRE(x->c.metric == GRIM_METRIC_MKS && x->c.dim >= 2 && x->c.N2 >= 6,
+               "grim_init_torus: needs an MKS grid with dim >= 2 and N2 >= 6");
+  const Dims& D = x->D;
+  const int64_t n = D.n;
+  GRIM_REQUIRE(2 * (int64_t)blocksFor(n) <= n, "grim_init_torus: grid too small");
+  if (!x->floors) GRIM_HIP_TRY(hipMalloc(&x->floors, sizeof(double) * 2 * (size_t)n));
+  x->tp = *t;
+  x->torus = 1;
+  x->a.fluxFilter = 1;
+  const unsigned nb = blocksFor(n);
+  floorPlanesKernel<<<nb, kBlock, 0, x->s>>>(D, x->tp, x->geomC, x->floors);
+  return withLayout(x, [&]<class L>() -> int {
+    double* prim = x->primOld;
+    double* Avec = x->uT;  // scratch plane
+    torusICKernel<L><<<nb, kBlock, 0, x->s>>>(x->a, x->tp, x->c.blackHoleSpin, x->c.hSlope, x->cg, x->geomC, prim);
+    planeMaxKernel<<<nb, kBlock, 0, x->s>>>(n, prim + RHO * n, 1., x->red);
+    maxReduceKernel<<<1, kBlock, 0, x->s>>>(nb, x->red, x->red + nb);
+    normalizeKernel<<<nb, kBlock, 0, x->s>>>(n, x->red + nb, prim);
+    vectorPotentialKernel<<<nb, kBlock, 0, x->s>>>(D, x->tp, x->geomC, prim + RHO * n, Avec);
+    curlKernel<L><<<nb, kBlock, 0, x->s>>>(D, x->a.dX[0], x->a.dX[1], x->geomC, Avec, prim);
+    if (int rc = communicate(x, prim, L::DOF)) return rc;
+    // the l2 and sExp work arrays hold the two per-block reductions
+    double* blockA = x->l2;
+    double* blockB = x->l2 + nb;
+    betaKernel<L><<<nb, kBlock, 0, x->s>>>(x->a, x->tp.UseMADdisk, prim, x->geomC, blockA, blockB);
+    maxReduceKernel<<<1, kBlock, 0, x->s>>>(nb, blockA, x->sExp);
+    maxReduceKernel<<<1, kBlock, 0, x->s>>>(nb, blockB, x->sExp + 1);
+    scaleFieldKernel<L><<<nb, kBlock, 0, x->s>>>(n, x->tp, x->sExp, x->sExp + 1, prim);
+    if (int rc = applyFloor<L>(x, prim)) return rc;
+    if (int rc = scrubNaNs(x, prim, L::DOF)) return rc;
+    if (int rc = applyFloor<L>(x, prim)) return rc;
+    GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+    return checkLaunch("grim_init_torus");
+  });
+}
+
+int grim_apply_floor(grim_ctx* x, double* prim) {
+  GRIM_REQUIRE(x && prim && x->torus, "grim_apply_floor: no torus (grim_init_torus) or null prim");
+  return withLayout(x, [&]<class L>() -> int { return applyFloor<L>(x, prim); });
+}
+
+int grim_torus_bcs(grim_ctx* x, double* prim) {
+  GRIM_REQUIRE(x && prim && x->torus, "grim_torus_bcs: no torus (grim_init_torus) or null prim");
+  return withLayout(x, [&]<class L>() -> int { return torusBCs<L>(x, prim); });
 }
 
 int grim_init_linear_mode(grim_ctx* x, int mode, double amplitude) {

@@ -66,7 +66,7 @@ struct Box {
 // ---------------------------------------------------------------------------
 // Geometry of one location (23 planes, see include/grim_hip.h)
 // ---------------------------------------------------------------------------
-enum { LOC_R = 22 };
+enum { LOC_R = 22, LOC_THETA = 23 };
 
 struct GeomFull {
   ZoneGeom zg;        // gCov, gCon row 0, alpha, g

@@ -111,6 +111,7 @@ struct Layout {
 // magnetic 4-vector and b^2 (physics.cpp:60-116).
 struct Kinematics {
   double rho, u, pressure, temperature;
+  double gammaLF;  // Lorentz factor (fluidElement::gammaLorentzFactor)
   double uCon[4], uCov[4], bCon[4], bCov[4];
   double bSqr;
 };
@@ -128,6 +129,7 @@ __device__ __forceinline__ void kinematics(const double* P, const ZoneGeom& G,
                             G.cov(3, 3) * u3 * u3 +
                             2. * (G.cov(1, 2) * u1 * u2 + G.cov(1, 3) * u1 * u3 +
                                   G.cov(2, 3) * u2 * u3));
+  k.gammaLF = gamma;
   k.uCon[0] = gamma / G.alpha;
   k.uCon[1] = u1 - gamma * G.gcon0[1] * G.alpha;
   k.uCon[2] = u2 - gamma * G.gcon0[2] * G.alpha;

@@ -106,6 +106,18 @@ class TimeStepper:
     def initLinearMode(self, mode: int, amplitude: float):
         capi.check(capi.lib().grim_init_linear_mode(self._ctx, mode, amplitude), "grim_init_linear_mode")
 
+    def initTorus(self, torus: capi.TorusParams):
+        """Torus initial conditions into primOld; enables the torus's per-stage hooks."""
+        capi.check(capi.lib().grim_init_torus(self._ctx, C.byref(torus)), "grim_init_torus")
+
+    def applyFloor(self, prim):
+        capi.check(capi.lib().grim_apply_floor(self._ctx, capi._ptr(prim)), "grim_apply_floor")
+        return prim
+
+    def setProblemSpecificBCs(self, prim):
+        capi.check(capi.lib().grim_torus_bcs(self._ctx, capi._ptr(prim)), "grim_torus_bcs")
+        return prim
+
     # -- the step -----------------------------------------------------------
     def timeStep(self) -> capi.StepStats:
         st = capi.StepStats()

@@ -202,11 +202,11 @@ int grim_known_root_rhs(const grim_params* p, const grim_zone_batch* b,
  * Geometry of one location: GRIM_LOC_PLANES planes
  *   0..9   gCov 00 01 02 03 11 12 13 22 23 33
  *   10..13 gCon 00 01 02 03,  14 alpha,  15 g = sqrt(-det gCov)
- *   16..21 gCon 11 12 13 22 23 33,  22 r (xCoords[X1])
+ *   16..21 gCon 11 12 13 22 23 33,  22 r (xCoords[X1]),  23 theta (xCoords[X2])
  * (planes 0..15 are the batched-zone layout).  The connection at the zone
  * centre has 64 planes, Gamma^l_{k n} at plane l*16 + k*4 + n.
  * ------------------------------------------------------------------------ */
-#define GRIM_LOC_PLANES 23
+#define GRIM_LOC_PLANES 24
 
 typedef enum grim_metric { GRIM_METRIC_MINKOWSKI = 0, GRIM_METRIC_MKS = 1 } grim_metric;
 /* src/params.hpp:48-54 (DIRICHLET is not supported) */
@@ -304,6 +304,31 @@ int grim_compute_dt(grim_ctx* ctx, const double* prim, double dt, double* new_dt
  * equations; not in the reference); mode 1 = the reference's EMHD mode,
  * k = (2 pi, 4 pi).  amplitude multiplies the eigenvector. */
 int grim_init_linear_mode(grim_ctx* ctx, int mode, double amplitude);
+
+/* The torus problem's parameters (src/problem/torus/params.cpp:38-80). */
+typedef struct grim_torus_params {
+  double InnerEdgeRadius, PressureMaxRadius, Adiabat, MinPlasmaBeta;
+  int MagneticLoops, UseMADdisk;
+  double InitialPerturbationAmplitude;
+  double MaxLorentzFactor;
+  double RhoFloorAmpl, UFloorAmpl, RhoFloorSlope, UFloorSlope;
+  double BsqrOverRhoMax, BsqrOverUMax;
+  uint64_t seed; /* counter-based noise stream replacing PETSc's RAND48 */
+} grim_torus_params;
+
+/* timeStepper::initialConditions of the torus (src/problem/torus/
+ * problem.cpp:145-540: Fishbone-Moncrief torus, loop field normalised to
+ * the minimum plasma beta, floors) into primOld, and from then on the
+ * torus's per-stage hooks in grim_time_step: setProblemSpecificBCs
+ * (inflowCheck, fixPoles, :941-1240), the flux filter (:1243-1285) and the
+ * half/full-step diagnostics (NaN scrub and applyFloor, :714-741).  Needs
+ * an MKS grid.  grim_get_geometry(ctx, 4, ...) then returns the two floor
+ * planes pow(r, slope) * ampl (rho, u). */
+int grim_init_torus(grim_ctx* ctx, const grim_torus_params* t);
+/* applyFloor (problem.cpp:542-712) on caller-provided primitive planes. */
+int grim_apply_floor(grim_ctx* ctx, double* prim);
+/* setProblemSpecificBCs of the torus on caller-provided primitive planes. */
+int grim_torus_bcs(grim_ctx* ctx, double* prim);
 
 #ifdef __cplusplus
 }

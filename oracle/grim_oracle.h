@@ -172,10 +172,11 @@ void grim_oracle_count_ops(const grim_oracle_params* p,
  *   15     g
  *   16..21 gCon 11 12 13 22 23 33
  *   22     r (xCoords[X1])
+ *   23     theta (xCoords[X2])
  * (planes 0..15 are the batched-zone layout above).  The connection at the
  * zone centre has 64 planes, Gamma^l_{k n} at plane l*16 + k*4 + n.
  * ------------------------------------------------------------------------ */
-#define GRIM_ORACLE_LOC_PLANES 23
+#define GRIM_ORACLE_LOC_PLANES 24
 
 #define GRIM_ORACLE_METRIC_MINKOWSKI 0
 #define GRIM_ORACLE_METRIC_MKS 1
@@ -254,6 +255,34 @@ double grim_oracle_compute_dt(const grim_oracle_grid* c, const grim_oracle_param
                               const grim_oracle_geometry* gm, const double* prim,
                               double dt);
 
+/* Torus problem (src/problem/torus/params.cpp, problem.cpp): initial
+ * conditions and the per-stage hooks.  minRho/minU: optional floor planes
+ * pow(r, slope)*ampl (NULL: computed here). */
+typedef struct grim_oracle_torus {
+  double InnerEdgeRadius, PressureMaxRadius, Adiabat, MinPlasmaBeta;
+  int MagneticLoops, UseMADdisk;
+  double InitialPerturbationAmplitude;
+  double MaxLorentzFactor;
+  double RhoFloorAmpl, UFloorAmpl, RhoFloorSlope, UFloorSlope;
+  double BsqrOverRhoMax, BsqrOverUMax;
+  uint64_t seed; /* counter-based noise replacing PETSc's RAND48 stream */
+  const double* minRho;
+  const double* minU;
+} grim_oracle_torus;
+
+/* FM torus + SANE/MAD loop field + floors (problem.cpp:145-540), all zones. */
+void grim_oracle_torus_ic(const grim_oracle_grid* c, const grim_oracle_params* p,
+                          const grim_oracle_torus* t, const grim_oracle_geometry* gm,
+                          double* prim);
+/* applyFloor (problem.cpp:542-712), all zones, in place. */
+void grim_oracle_apply_floor(const grim_oracle_grid* c, const grim_oracle_params* p,
+                             const grim_oracle_torus* t, const grim_oracle_geometry* gm,
+                             double* prim);
+/* setProblemSpecificBCs: inflowCheck then fixPoles (problem.cpp:941-1240). */
+void grim_oracle_torus_bcs(const grim_oracle_grid* c, const grim_oracle_params* p,
+                           const grim_oracle_torus* t, const grim_oracle_geometry* gm,
+                           double* prim);
+
 typedef struct grim_oracle_step_stats {
   double dt;
   int64_t unconverged[2];   /* bulk zones above atol after each stage */
@@ -261,14 +290,15 @@ typedef struct grim_oracle_step_stats {
 } grim_oracle_step_stats;
 
 /* timeStepper::timeStep (src/timestepper/timestep.cpp:3-334): computeDt from
- * primDt (the state elemOld was last set from), then the half and full
- * stages.  primOld (dof planes) is advanced in place, primDt is replaced by
- * the boundary-filled primOld of this step, *dt and *time advance.
+ * primDt (the state elemOld was last set from: primOld at the end of the
+ * previous step, or the initial state), then the half and full stages with
+ * the torus hooks when t != NULL.  primOld (dof planes) is advanced in
+ * place and copied to primDt; *dt and *time advance.
  * semantics: GRIM_ORACLE_NEWTON_PER_ZONE or GLOBAL. */
 void grim_oracle_time_step(const grim_oracle_grid* c, const grim_oracle_params* p,
-                           const grim_oracle_geometry* gm, double* primOld,
-                           double* primDt, double* dt, double* time, int semantics,
-                           grim_oracle_step_stats* stats);
+                           const grim_oracle_geometry* gm, const grim_oracle_torus* t,
+                           double* primOld, double* primDt, double* dt, double* time,
+                           int semantics, grim_oracle_step_stats* stats);
 
 /* The two stages as separate calls, for stage-level parity tests.
  * stage 0 (half): boundaries on primOld, consOld, sources, divF, induction,
@@ -276,8 +306,8 @@ void grim_oracle_time_step(const grim_oracle_grid* c, const grim_oracle_params* 
  * stage 1 (full): boundaries on primHalf, sources, divF, induction, Newton
  * from the guess primHalf; writes primNew. */
 void grim_oracle_stage(const grim_oracle_grid* c, const grim_oracle_params* p,
-                       const grim_oracle_geometry* gm, int stage, double dt,
-                       double* primOld, double* primHalf, double* consOld,
+                       const grim_oracle_geometry* gm, const grim_oracle_torus* t, int stage,
+                       double dt, double* primOld, double* primHalf, double* consOld,
                        double* primNew, int semantics, grim_oracle_step_stats* stats);
 
 #ifdef __cplusplus

@@ -236,14 +236,15 @@ struct Coord {
 const int kCov10[10][2] = {{0, 0}, {0, 1}, {0, 2}, {0, 3}, {1, 1}, {1, 2}, {1, 3}, {2, 2}, {2, 3}, {3, 3}};
 const int kCon6[6][2] = {{1, 1}, {1, 2}, {1, 3}, {2, 2}, {2, 3}, {3, 3}};
 
-void storeLocation(const double G[4][4], const double C[4][4], double gDet, double r, int64_t n,
-                   int64_t z, double* out) {
+void storeLocation(const double G[4][4], const double C[4][4], double gDet, double r, double theta,
+                   int64_t n, int64_t z, double* out) {
   for (int q = 0; q < 10; q++) out[q * n + z] = G[kCov10[q][0]][kCov10[q][1]];
   for (int q = 0; q < 4; q++) out[(10 + q) * n + z] = C[0][q];
   out[14 * n + z] = 1. / std::sqrt(-C[0][0]);
   out[15 * n + z] = std::sqrt(-gDet);
   for (int q = 0; q < 6; q++) out[(16 + q) * n + z] = C[kCon6[q][0]][kCon6[q][1]];
   out[22 * n + z] = r;
+  out[23 * n + z] = theta;
 }
 
 template <class T>
@@ -481,7 +482,7 @@ void riemannAll(const grim_oracle_grid& c, const grim_oracle_params& p, const Di
 // timeStepper::computeDivOfFluxes + fluxCT (fvmfluxes.cpp, constrainedtransport.cpp)
 void divFluxes(const grim_oracle_grid& c, const grim_oracle_params& p, const Dims& D,
                const grim_oracle_geometry& gm, const double* prim, double* divF,
-               double* fluxesOut) {
+               double* fluxesOut, bool torusFilter = false) {
   const Vars vars(p);
   const int nv = vars.dof;
   const int64_t n = D.n;
@@ -529,6 +530,27 @@ void divFluxes(const grim_oracle_grid& c, const grim_oracle_params& p, const Dim
         F[2][vars.B3 * n + z] = 0.;
       }
     });
+  }
+
+  if (torusFilter) {
+    // applyProblemSpecificFluxFilter, torus/problem.cpp:1243-1285: no mass
+    // inflow through the radial boundary faces, no flux through the polar
+    // faces (the F1_B2 ghost-row copies it also makes are never read).
+    const int i0 = D.g[0], i1 = D.N[0] + D.g[0];
+    const int j0 = D.g[1], j1 = D.N[1] + D.g[1];
+    for (int k = 0; k < D.NT[2]; k++)
+      for (int j = 0; j < D.NT[1]; j++) {
+        double* f = F[0] + RHO * n;
+        f[D.idx(i0, j, k)] = std::min(f[D.idx(i0, j, k)], 0.);
+        f[D.idx(i1, j, k)] = std::max(f[D.idx(i1, j, k)], 0.);
+      }
+    if (D.dim >= 2)
+      for (int k = 0; k < D.NT[2]; k++)
+        for (int i = 0; i < D.NT[0]; i++)
+          for (int v = 0; v < nv; v++) {
+            F[1][v * n + D.idx(i, j0, k)] = 0.;
+            F[1][v * n + D.idx(i, j1, k)] = 0.;
+          }
   }
 
   // Divergence over the bulk.  dim < 3: af::convolve with {1,-1,0}/dX
@@ -669,32 +691,35 @@ void explicitSources(const grim_oracle_grid& c, const grim_oracle_params& p, con
   });
 }
 
-// grid::communicate on one rank: ghost zones whose out-of-range coordinates
-// all lie along periodic directions take the periodic image (DMDA box
-// stencil); the others are left for applyBoundaryConditions.
+// grid::communicate on one rank (grid.cpp:306-360): the local vector of a
+// DMDA with a box stencil receives the periodic image in every ghost zone
+// whose out-of-range coordinates all lie along periodic directions; the
+// other ghost zones lie outside the global domain, are never written by
+// DMGlobalToLocal and keep the local vector's initial zeros, which
+// copyLocalVecToVars then writes back.
 void communicate(const grim_oracle_grid& c, const Dims& D, int nv, double* prim) {
   const int bLo[3] = {c.boundaryLeft, c.boundaryBottom, c.boundaryBack};
   const int bHi[3] = {c.boundaryRight, c.boundaryTop, c.boundaryFront};
   int lo[3] = {0, 0, 0}, hi[3] = {D.NT[0], D.NT[1], D.NT[2]};
   forRange(lo, hi, [&](int i, int j, int k) {
     int I[3] = {i, j, k}, W[3];
-    bool ghost = false;
+    bool ghost = false, outside = false;
     for (int d = 0; d < 3; d++) {
       W[d] = I[d];
       if (D.g[d] == 0) continue;
       if (I[d] < D.g[d]) {
-        if (bLo[d] != GRIM_ORACLE_BC_PERIODIC) return;
+        if (bLo[d] != GRIM_ORACLE_BC_PERIODIC) outside = true;
         W[d] = I[d] + D.N[d];
         ghost = true;
       } else if (I[d] >= D.g[d] + D.N[d]) {
-        if (bHi[d] != GRIM_ORACLE_BC_PERIODIC) return;
+        if (bHi[d] != GRIM_ORACLE_BC_PERIODIC) outside = true;
         W[d] = I[d] - D.N[d];
         ghost = true;
       }
     }
     if (!ghost) return;
     const int64_t z = D.idx(i, j, k), w = D.idx(W[0], W[1], W[2]);
-    for (int v = 0; v < nv; v++) prim[v * D.n + z] = prim[v * D.n + w];
+    for (int v = 0; v < nv; v++) prim[v * D.n + z] = outside ? 0. : prim[v * D.n + w];
   });
 }
 
@@ -757,25 +782,363 @@ double computeDt(const grim_oracle_grid& c, const grim_oracle_params& p, const D
   return newDt;
 }
 
+// ---------------------------------------------------------------------------
+// Torus problem (src/problem/torus/problem.cpp)
+// ---------------------------------------------------------------------------
+// Counter-based uniform in [0, 1): splitmix64 (Steele, Lea & Flood 2014) of
+// seed + G (32 z + 1), the same stream as the product's batch generator.
+double uniformAt(uint64_t seed, uint64_t z) {
+  uint64_t x = seed + 0x9E3779B97F4A7C15ull * (32ull * z + 1ull);
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  x = x ^ (x >> 31);
+  return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// problem.cpp:56-143 (M = 1)
+double lFishboneMoncrief(double a, double r, double) {
+  const double M = 1.;
+  return std::sqrt(M / std::pow(r, 3.)) *
+         (std::pow(r, 4.) + r * r * a * a - 2. * M * r * a * a - a * std::sqrt(M * r) * (r * r - a * a)) /
+         (r * r - 3 * M * r + 2. * a * std::sqrt(M * r));
+}
+double computeDelta(double a, double r, double) { return r * r - 2 * 1. * r + a * a; }
+double computeSigma(double a, double r, double theta) {
+  return r * r + a * a * std::cos(theta) * std::cos(theta);
+}
+double computeA(double a, double r, double theta) {
+  const double Delta = computeDelta(a, r, theta);
+  return std::pow(r * r + a * a, 2.) - Delta * a * a * std::sin(theta) * std::sin(theta);
+}
+double lnOfhTerm1(double a, double r, double theta, double l) {
+  const double Delta = computeDelta(a, r, theta), Sigma = computeSigma(a, r, theta),
+               A = computeA(a, r, theta);
+  return 0.5 * std::log((1. + std::sqrt(1. + (4. * l * l * Sigma * Sigma * Delta) /
+                                                 (A * std::sin(theta) * A * std::sin(theta)))) /
+                        (Sigma * Delta / A));
+}
+double lnOfhTerm2(double a, double r, double theta, double l) {
+  const double Delta = computeDelta(a, r, theta), Sigma = computeSigma(a, r, theta),
+               A = computeA(a, r, theta);
+  return -0.5 * std::sqrt(1. + (4. * l * l * Sigma * Sigma * Delta) /
+                                   (A * A * std::sin(theta) * std::sin(theta)));
+}
+double lnOfhTerm3(double a, double r, double theta, double l) {
+  const double A = computeA(a, r, theta);
+  const double M = 1.;
+  return -2 * a * M * r * l / A;
+}
+double computeLnOfh(const grim_oracle_torus& t, double a, double r, double theta) {
+  const double l = lFishboneMoncrief(a, t.PressureMaxRadius, M_PI / 2.);
+  const double term1 = lnOfhTerm1(a, r, theta, l);
+  const double term2 = lnOfhTerm2(a, r, theta, l);
+  const double term3 = lnOfhTerm3(a, r, theta, l);
+  const double term1InnerEdge = lnOfhTerm1(a, t.InnerEdgeRadius, M_PI / 2., l);
+  const double term2InnerEdge = lnOfhTerm2(a, t.InnerEdgeRadius, M_PI / 2., l);
+  const double term3InnerEdge = lnOfhTerm3(a, t.InnerEdgeRadius, M_PI / 2., l);
+  return term1 + term2 + term3 - term1InnerEdge - term2InnerEdge - term3InnerEdge;
+}
+
+// The torus's fluid state at one zone (problem.cpp:175-322): fills rho, u,
+// u^1..u^3 of P (B left to the caller).
+void torusZone(const grim_oracle_grid& c, const grim_oracle_params& p, const grim_oracle_torus& t,
+               double r, double theta, double X2, double lapse, const double beta[4], double xi,
+               double* P) {
+  const double aBH = c.blackHoleSpin;
+  double lnOfh = 1.;
+  if (r >= t.InnerEdgeRadius) lnOfh = computeLnOfh(t, aBH, r, theta);
+  if (lnOfh < 0. || r < t.InnerEdgeRadius) {
+    P[RHO] = p.rhoFloorInFluidElement;
+    P[UU] = p.uFloorInFluidElement;
+    P[U1] = 0.;
+    P[U2] = 0.;
+    P[U3] = 0.;
+    return;
+  }
+  const double h = std::exp(lnOfh);
+  const double Gamma = p.adiabaticIndex, Kappa = t.Adiabat;
+  P[RHO] = std::pow((h - 1) * (Gamma - 1.) / (Kappa * Gamma), 1. / (Gamma - 1.));
+  P[UU] = Kappa * std::pow(P[RHO], Gamma) / (Gamma - 1.) *
+          (1. + t.InitialPerturbationAmplitude * (xi - 0.5));
+  const double A = computeA(aBH, r, theta), Sigma = computeSigma(aBH, r, theta),
+               Delta = computeDelta(aBH, r, theta);
+  const double l = lFishboneMoncrief(aBH, t.PressureMaxRadius, M_PI / 2.);
+  const double expOfMinus2Chi = Sigma * Sigma * Delta / (A * A * std::sin(theta) * std::sin(theta));
+  const double uCovPhiBL = std::sqrt((-1. + std::sqrt(1. + 4 * l * l * expOfMinus2Chi)) / 2.);
+  const double uConPhiBL = 2. * aBH * r * std::sqrt(1. + uCovPhiBL * uCovPhiBL) / std::sqrt(A * Sigma * Delta) +
+                           std::sqrt(Sigma / A) * uCovPhiBL / std::sin(theta);
+  double uConBL[4] = {0., 0., 0., uConPhiBL};
+  double gCovBL[4][4], T[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) gCovBL[a][b] = T[a][b] = 0.;
+  const double mu = 1 + aBH * aBH * std::cos(theta) * std::cos(theta) / (r * r);
+  gCovBL[0][0] = -(1. - 2. / (r * mu));
+  gCovBL[0][3] = -2. * aBH * std::sin(theta) * std::sin(theta) / (r * mu);
+  gCovBL[3][0] = gCovBL[0][3];
+  gCovBL[1][1] = mu * r * r / Delta;
+  gCovBL[2][2] = r * r * mu;
+  gCovBL[3][3] = r * r * std::sin(theta) * std::sin(theta) *
+                 (1. + aBH * aBH / (r * r) + 2. * aBH * aBH * std::sin(theta) * std::sin(theta) / (r * r * r * mu));
+  T[0][0] = T[1][1] = T[2][2] = T[3][3] = 1.;
+  T[0][1] = 2. * r / Delta;
+  T[3][1] = aBH / Delta;
+  const double AA = gCovBL[0][0];
+  const double BB = 2. * (gCovBL[0][1] * uConBL[1] + gCovBL[0][2] * uConBL[2] + gCovBL[0][3] * uConBL[3]);
+  const double CC = 1. + gCovBL[1][1] * uConBL[1] * uConBL[1] + gCovBL[2][2] * uConBL[2] * uConBL[2] +
+                    gCovBL[3][3] * uConBL[3] * uConBL[3] +
+                    2. * (gCovBL[1][2] * uConBL[1] * uConBL[2] + gCovBL[1][3] * uConBL[1] * uConBL[3] +
+                          gCovBL[2][3] * uConBL[2] * uConBL[3]);
+  const double discriminent = BB * BB - 4. * AA * CC;
+  uConBL[0] = -(BB + std::sqrt(discriminent)) / (2. * AA);
+  double uConKS[4];
+  for (int a = 0; a < 4; a++) {
+    uConKS[a] = 0.;
+    for (int b = 0; b < 4; b++) uConKS[a] += T[a][b] * uConBL[b];
+  }
+  const double rFactor = r;
+  const double hFactor = M_PI + (1. - c.hSlope) * M_PI * std::cos(2. * M_PI * X2);
+  const double uConMKS[4] = {uConKS[0], uConKS[1] / rFactor, uConKS[2] / hFactor, uConKS[3]};
+  P[U1] = uConMKS[1] + std::pow(lapse, 2.) * beta[1] * uConMKS[0];
+  P[U2] = uConMKS[2] + std::pow(lapse, 2.) * beta[2] * uConMKS[0];
+  P[U3] = uConMKS[3] + std::pow(lapse, 2.) * beta[3] * uConMKS[0];
+}
+
+// applyFloor, problem.cpp:542-712, one zone.  minRho/minU are the floor
+// values pow(r, slope)*ampl of the zone.
+void applyFloorZone(const grim_oracle_params& p, const grim_oracle_torus& t, const ZoneGeom<double>& G,
+                    double minRho, double minU, double* P) {
+  const Vars vars(p);
+  const ZoneClosure zc{1., 1.};
+  const double rho_prefloor = P[RHO] * 1.;
+  const double u_prefloor = P[UU] * 1.;
+  double condition = P[RHO] < minRho ? 1. : 0.;
+  double usefloor = condition;
+  P[RHO] = condition * minRho + (1. - condition) * P[RHO];
+  condition = P[UU] < minU ? 1. : 0.;
+  usefloor = std::max(condition, usefloor);
+  P[UU] = condition * minU + (1. - condition) * P[UU];
+  FluidElement<double> e;
+  e.set(P, G, p, zc);
+  double bSqr = e.bSqr;
+  condition = bSqr > t.BsqrOverRhoMax * P[RHO] ? 1. : 0.;
+  usefloor = std::max(condition, usefloor);
+  P[RHO] = P[RHO] * (1. - condition) + condition * bSqr / t.BsqrOverRhoMax;
+  condition = bSqr > t.BsqrOverUMax * P[UU] ? 1. : 0.;
+  usefloor = std::max(condition, usefloor);
+  P[UU] = P[UU] * (1. - condition) + condition * bSqr / t.BsqrOverUMax;
+  const double zero = 0.;
+  const double trans =
+      std::max(std::min(zero + 1., (bSqr - 0.1 * P[RHO]) / P[RHO]), zero) * (usefloor > zero ? 1. : 0.);
+  // Sasha's floor method
+  const double betapar = -e.bCon[0] / bSqr / e.uCon[0];
+  double betasqr = betapar * betapar * bSqr;
+  const double betasqrmax = 1. - 1. / t.MaxLorentzFactor / t.MaxLorentzFactor;
+  betasqr = std::min(betasqr, betasqrmax);
+  const double gamma = 1. / std::sqrt(1. - betasqr);
+  double ucondr[4];
+  for (int m = 0; m < 4; m++) ucondr[m] = gamma * (e.uCon[m] + betapar * e.bCon[m]);
+  const double Bcon[4] = {zero, P[vars.B1], P[vars.B2], P[vars.B3]};
+  double Bcov[4];
+  for (int m = 0; m < 4; m++) {
+    Bcov[m] = zero;
+    for (int n = 0; n < 4; n++) Bcov[m] += G.gCov[n][m] * Bcon[n];
+  }
+  double udotB = zero, bSqr_fl = zero;
+  for (int n = 0; n < 4; n++) {
+    udotB += Bcov[n] * e.uCon[n];
+    bSqr_fl += Bcov[n] * Bcon[n];
+  }
+  double bNorm = std::sqrt(bSqr_fl);
+  const double bMin = std::sqrt(p.bSqrFloorInFluidElement);
+  bNorm = std::max(bNorm, bMin);
+  const double wold = rho_prefloor + u_prefloor * p.adiabaticIndex;
+  const double QdotB = udotB * wold * e.uCon[0];
+  const double wnew = P[RHO] + P[UU] * p.adiabaticIndex;
+  const double x = 2. * QdotB / (bNorm * wnew * ucondr[0]);
+  const double vpar = x / (ucondr[0] * (1. + std::sqrt(1. + x * x)));
+  const double one_over_ucondr_t = 1. / ucondr[0];
+  double vcon[4];
+  vcon[0] = zero + 1.;
+  for (int m = 1; m < 4; m++) vcon[m] = vpar * Bcon[m] / bNorm + ucondr[m] * one_over_ucondr_t;
+  double vsqr = zero;
+  for (int m = 0; m < 4; m++)
+    for (int n = 0; n < 4; n++) vsqr += G.gCov[m][n] * vcon[m] * vcon[n];
+  const double condition_v = (vsqr >= 0. || vsqr < 1. / G.gCon[0][0]) ? 1. : 0.;
+  vsqr = (1. - condition_v) * vsqr + condition_v / G.gCon[0][0];
+  const double ut = std::sqrt(-1. / vsqr);
+  double utcon[4];
+  utcon[0] = zero;
+  for (int m = 1; m < 4; m++) utcon[m] = ut * (vcon[m] - G.gCon[0][m] / G.gCon[0][0]);
+  P[U1] = P[U1] * (1. - trans) + trans * utcon[1];
+  P[U2] = P[U2] * (1. - trans) + trans * utcon[2];
+  P[U3] = P[U3] * (1. - trans) + trans * utcon[3];
+  // Maximum Lorentz factor.  grim multiplies only when some zone of the grid
+  // exceeds it; every other zone's factor is exactly 1, so applying it
+  // everywhere changes no value.
+  e.set(P, G, p, zc);
+  const double lorentzFactorSqr = e.gammaLorentzFactor * e.gammaLorentzFactor;
+  double MultFac = std::sqrt(std::max((lorentzFactorSqr - 1.) /
+                                          (t.MaxLorentzFactor * t.MaxLorentzFactor - 1.),
+                                      1.));
+  MultFac = 1. / MultFac;
+  P[U1] *= MultFac;
+  P[U2] *= MultFac;
+  P[U3] *= MultFac;
+  e.set(P, G, p, zc);
+  if (p.conduction) {
+    const double Qmax = 1.07 * p.ConductionClosureFactor * P[RHO] * std::pow(e.soundSpeed, 3.);
+    const double LimFac = std::max(std::fabs(e.q) / Qmax, 1.);
+    P[vars.Q] = P[vars.Q] / LimFac;
+  }
+  if (p.viscosity) {
+    const double pressure = e.pressure, deltaP = e.deltaP;
+    bSqr = e.bSqr;
+    const double dPmod = std::max(pressure - 2. / 3. * deltaP, 0.01 * p.bSqrFloorInFluidElement) /
+                         std::max(pressure + 1. / 3. * deltaP, p.bSqrFloorInFluidElement);
+    const double dPmaxPlus = std::min(1.07 * p.ViscosityClosureFactor * bSqr * 0.5 * dPmod, 1.49 * pressure);
+    const double dPmaxMinus = std::max(-1.07 * p.ViscosityClosureFactor * bSqr, -2.99 * pressure);
+    const double cond = deltaP > 0. ? 1. : 0.;
+    P[vars.DP] = P[vars.DP] * (cond / std::max(deltaP / dPmaxPlus, 1.) +
+                               (1. - cond) / std::max(deltaP / dPmaxMinus, 1.));
+  }
+}
+
+void applyFloorAll(const grim_oracle_grid& c, const grim_oracle_params& p, const grim_oracle_torus& t,
+                   const Dims& D, const grim_oracle_geometry& gm, double* prim) {
+  const Vars vars(p);
+  const int64_t n = D.n;
+  int lo[3] = {0, 0, 0}, hi[3] = {D.NT[0], D.NT[1], D.NT[2]};
+  forRange(lo, hi, [&](int i, int j, int k) {
+    const int64_t z = D.idx(i, j, k);
+    ZoneGeom<double> G;
+    loadLocation(gm.center, n, z, &G);
+    const double Radius = gm.center[22 * n + z];
+    const double minRho = t.minRho ? t.minRho[z] : std::pow(Radius, t.RhoFloorSlope) * t.RhoFloorAmpl;
+    const double minU = t.minU ? t.minU[z] : std::pow(Radius, t.UFloorSlope) * t.UFloorAmpl;
+    double P[GRIM_ORACLE_MAX_VARS];
+    loadPrim(prim, vars.dof, n, z, P);
+    applyFloorZone(p, t, G, minRho, minU, P);
+    for (int v = 0; v < vars.dof; v++) prim[v * n + z] = P[v];
+  });
+}
+
+// inflowCheck (problem.cpp:941-1044) then fixPoles (:1046-1215).
+void torusBCs(const grim_oracle_grid& c, const grim_oracle_params& p, const grim_oracle_torus& t,
+              const Dims& D, const grim_oracle_geometry& gm, double* prim) {
+  const Vars vars(p);
+  const int64_t n = D.n;
+  const int g = D.g[0];
+  auto at = [&](int v, int i, int j, int k) -> double& { return prim[v * n + D.idx(i, j, k)]; };
+  auto geo = [&](int plane, int i, int j, int k) { return gm.center[plane * n + D.idx(i, j, k)]; };
+  const double vSqrMax = 1. - 1. / t.MaxLorentzFactor / t.MaxLorentzFactor;
+  const double vSqrMin = 1.e-13;
+  for (int side = 0; side < 2; side++) {
+    const int iLo = side == 0 ? 0 : D.N[0] + g, iHi = side == 0 ? g : D.N[0] + 2 * g;
+    int lo[3] = {iLo, 0, 0}, hi[3] = {iHi, D.NT[1], D.NT[2]};
+    forRange(lo, hi, [&](int i, int j, int k) {
+      const int64_t z = D.idx(i, j, k);
+      ZoneGeom<double> G;
+      loadLocation(gm.center, n, z, &G);
+      double P[GRIM_ORACLE_MAX_VARS];
+      loadPrim(prim, vars.dof, n, z, P);
+      FluidElement<double> e;
+      e.set(P, G, p, ZoneClosure{1., 1.});
+      double u[3] = {P[U1] / e.gammaLorentzFactor, P[U2] / e.gammaLorentzFactor,
+                     P[U3] / e.gammaLorentzFactor};
+      if (side == 1) {
+        // minUr of the last bulk column, same (j, k)
+        const int il = D.N[0] + g - 1;
+        const double minUr = geo(10 + 1, il, j, k) * geo(14, il, j, k);
+        u[0] = std::max(minUr, u[0]);
+      } else {
+        u[0] = std::min(G.gCon[0][1] * G.alpha, u[0]);
+      }
+      double vSqr = u[0] * 0.;
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) vSqr += G.gCov[a + 1][b + 1] * u[a] * u[b];
+      vSqr = std::max(std::min(vSqr, vSqrMax), vSqrMin);
+      const double newLorentzFactor = 1. / std::sqrt(1. - vSqr);
+      for (int a = 0; a < 3; a++) prim[(U1 + a) * n + z] = u[a] * newLorentzFactor;
+    });
+  }
+  if (D.dim < 2) return;
+  // fixPoles
+  const int g2 = D.g[1], N2 = D.N[1];
+  for (int pole = 0; pole < 2; pole++) {
+    const int idx0 = pole == 0 ? g2 : N2 + g2 - 1;
+    const int idx1 = pole == 0 ? g2 + 1 : idx0 - 1;
+    const int idx2 = pole == 0 ? g2 + 2 : idx0 - 2;
+    for (int k = 0; k < D.NT[2]; k++)
+      for (int i = 0; i < D.NT[0]; i++) {
+        auto Theta = [&](int j) {
+          const double th = geo(23, i, j, k);
+          return pole == 0 ? th : th * (-1.) + M_PI;
+        };
+        for (int jj : {idx0, idx1}) {
+          at(RHO, i, jj, k) = at(RHO, i, idx2, k);
+          at(UU, i, jj, k) = at(UU, i, idx2, k);
+          at(U1, i, jj, k) = at(U1, i, idx2, k);
+          at(U2, i, jj, k) = at(U2, i, idx2, k) * Theta(jj) / Theta(idx2);
+          at(U3, i, jj, k) = at(U3, i, idx2, k);
+          if (p.conduction) at(vars.Q, i, jj, k) = at(vars.Q, i, idx2, k) * Theta(jj) / Theta(idx2);
+          if (p.viscosity) at(vars.DP, i, jj, k) = at(vars.DP, i, idx2, k) * Theta(jj) / Theta(idx2);
+        }
+        for (int q = 0; q < g2; q++) {
+          const int dst = pole == 0 ? g2 - 1 - q : N2 + g2 + q;
+          const int src = pole == 0 ? g2 + q : N2 + g2 - 1 - q;
+          at(RHO, i, dst, k) = at(RHO, i, src, k);
+          at(UU, i, dst, k) = at(UU, i, src, k);
+          at(U1, i, dst, k) = at(U1, i, src, k);
+          at(U3, i, dst, k) = at(U3, i, src, k);
+          at(U2, i, dst, k) = at(U2, i, src, k) * (-1.0);
+          at(vars.B1, i, dst, k) = at(vars.B1, i, src, k);
+          at(vars.B3, i, dst, k) = at(vars.B3, i, src, k);
+          at(vars.B2, i, dst, k) = at(vars.B2, i, src, k) * (-1.0);
+          if (p.conduction) at(vars.Q, i, dst, k) = at(vars.Q, i, src, k) * (-1.0);
+          if (p.viscosity) at(vars.DP, i, dst, k) = at(vars.DP, i, src, k) * (-1.0);
+        }
+      }
+  }
+}
+
+// fullStepDiagnostics' NaN scrub (problem.cpp:722-741): when the bulk holds a
+// NaN, every zone holding one is reset to zero in all variables.
+void scrubNaNs(const Dims& D, int nv, double* prim) {
+  int64_t bulkNaN = 0;
+  for (int k = D.g[2]; k < D.g[2] + D.N[2]; k++)
+    for (int j = D.g[1]; j < D.g[1] + D.N[1]; j++)
+      for (int i = D.g[0]; i < D.g[0] + D.N[0]; i++)
+        for (int v = 0; v < nv; v++) bulkNaN += std::isnan(prim[v * D.n + D.idx(i, j, k)]) ? 1 : 0;
+  if (bulkNaN == 0) return;
+  for (int64_t z = 0; z < D.n; z++) {
+    bool bad = false;
+    for (int v = 0; v < nv; v++) bad = bad || std::isnan(prim[v * D.n + z]);
+    if (bad)
+      for (int v = 0; v < nv; v++) prim[v * D.n + z] = 0.;
+  }
+}
+
 // One stage of timeStep (timestep.cpp:14-107 half, 183-255 full).
 void stage(const grim_oracle_grid& c, const grim_oracle_params& p, const Dims& D,
-           const grim_oracle_geometry& gm, int st, double dt, double* primOld, double* primHalf,
-           double* consOld, double* primNew, int semantics, grim_oracle_step_stats* stats) {
+           const grim_oracle_geometry& gm, const grim_oracle_torus* torus, int st, double dt,
+           double* primOld, double* primHalf, double* consOld, double* primNew, int semantics,
+           grim_oracle_step_stats* stats) {
   const Vars vars(p);
   const int nv = vars.dof, nf = vars.numFluidVars;
   const int64_t n = D.n;
   const bool emhd = p.conduction || p.viscosity;
   // The stage state was communicated at the end of the previous solve (or
   // holds the initial conditions everywhere); only the physical boundaries
-  // are applied here, as timestep.cpp:17-22 / 186-191 do.
+  // and the problem's boundary hook are applied here, as timestep.cpp:17-22
+  // / 186-191 do.
   double* primStage = (st == 0) ? primOld : primHalf;
   applyBCs(c, D, nv, primStage);
+  if (torus) torusBCs(c, p, *torus, D, gm, primStage);
   const double dtPrime = (st == 0) ? dt / 2. : dt;
 
   std::vector<double> Sexp((size_t)nv * n, 0.), divF((size_t)nv * n, 0.);
-  std::vector<double> SimplOld((size_t)nv * n, 0.);
   explicitSources(c, p, D, gm, primStage, Sexp.data());
-  divFluxes(c, p, D, gm, primStage, divF.data(), nullptr);
+  divFluxes(c, p, D, gm, primStage, divF.data(), nullptr, torus != nullptr);
 
   std::vector<double> rhs((size_t)nf * n, 0.), mask(n, 0.);
   int lo[3] = {D.g[0], D.g[1], D.g[2]};
@@ -852,6 +1215,112 @@ void stage(const grim_oracle_grid& c, const grim_oracle_params& p, const Dims& D
     stats->maxIterations[st] = mx;
   }
   communicate(c, D, nv, primNew);
+  // halfStepDiagnostics / fullStepDiagnostics (torus/problem.cpp:714-741)
+  if (torus) {
+    if (st == 1) scrubNaNs(D, nv, primNew);
+    applyFloorAll(c, p, *torus, D, gm, primNew);
+  }
+}
+
+// timeStepper::initialConditions of the torus (problem.cpp:145-540).
+void torusIC(const grim_oracle_grid& c, const grim_oracle_params& p, const grim_oracle_torus& t,
+             const Dims& D, const grim_oracle_geometry& gm, double* prim) {
+  const Vars vars(p);
+  const int nv = vars.dof;
+  const int64_t n = D.n;
+  int lo[3] = {0, 0, 0}, hi[3] = {D.NT[0], D.NT[1], D.NT[2]};
+  const double dX2 = dXof(c, 1);
+  forRange(lo, hi, [&](int i, int j, int k) {
+    const int64_t z = D.idx(i, j, k);
+    double P[GRIM_ORACLE_MAX_VARS];
+    for (int v = 0; v < nv; v++) P[v] = 0.;
+    const double X2 = c.X2Start + ((double)j - D.g[1] + 0.5) * dX2;
+    const double beta[4] = {0., gm.center[11 * n + z], gm.center[12 * n + z], gm.center[13 * n + z]};
+    torusZone(c, p, t, gm.center[22 * n + z], gm.center[23 * n + z], X2, gm.center[14 * n + z], beta,
+              uniformAt(t.seed, (uint64_t)z), P);
+    for (int v = 0; v < nv; v++) prim[v * n + z] = P[v];
+  });
+  double rhoMax = -INFINITY;
+  for (int64_t z = 0; z < n; z++) rhoMax = std::max(rhoMax, prim[RHO * n + z]);
+  for (int64_t z = 0; z < n; z++) {
+    prim[RHO * n + z] = prim[RHO * n + z] / rhoMax;
+    prim[UU * n + z] = prim[UU * n + z] / rhoMax;
+  }
+  // Vector potential from the neighbour-averaged density (af::shift is
+  // circular), then B = curl A (problem.cpp:355-404).
+  std::vector<double> Avec(n, 0.);
+  const int64_t s1 = D.stride(0), s2 = D.stride(1), s3 = D.stride(2);
+  auto wrap = [&](int64_t z, int d, int off) {
+    int I[3] = {(int)(z % D.NT[0]), (int)((z / D.NT[0]) % D.NT[1]), (int)(z / ((int64_t)D.NT[0] * D.NT[1]))};
+    I[d] = (I[d] + off + D.NT[d]) % D.NT[d];
+    return D.idx(I[0], I[1], I[2]);
+  };
+  (void)s1;
+  (void)s2;
+  (void)s3;
+  for (int64_t z = 0; z < n; z++) {
+    const double* rho = prim + RHO * n;
+    double rhoAvg = rho[wrap(z, 0, -1)] + rho[wrap(z, 0, 1)] + rho[wrap(z, 1, -1)] + rho[wrap(z, 1, 1)];
+    if (D.dim > 2)
+      rhoAvg = (rhoAvg + rho[wrap(z, 2, -1)] + rho[wrap(z, 2, 1)]) / 6.;
+    else
+      rhoAvg = rhoAvg / 4.;
+    const double zero = rhoAvg * 0.;
+    if (t.UseMADdisk) {
+      const double Radius = gm.center[22 * n + z];
+      Avec[z] = std::max(zero, rhoAvg * rhoAvg * std::pow(Radius, 4));
+    } else {
+      Avec[z] = std::max(rhoAvg - 0.2, zero) * std::cos(gm.center[23 * n + z] * (t.MagneticLoops - 1));
+    }
+  }
+  const double dX1 = dXof(c, 0);
+  for (int64_t z = 0; z < n; z++) {
+    const double g = gm.center[15 * n + z];
+    const double A00 = Avec[z], A01 = Avec[wrap(z, 1, 1)], A10 = Avec[wrap(z, 0, 1)];
+    const double A11 = Avec[wrap(wrap(z, 0, 1), 1, 1)];
+    double B1 = (A01 - A00 + A11 - A10) / (2. * dX2 * g);
+    double B2 = (A00 - A10 + A01 - A11) / (2. * dX1 * g);
+    int I[3] = {(int)(z % D.NT[0]), (int)((z / D.NT[0]) % D.NT[1]), (int)(z / ((int64_t)D.NT[0] * D.NT[1]))};
+    const double mask = D.bulk(I[0], I[1], I[2]) ? 1. : 0.;
+    prim[vars.B1 * n + z] = B1 * mask;
+    prim[vars.B2 * n + z] = B2 * mask;
+    prim[vars.B3 * n + z] = 0.;
+  }
+  communicate(c, D, nv, prim);
+  double BFactor;
+  if (t.UseMADdisk) {
+    double PmagMax = -INFINITY, PgasMax = -INFINITY;
+    for (int64_t z = 0; z < n; z++) {
+      ZoneGeom<double> G;
+      loadLocation(gm.center, n, z, &G);
+      double P[GRIM_ORACLE_MAX_VARS];
+      loadPrim(prim, nv, n, z, P);
+      FluidElement<double> e;
+      e.set(P, G, p, ZoneClosure{1., 1.});
+      PmagMax = std::max(PmagMax, e.bSqr / 2.);
+      PgasMax = std::max(PgasMax, e.pressure);
+    }
+    BFactor = std::sqrt(PgasMax / PmagMax / t.MinPlasmaBeta);
+  } else {
+    double BetaMin = INFINITY;
+    for (int64_t z = 0; z < n; z++) {
+      ZoneGeom<double> G;
+      loadLocation(gm.center, n, z, &G);
+      double P[GRIM_ORACLE_MAX_VARS];
+      loadPrim(prim, nv, n, z, P);
+      FluidElement<double> e;
+      e.set(P, G, p, ZoneClosure{1., 1.});
+      BetaMin = std::min(BetaMin, 2. * (e.pressure + 1.e-13) / (e.bSqr + 1.e-18));
+    }
+    BFactor = std::sqrt(BetaMin / t.MinPlasmaBeta);
+  }
+  for (int64_t z = 0; z < n; z++)
+    for (int b = 0; b < 3; b++) prim[(vars.B1 + b) * n + z] *= BFactor;
+  // Q and dP start at zero (set above); floors, then fullStepDiagnostics'
+  // scrub and floors (problem.cpp:525-538).
+  applyFloorAll(c, p, t, D, gm, prim);
+  scrubNaNs(D, nv, prim);
+  applyFloorAll(c, p, t, D, gm, prim);
 }
 
 }  // namespace
@@ -890,7 +1359,7 @@ void grim_oracle_build_geometry(const grim_oracle_grid* c, grim_oracle_geometry*
       C.XCoordsToxCoords(X, x);
       C.gCovAt(X, G);
       gDetAndgCon(G, &det, Cn);
-      storeLocation(G, Cn, det, x[0], n, D.idx(i, j, k), outs[loc]);
+      storeLocation(G, Cn, det, x[0], x[1], n, D.idx(i, j, k), outs[loc]);
     });
   }
   if (gm->gamma) {
@@ -992,31 +1461,54 @@ double grim_oracle_compute_dt(const grim_oracle_grid* c, const grim_oracle_param
 }
 
 void grim_oracle_stage(const grim_oracle_grid* c, const grim_oracle_params* p,
-                       const grim_oracle_geometry* gm, int st, double dt, double* primOld,
-                       double* primHalf, double* consOld, double* primNew, int semantics,
-                       grim_oracle_step_stats* stats) {
+                       const grim_oracle_geometry* gm, const grim_oracle_torus* t, int st,
+                       double dt, double* primOld, double* primHalf, double* consOld,
+                       double* primNew, int semantics, grim_oracle_step_stats* stats) {
   const Dims D(*c);
-  stage(*c, *p, D, *gm, st, dt, primOld, primHalf, consOld, primNew, semantics, stats);
+  stage(*c, *p, D, *gm, t, st, dt, primOld, primHalf, consOld, primNew, semantics, stats);
 }
 
 void grim_oracle_time_step(const grim_oracle_grid* c, const grim_oracle_params* p,
-                           const grim_oracle_geometry* gm, double* primOld, double* primDt,
-                           double* dt, double* time, int semantics,
-                           grim_oracle_step_stats* stats) {
+                           const grim_oracle_geometry* gm, const grim_oracle_torus* t,
+                           double* primOld, double* primDt, double* dt, double* time,
+                           int semantics, grim_oracle_step_stats* stats) {
   const Dims D(*c);
   const Vars vars(*p);
   const size_t sz = (size_t)vars.dof * D.n;
   *dt = computeDt(*c, *p, D, *gm, primDt, *dt);
   if (stats) stats->dt = *dt;
   std::vector<double> primHalf(sz), consOld(sz), primNew(sz);
-  stage(*c, *p, D, *gm, 0, *dt, primOld, nullptr, consOld.data(), primHalf.data(), semantics,
+  stage(*c, *p, D, *gm, t, 0, *dt, primOld, nullptr, consOld.data(), primHalf.data(), semantics,
         stats);
-  // elemOld of the next computeDt is the boundary-filled primOld of this step
-  std::copy(primOld, primOld + sz, primDt);
-  stage(*c, *p, D, *gm, 1, *dt, primOld, primHalf.data(), consOld.data(), primNew.data(),
+  stage(*c, *p, D, *gm, t, 1, *dt, primOld, primHalf.data(), consOld.data(), primNew.data(),
         semantics, stats);
   std::copy(primNew.begin(), primNew.end(), primOld);
+  // elemOld of the next computeDt: the problem's fullStepDiagnostics set it
+  // from this step's final primOld (linear_modes/problem.cpp:182,
+  // torus/problem.cpp:743 via applyFloor).
+  std::copy(primOld, primOld + sz, primDt);
   *time += *dt;
+}
+
+void grim_oracle_torus_ic(const grim_oracle_grid* c, const grim_oracle_params* p,
+                          const grim_oracle_torus* t, const grim_oracle_geometry* gm,
+                          double* prim) {
+  const Dims D(*c);
+  torusIC(*c, *p, *t, D, *gm, prim);
+}
+
+void grim_oracle_apply_floor(const grim_oracle_grid* c, const grim_oracle_params* p,
+                             const grim_oracle_torus* t, const grim_oracle_geometry* gm,
+                             double* prim) {
+  const Dims D(*c);
+  applyFloorAll(*c, *p, *t, D, *gm, prim);
+}
+
+void grim_oracle_torus_bcs(const grim_oracle_grid* c, const grim_oracle_params* p,
+                           const grim_oracle_torus* t, const grim_oracle_geometry* gm,
+                           double* prim) {
+  const Dims D(*c);
+  torusBCs(*c, *p, *t, D, *gm, prim);
 }
 
 }  // extern "C"

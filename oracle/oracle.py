@@ -86,7 +86,7 @@ RECON_WENO5 = 1
 RECON_PPM = 2
 RIEMANN_HLL = 0
 RIEMANN_LLF = 1
-LOC_PLANES = 23
+LOC_PLANES = 24
 
 
 class Grid(C.Structure):
@@ -182,8 +182,12 @@ def lib():
         L.grim_oracle_explicit_sources.argtypes = [Gp, P, Gm, vp, vp]
         L.grim_oracle_compute_dt.argtypes = [Gp, P, Gm, vp, d]
         L.grim_oracle_compute_dt.restype = d
-        L.grim_oracle_stage.argtypes = [Gp, P, Gm, C.c_int, d, vp, vp, vp, vp, C.c_int, C.POINTER(StepStats)]
-        L.grim_oracle_time_step.argtypes = [Gp, P, Gm, vp, vp, vp, vp, C.c_int, C.POINTER(StepStats)]
+        Tp = C.POINTER(Torus)
+        L.grim_oracle_stage.argtypes = [Gp, P, Gm, Tp, C.c_int, d, vp, vp, vp, vp, C.c_int, C.POINTER(StepStats)]
+        L.grim_oracle_time_step.argtypes = [Gp, P, Gm, Tp, vp, vp, vp, vp, C.c_int, C.POINTER(StepStats)]
+        L.grim_oracle_torus_ic.argtypes = [Gp, P, Tp, Gm, vp]
+        L.grim_oracle_apply_floor.argtypes = [Gp, P, Tp, Gm, vp]
+        L.grim_oracle_torus_bcs.argtypes = [Gp, P, Tp, Gm, vp]
         _lib = L
     return _lib
 
@@ -438,30 +442,80 @@ def compute_dt(g: Grid, p: Params, gm: Geometry, prim, dt: float) -> float:
 
 
 def stage(g: Grid, p: Params, gm: Geometry, st: int, dt: float, prim_old, prim_half=None, cons_old=None,
-          semantics=NEWTON_PER_ZONE):
+          semantics=NEWTON_PER_ZONE, torus: "Torus | None" = None):
     """One stage; returns (prim_new, cons_old, stats); prim_old/prim_half get their boundaries applied."""
     cons_old = np.zeros((p.dof, g.n)) if cons_old is None else cons_old
     new = np.zeros((p.dof, g.n))
     st_ = StepStats()
-    lib().grim_oracle_stage(C.byref(g), C.byref(p), C.byref(gm._c()), st, dt, _ptr(prim_old), _ptr(prim_half),
-                            _ptr(cons_old), _ptr(new), semantics, C.byref(st_))
+    lib().grim_oracle_stage(C.byref(g), C.byref(p), C.byref(gm._c()), C.byref(torus) if torus else None, st, dt,
+                            _ptr(prim_old), _ptr(prim_half), _ptr(cons_old), _ptr(new), semantics, C.byref(st_))
     return new, cons_old, st_
 
 
 class Simulation:
-    """timeStepper state: primOld, the state elemOld was set from, dt, time."""
+    """timeStepper state: primOld, the state elemOld was set from, dt, time
+    (and the torus problem's hooks when torus is given)."""
 
-    def __init__(self, g: Grid, p: Params, prim, dt: float, gm: Geometry | None = None, time: float = 0.):
+    def __init__(self, g: Grid, p: Params, prim, dt: float, gm: Geometry | None = None, time: float = 0.,
+                 torus: "Torus | None" = None):
         self.g, self.p = g, p
         self.gm = gm if gm is not None else Geometry(g)
         self.prim = _f64(prim).copy()
         self.prim_dt = self.prim.copy()
         self.dt = C.c_double(dt)
         self.time = C.c_double(time)
+        self.torus = torus
 
     def step(self, semantics=NEWTON_PER_ZONE) -> StepStats:
         st = StepStats()
-        lib().grim_oracle_time_step(C.byref(self.g), C.byref(self.p), C.byref(self.gm._c()), _ptr(self.prim),
+        lib().grim_oracle_time_step(C.byref(self.g), C.byref(self.p), C.byref(self.gm._c()),
+                                    C.byref(self.torus) if self.torus else None, _ptr(self.prim),
                                     _ptr(self.prim_dt), C.byref(self.dt), C.byref(self.time), semantics,
                                     C.byref(st))
         return st
+
+
+# ---------------------------------------------------------------------------
+# Torus problem
+# ---------------------------------------------------------------------------
+class Torus(C.Structure):
+    """Mirror of grim_oracle_torus (src/problem/torus/params.cpp)."""
+
+    _fields_ = [
+        ("InnerEdgeRadius", C.c_double), ("PressureMaxRadius", C.c_double), ("Adiabat", C.c_double),
+        ("MinPlasmaBeta", C.c_double), ("MagneticLoops", C.c_int), ("UseMADdisk", C.c_int),
+        ("InitialPerturbationAmplitude", C.c_double), ("MaxLorentzFactor", C.c_double),
+        ("RhoFloorAmpl", C.c_double), ("UFloorAmpl", C.c_double), ("RhoFloorSlope", C.c_double),
+        ("UFloorSlope", C.c_double), ("BsqrOverRhoMax", C.c_double), ("BsqrOverUMax", C.c_double),
+        ("seed", C.c_uint64), ("minRho", C.c_void_p), ("minU", C.c_void_p),
+    ]
+
+
+def torus_params(**kw) -> Torus:
+    """The SANE single-loop torus: r_in = 6, r_max = 12, kappa = 1e-3, beta_min = 100,
+    4% noise, grim's floors (torus/params.cpp:38-80), seed 20261202."""
+    t = Torus(InnerEdgeRadius=6., PressureMaxRadius=12., Adiabat=1e-3, MinPlasmaBeta=100., MagneticLoops=1,
+              UseMADdisk=0, InitialPerturbationAmplitude=4e-2, MaxLorentzFactor=10., RhoFloorAmpl=1e-3,
+              UFloorAmpl=1e-5, RhoFloorSlope=-2.5, UFloorSlope=-2.5, BsqrOverRhoMax=10., BsqrOverUMax=500.,
+              seed=20261202)
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
+
+
+def torus_ic(g: Grid, p: Params, t: Torus, gm: Geometry):
+    out = np.zeros((p.dof, g.n))
+    lib().grim_oracle_torus_ic(C.byref(g), C.byref(p), C.byref(t), C.byref(gm._c()), _ptr(out))
+    return out
+
+
+def apply_floor(g: Grid, p: Params, t: Torus, gm: Geometry, prim):
+    prim = _f64(prim).copy()
+    lib().grim_oracle_apply_floor(C.byref(g), C.byref(p), C.byref(t), C.byref(gm._c()), _ptr(prim))
+    return prim
+
+
+def torus_bcs(g: Grid, p: Params, t: Torus, gm: Geometry, prim):
+    prim = _f64(prim).copy()
+    lib().grim_oracle_torus_bcs(C.byref(g), C.byref(p), C.byref(t), C.byref(gm._c()), _ptr(prim))
+    return prim

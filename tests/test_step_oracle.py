@@ -443,3 +443,106 @@ def test_emhd_linear_mode_matches_notebook_table():
             assert NOTEBOOK[N][k] / 5 < v < NOTEBOOK[N][k] * 5, (N, k, v, NOTEBOOK[N][k])
     for k in errs[32]:
         assert errs[32][k] / errs[64][k] > 3., (k, errs[32][k], errs[64][k])
+
+
+# ---------------------------------------------------------------------------
+# Torus problem (src/problem/torus/problem.cpp)
+# ---------------------------------------------------------------------------
+def torus_grid(N1=96, N2=32, N3=4):
+    a = 0.9375
+    Rin = 0.85 * (1 + np.sqrt(1 - a * a))
+    return O.grid(N1=N1, N2=N2, N3=N3, dim=3, metric=O.METRIC_MKS, blackHoleSpin=a, X1Start=np.log(Rin),
+                  X1End=np.log(3000.), X2Start=1e-8, X2End=1 - 1e-8, X3Start=0., X3End=2 * np.pi,
+                  DerefineThetaHorizon=1, DoCylindrify=1, X1cyl=np.log(4 * Rin), X2cyl=1. / N2,
+                  boundaryLeft=O.BC_OUTFLOW, boundaryRight=O.BC_OUTFLOW, boundaryBottom=O.BC_MIRROR,
+                  boundaryTop=O.BC_MIRROR, reconstruction=O.RECON_WENO5, riemannSolver=O.RIEMANN_LLF)
+
+
+def torus_setup(emhd, size=(96, 32, 4), **tkw):
+    g = torus_grid(*size)
+    p = O.default_params(emhd)
+    p.adiabaticIndex = 5. / 3
+    p.closure = O.CLOSURE_TORUS
+    gm = O.Geometry(g)
+    t = O.torus_params(**tkw)
+    return g, p, gm, t
+
+
+def test_torus_initial_conditions():
+    """Fishbone-Moncrief torus: density normalised to 1 at its maximum, the
+    loop field normalised to beta_min = 100 (problem.cpp:343-504), floors
+    respected, equatorial symmetry without noise, field confined to the torus."""
+    # At N2 >= 64 the loop field stays inside the torus (at coarser grids the
+    # curl of A reaches zones of the 1e-20 atmosphere, whose tiny pressure
+    # then sets beta_min before the floors raise it, as in the reference).
+    g, p, gm, t = torus_setup(False, size=(128, 64, 2), InitialPerturbationAmplitude=0.)
+    P = O.torus_ic(g, p, t, gm)
+    b = lambda a: planes(g, a)[g.bulk()]  # noqa: E731
+    rho, B = b(P)[0], b(P)[5:8]
+    assert rho.max() == pytest.approx(1., rel=1e-12)
+    el = O.element(p, P, gm.center[:16], radius=gm.center[22])
+    beta = 2 * (el["pressure"] + 1e-13) / (el["bSqr"] + 1e-18)
+    assert beta.min() == pytest.approx(100., rel=1e-9)
+    r = gm.center[22]
+    assert np.all(P[0] >= 1e-3 * r ** -2.5 * (1 - 1e-15))
+    # equatorial mirror symmetry of rho and u (rows j and N2T-1-j)
+    full = planes(g, P)
+    np.testing.assert_allclose(full[0], full[0][:, ::-1, :], rtol=1e-9, atol=1e-14)
+    assert np.abs(B).max() > 0 and np.all(np.isfinite(P))
+
+
+def test_torus_floor_invariants():
+    g, p, gm, t = torus_setup(True)
+    P = O.torus_ic(g, p, t, gm)
+    rng = np.random.default_rng(3)
+    Q = P.copy()
+    Q[0] *= rng.uniform(0.01, 2., g.n)
+    Q[2:5] *= 50.
+    F = O.apply_floor(g, p, t, gm, Q)
+    r = gm.center[22]
+    assert np.all(F[0] >= 1e-3 * r ** -2.5 * (1 - 1e-15))
+    assert np.all(F[1] >= 1e-5 * r ** -2.5 * (1 - 1e-15))
+    el = O.element(p, F, gm.center[:16], radius=gm.center[22])
+    gamma = el["uCon0"] * gm.center[14]  # u^0 alpha
+    assert gamma.max() <= 10. * (1 + 1e-12)
+    # idempotent up to rounding where nothing is floored
+    F2 = O.apply_floor(g, p, t, gm, F)
+    np.testing.assert_allclose(F2, F, rtol=1e-9, atol=1e-300)
+
+
+def test_torus_boundary_hook_identities():
+    """fixPoles: the two active rows next to each pole copy the third (u^2,
+    q, dP scaled by theta), ghosts mirror with u^2, B^2, q, dP flipped;
+    inflowCheck: no radial inflow through the outer ghosts."""
+    g, p, gm, t = torus_setup(True)
+    P = O.torus_ic(g, p, t, gm)
+    rng = np.random.default_rng(5)
+    P[2:5] += 0.01 * rng.standard_normal((3, g.n))
+    out = planes(g, O.torus_bcs(g, p, t, gm, P))
+    th = planes(g, gm.center[23:24])[0]
+    ng = 3
+    for v in (0, 1, 2, 4):
+        np.testing.assert_array_equal(out[v][:, ng], out[v][:, ng + 2])
+        np.testing.assert_array_equal(out[v][:, ng + 1], out[v][:, ng + 2])
+    np.testing.assert_allclose(out[3][:, ng], out[3][:, ng + 2] * th[:, ng] / th[:, ng + 2], rtol=1e-15)
+    for q in range(ng):
+        for v, s in ((0, 1), (1, 1), (2, 1), (4, 1), (3, -1), (5, -1), (6, -1), (7, 1), (8, -1), (9, 1)):
+            np.testing.assert_array_equal(out[v][:, ng - 1 - q], s * out[v][:, ng + q])
+    # outer radial ghosts: u^r at least the lapse-weighted shift of the last active zone
+    gc = planes(g, gm.center)
+    il = g.N1 + ng - 1
+    ur = out[2][:, :, il + 1:]
+    lf = np.sqrt(1 + 0)  # rescaled by the new Lorentz factor, sign preserved
+    assert np.all(ur * lf >= (gc[11][:, :, il] * gc[14][:, :, il])[..., None] - 1e-12)
+
+
+@pytest.mark.parametrize("emhd", [False, True])
+def test_torus_steps_run(emhd):
+    g, p, gm, t = torus_setup(emhd)
+    p.maxNonLinearIter = 5
+    P = O.torus_ic(g, p, t, gm)
+    sim = O.Simulation(g, p, P, dt=0.005, gm=gm, torus=t)
+    for _ in range(2):
+        st = sim.step()
+    assert np.all(np.isfinite(sim.prim))
+    assert st.dt > 0.005
