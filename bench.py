@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--phase1", type=int, default=2)
     ap.add_argument("--phase2-group", type=int, default=4)
+    ap.add_argument("--no-full-step", action="store_true", help="skip the full time-step workloads")
+    ap.add_argument("--torus", type=int, nargs=3, default=[192, 192, 128], metavar=("N1", "N2", "N3"))
+    ap.add_argument("--torus-steps", type=int, default=20)
     ap.add_argument("--paths", default=",".join(["fd_fp64", "fd_fp64_fused", "analytic_fp64", "mixed",
                                                  "mixed_compacted"]),
                     help="comma-separated subset of the Newton paths to time (profiling)")
@@ -242,9 +245,115 @@ def main():
         "paths": results,
     }
 
+    if not args.no_full_step:
+        del b, rhs, mask, prim, batch, ws, finals
+        torch.cuda.empty_cache()
+        out["full_step"] = full_step_bench(args)
     if not args.no_cpu:
+        p = capi.default_params(False)
+        p.maxNonLinearIter = 30
+        b = capi.generate_kerr_schild_batch(p, n, SEED)
+        rhs = torch.empty((p.nf, n), dtype=torch.float64, device="cuda")
+        mask = torch.empty(n, dtype=torch.float64, device="cuda")
+        capi.known_root_rhs(p, capi.make_batch(b["prim_star"], b["geom"], rhs), rhs, mask)
         out["cpu_baseline"] = cpu_baseline(p, b, rhs, mask, n, args.cpu_reps)
     print(json.dumps(out), flush=True)
+
+
+def torus_config(N1, N2, N3):
+    """Grid of the 3D Kerr torus workload (src/problem/torus/params.cpp:51-105 with
+    a = 0.9375, WENO5 + LLF)."""
+    import numpy as np
+
+    from grim_amd import capi
+    from grim_amd.timestepper import grid_config
+
+    a = 0.9375
+    Rin = 0.85 * (1 + np.sqrt(1 - a * a))
+    return grid_config(N1=N1, N2=N2, N3=N3, dim=3, metric=capi.METRIC_MKS, blackHoleSpin=a, X1Start=np.log(Rin),
+                       X1End=np.log(3000.), X2Start=1e-8, X2End=1 - 1e-8, X3Start=0., X3End=2 * np.pi,
+                       DerefineThetaHorizon=1, DoCylindrify=1, X1cyl=np.log(4 * Rin), X2cyl=1. / N2,
+                       boundaryLeft=capi.BC_OUTFLOW, boundaryRight=capi.BC_OUTFLOW, boundaryBottom=capi.BC_MIRROR,
+                       boundaryTop=capi.BC_MIRROR, boundaryBack=capi.BC_PERIODIC, boundaryFront=capi.BC_PERIODIC,
+                       reconstruction=capi.RECON_WENO5, riemannSolver=capi.RIEMANN_LLF, residualScaling=1)
+
+
+def time_steps(ts, steps, warmup):
+    """Wall time of `steps` grim_time_step calls after `warmup` (each step syncs
+    once for dt and once per stage for its statistics)."""
+    import torch
+
+    for _ in range(warmup):
+        ts.timeStep()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = [ts.timeStep() for _ in range(steps)]
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, stats
+
+
+def full_step_bench(args):
+    """The whole time step (timeStepper::timeStep) on two workloads:
+    the 3D EMHD Kerr torus at 192x192x128 with both Newton paths, and the 2D
+    64x64 Minkowski linear fast wave (latency-bound)."""
+    import torch
+
+    from grim_amd import capi
+    from grim_amd.timestepper import TimeStepper, grid_config
+
+    res = {}
+    N1, N2, N3 = args.torus
+    cfg = torus_config(N1, N2, N3)
+    zones = N1 * N2 * N3
+    torus = {}
+    for name, opts in (("fd_fp64", capi.newton_opts()),
+                       ("mixed", capi.newton_opts(jacobian=capi.JAC_ANALYTIC, lu_precision=capi.LU_FP32))):
+        p = capi.default_params(True)
+        p.adiabaticIndex = 5. / 3
+        p.closure = capi.CLOSURE_TORUS
+        p.maxNonLinearIter = 10
+        print(f"# full step: torus {N1}x{N2}x{N3} EMHD, {name}", file=sys.stderr, flush=True)
+        ts = TimeStepper(p, cfg, opts=opts)
+        ts.initTorus(capi.torus_params())
+        ts.set_time(0., 0.005)
+        ms, stats = time_steps(ts, args.torus_steps, 2)
+        torus[name] = {
+            "ms_per_step": ms,
+            "zone_updates_per_s": zones / (ms * 1e-3),
+            "unconverged_last_step": [int(stats[-1].unconverged[0]), int(stats[-1].unconverged[1])],
+            "max_iterations_last_step": [int(stats[-1].max_iterations[0]), int(stats[-1].max_iterations[1])],
+            "dt_last": stats[-1].dt,
+        }
+        if name == "fd_fp64":
+            fd_prim = ts.primOld
+        else:
+            import numpy as np
+
+            d = np.abs(ts.primOld - fd_prim) / (np.abs(fd_prim) + 1e-30)
+            g = N1 + 6, N2 + 6, N3 + 6
+            d = d.reshape(p.dof, g[2], g[1], g[0])[:, 3:-3, 3:-3, 3:-3]
+            torus[name]["prim_err_vs_fd_fp64_p99"] = float(np.percentile(d, 99))
+            torus[name]["prim_err_vs_fd_fp64_median"] = float(np.median(d))
+        ts.close()
+        del ts
+        torch.cuda.empty_cache()
+    torus["speedup_mixed_vs_fd_fp64"] = torus["mixed"]["zone_updates_per_s"] / torus["fd_fp64"]["zone_updates_per_s"]
+    torus["config"] = (f"3D MKS torus {N1}x{N2}x{N3} (a=0.9375, r_in=6, r_max=12, SANE beta_min=100), EMHD "
+                       f"(torus closure), WENO5+LLF, {args.torus_steps} steps after 2 warm-up, per-zone Newton to "
+                       "relative residual 1e-10 (maxNonLinearIter 10)")
+    res["torus_emhd"] = torus
+
+    p = capi.default_params(False)
+    p.maxNonLinearIter = 10
+    ts = TimeStepper(p, grid_config(N1=64, N2=64, reconstruction=capi.RECON_WENO5, riemannSolver=capi.RIEMANN_LLF))
+    ts.initLinearMode(0, 1e-4)
+    ts.set_time(0., 0.5 / 64)
+    ms, stats = time_steps(ts, 10, 2)
+    res["linear_wave_64x64"] = {"ms_per_step": ms, "zone_updates_per_s": 64 * 64 / (ms * 1e-3),
+                                "config": "2D Minkowski fast magnetosonic wave 64x64, WENO5+LLF, FD-FP64 Newton, "
+                                          "10 steps after 2 warm-up"}
+    ts.close()
+    return res
 
 
 def cpu_baseline(p, b, rhs, mask, n, reps):

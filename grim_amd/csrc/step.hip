@@ -395,13 +395,15 @@ __global__ __launch_bounds__(kBlock) void foldKernel(StepArgs a, Box box, const 
                                                      const double* __restrict__ sImp, double dtPrime,
                                                      double* __restrict__ divOut,
                                                      double* __restrict__ rhs,
-                                                     double* __restrict__ primNew) {
+                                                     double* __restrict__ primNew,
+                                                     double* __restrict__ mask) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= box.count()) return;
   int i, j, k;
   box.at(t, i, j, k);
   const Dims& D = a.D;
   const int64_t n = D.n, z = flatIndex(D, i, j, k);
+  double rmax = 0.;
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) {
     const double div = divergence<L>(D, a.dX, F, emf, v, z, i, j, a.fluxFilter);
@@ -410,12 +412,15 @@ __global__ __launch_bounds__(kBlock) void foldKernel(StepArgs a, Box box, const 
     const double U = consOld[v * n + z];
     if (v < L::NF) {
       const double si = L::EMHD ? sImp[v * n + z] : 0.;
-      rhs[v * n + z] = U - dtPrime * ((div + sExp[v * n + z]) + 0.5 * si);
+      const double r = U - dtPrime * ((div + sExp[v * n + z]) + 0.5 * si);
+      rhs[v * n + z] = r;
+      rmax = stdmax(rmax, fabs(r));
     } else {
       const double consB = U - dtPrime * div;
       primNew[v * n + z] = consB / geomC[GEOM_G * n + z];
     }
   }
+  if (rhs && mask) mask[z] = 1. / rmax;
 }
 
 // Post-CT flux arrays for the faces of the bulk zones (API output only).
@@ -1255,7 +1260,7 @@ int stage(grim_ctx* x, int st, double dt, grim_step_stats* stats) {
   const Box b = bulkBox(D);
   foldKernel<L><<<blocksFor(b.count()), kBlock, 0, x->s>>>(x->a, b, x->fluxes, x->emf, x->geomC, x->consOld,
                                                            x->sExp, x->sImp, dtPrime, nullptr, x->rhs,
-                                                           x->primNew);
+                                                           x->primNew, x->c.residualScaling ? x->mask : nullptr);
   if (int rc = checkLaunch("fold")) return rc;
 
   grim_zone_batch zb = {};
@@ -1582,7 +1587,8 @@ int grim_div_fluxes(grim_ctx* x, const double* prim, double* div_f, double* flux
     if (int rc = fluxesAndEmf<L>(x, prim)) return rc;
     const Box b = bulkBox(x->D);
     foldKernel<L><<<blocksFor(b.count()), kBlock, 0, x->s>>>(x->a, b, x->fluxes, x->emf, x->geomC, nullptr,
-                                                             nullptr, nullptr, 0., div_f, nullptr, nullptr);
+                                                             nullptr, nullptr, 0., div_f, nullptr, nullptr,
+                                                             nullptr);
     if (fluxes) {
       GRIM_HIP_TRY(hipMemcpyAsync(fluxes, x->fluxes, sizeof(double) * (size_t)L::DOF * x->D.dim * x->D.n,
                                   hipMemcpyDeviceToDevice, x->s));

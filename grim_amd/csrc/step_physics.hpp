@@ -335,11 +335,11 @@ struct CoordMap {
   int metric, derefine, cylindrify;
   double a, hSlope, X1Start, X1cyl, X2cyl;
 
-  __device__ double GammieRadius(double X1) const { return exp(X1); }
-  __device__ double GammieTheta(double X2) const {
+  __host__ __device__ double GammieRadius(double X1) const { return exp(X1); }
+  __host__ __device__ double GammieTheta(double X2) const {
     return M_PI * X2 + 0.5 * (1 - hSlope) * sin(2. * M_PI * X2);
   }
-  __device__ double ThetaNoCyl(double X1, double X2) const {
+  __host__ __device__ double ThetaNoCyl(double X1, double X2) const {
     if (derefine) {
       const double cTh0 = cos(M_PI * X2);
       const double cTh1 = cos(GammieTheta(X2));
@@ -348,7 +348,7 @@ struct CoordMap {
     }
     return GammieTheta(X2);
   }
-  __device__ static double Fangle(double x) {
+  __host__ __device__ static double Fangle(double x) {
     const double condition1 = (x <= -1) ? 1. : 0.;
     const double condition2 = (x >= 1) ? 1. : 0.;
     return condition2 * x +
@@ -359,27 +359,27 @@ struct CoordMap {
                     (64. * M_PI)) /
                2.;
   }
-  __device__ static double limlin(double x, double x0, double dx, double y0) {
+  __host__ __device__ static double limlin(double x, double x0, double dx, double y0) {
     return y0 - dx * Fangle((x - x0) * (-1.) / dx);
   }
-  __device__ static double mins(double f1, double f2, double df) { return limlin(f1, f2, df, f2); }
-  __device__ static double maxs(double f1, double f2, double df) { return (mins(-f1, -f2, df) * (-1.)); }
-  __device__ static double minmaxs(double f1, double f2, double df, double dir) {
+  __host__ __device__ static double mins(double f1, double f2, double df) { return limlin(f1, f2, df, f2); }
+  __host__ __device__ static double maxs(double f1, double f2, double df) { return (mins(-f1, -f2, df) * (-1.)); }
+  __host__ __device__ static double minmaxs(double f1, double f2, double df, double dir) {
     const double condition = (dir >= 0) ? 1. : 0.;
     return condition * maxs(f1, f2, df) + (1 - condition) * mins(f1, f2, df);
   }
-  __device__ double sinth1in(const double* X0, const double* X) const {
+  __host__ __device__ double sinth1in(const double* X0, const double* X) const {
     return GammieRadius(X0[0]) * sin(ThetaNoCyl(X0[0], X[1])) / GammieRadius(X[0]);
   }
-  __device__ double th2in(const double* X0, const double* X) const {
+  __host__ __device__ double th2in(const double* X0, const double* X) const {
     const double th0 =
         asin(GammieRadius(X0[0]) * sin(ThetaNoCyl(X0[0], X0[1])) / GammieRadius(X[0]));
     return (ThetaNoCyl(X[0], X[1]) - ThetaNoCyl(X[0], X0[1])) /
                (ThetaNoCyl(X[0], 0.5) - ThetaNoCyl(X[0], X0[1])) * (ThetaNoCyl(X[0], 0.5) - th0) +
            th0;
   }
-  __device__ double func1(const double* X) const { return sin(ThetaNoCyl(X[0], X[1])); }
-  __device__ double func2(const double* X0, const double* X) const {
+  __host__ __device__ double func1(const double* X) const { return sin(ThetaNoCyl(X[0], X[1])); }
+  __host__ __device__ double func2(const double* X0, const double* X) const {
     const double Xca[3] = {X[0], 0., X[2]};
     const double sth1in = sinth1in(X0, X);
     const double sth2in = sin(th2in(X0, X));
@@ -388,7 +388,7 @@ struct CoordMap {
     return minmaxs(sth1in, sth2in, fabs(sth2inaxis - sth1inaxis) + 1.e-20, X[0] - X0[0]);
   }
 
-  __device__ void toX(const double* X, double* x) const {
+  __host__ __device__ void toX(const double* X, double* x) const {
     if (metric == GRIM_METRIC_MINKOWSKI) {
       x[0] = X[0];
       x[1] = X[1];
@@ -421,7 +421,7 @@ struct CoordMap {
     x[1] = IsMirrored * (M_PI - th) + (1 - IsMirrored) * th;
   }
 
-  __device__ void gCov(const double* X, double G[4][4]) const {
+  __host__ __device__ void gCov(const double* X, double G[4][4]) const {
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
@@ -476,7 +476,7 @@ struct CoordMap {
 
 // geometry::setgDetAndgConFromgCov (geometry.cpp:123-253): determinant by the
 // Leibniz expansion, inverse by cofactors.
-__device__ __forceinline__ void gDetAndgCon(const double G[4][4], double& det, double C[4][4]) {
+__host__ __device__ __forceinline__ void gDetAndgCon(const double G[4][4], double& det, double C[4][4]) {
   const double d =
       G[0][0] * G[1][1] * G[2][2] * G[3][3] + G[0][0] * G[1][2] * G[2][3] * G[3][1] +
       G[0][0] * G[1][3] * G[2][1] * G[3][2] + G[0][1] * G[1][0] * G[2][3] * G[3][2] +

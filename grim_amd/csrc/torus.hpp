@@ -11,21 +11,21 @@
 namespace grim {
 
 // problem.cpp:56-143 (M = 1)
-__device__ inline double lFishboneMoncrief(double a, double r) {
+__host__ __device__ inline double lFishboneMoncrief(double a, double r) {
   const double M = 1.;
   return sqrt(M / pow(r, 3.)) *
          (pow(r, 4.) + r * r * a * a - 2. * M * r * a * a - a * sqrt(M * r) * (r * r - a * a)) /
          (r * r - 3 * M * r + 2. * a * sqrt(M * r));
 }
-__device__ inline double computeDelta(double a, double r) { return r * r - 2 * 1. * r + a * a; }
-__device__ inline double computeSigma(double a, double r, double theta) {
+__host__ __device__ inline double computeDelta(double a, double r) { return r * r - 2 * 1. * r + a * a; }
+__host__ __device__ inline double computeSigma(double a, double r, double theta) {
   return r * r + a * a * cos(theta) * cos(theta);
 }
-__device__ inline double computeA(double a, double r, double theta) {
+__host__ __device__ inline double computeA(double a, double r, double theta) {
   const double Delta = computeDelta(a, r);
   return pow(r * r + a * a, 2.) - Delta * a * a * sin(theta) * sin(theta);
 }
-__device__ inline double computeLnOfh(const grim_torus_params& t, double a, double r, double theta) {
+__host__ __device__ inline double computeLnOfh(const grim_torus_params& t, double a, double r, double theta) {
   const double l = lFishboneMoncrief(a, t.PressureMaxRadius);
   auto term1 = [&](double rr, double th) {
     const double Delta = computeDelta(a, rr), Sigma = computeSigma(a, rr, th), A = computeA(a, rr, th);
@@ -48,7 +48,7 @@ __device__ inline double computeLnOfh(const grim_torus_params& t, double a, doub
 }
 
 // Fluid state of one zone of the torus initial conditions (problem.cpp:175-322).
-__device__ inline void torusZone(const grim_torus_params& t, double aBH, double hSlope, double gam,
+__host__ __device__ inline void torusZone(const grim_torus_params& t, double aBH, double hSlope, double gam,
                                  double rhoFloor, double uFloor, double r, double theta, double X2,
                                  double lapse, const double* beta, double xi, double* P) {
   double lnOfh = 1.;
@@ -112,7 +112,7 @@ __device__ inline void torusZone(const grim_torus_params& t, double aBH, double 
 // applies it when any zone exceeds the limit, and the factor of every other
 // zone is exactly 1.
 template <class L>
-__device__ inline void applyFloorZone(const DevParams& prm, const grim_torus_params& t, const GeomFull& G,
+__host__ __device__ inline void applyFloorZone(const DevParams& prm, const grim_torus_params& t, const GeomFull& G,
                                       double minRho, double minU, double* P) {
   const ZoneGeom& zg = G.zg;
   const double rho_prefloor = P[RHO] * 1.;

@@ -234,6 +234,12 @@ typedef struct grim_grid_config {
   int riemannSolver;  /* grim_riemann */
   double slopeLimTheta;
   double CourantFactor, maxDtIncrement;
+  /* 0: grim's residual mask (1 in the bulk): nonlinearsolve_atol bounds the
+   * absolute sum of squared residual rows.  1: the bulk mask is
+   * 1/max_v |rhs_v| per zone, so the tolerance bounds the residual relative
+   * to the zone's conserved variables (no grim counterpart: an absolute
+   * 1e-20 lies below the FP64 rounding floor of zones with large U). */
+  int residualScaling;
 } grim_grid_config;
 
 typedef struct grim_ctx grim_ctx;

@@ -202,6 +202,7 @@ typedef struct grim_oracle_grid {
   int reconstruction, riemannSolver;
   double slopeLimTheta;
   double CourantFactor, maxDtIncrement;
+  int residualScaling; /* 0: mask 1 in the bulk; 1: 1/max_v |rhs_v| */
 } grim_oracle_grid;
 
 /* Geometry of every zone of the grid.  center, faceX1 (LEFT), faceX2

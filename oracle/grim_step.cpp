@@ -1179,8 +1179,12 @@ void stage(const grim_oracle_grid& c, const grim_oracle_params& p, const Dims& D
       }
       eOld.computeImplicitSources(Si, tauDamp, p);
     }
-    for (int v = 0; v < nf; v++)
+    double rmax = 0.;
+    for (int v = 0; v < nf; v++) {
       rhs[v * n + z] = U[v] - dtPrime * ((divF[v * n + z] + Sexp[v * n + z]) + 0.5 * Si[v]);
+      rmax = std::max(rmax, std::fabs(rhs[v * n + z]));
+    }
+    if (c.residualScaling) mask[z] = 1. / rmax;
     // induction equation, timestep.cpp:80-89 / 226-237
     for (int b = 0; b < 3; b++) {
       const int v = vars.B1 + b;
