@@ -648,15 +648,26 @@ __global__ __launch_bounds__(kBlock) void maxReduceKernel(int64_t count, const d
 }
 
 // Bulk-zone Newton statistics: count above atol and the largest iteration
-// count (vector-memory atomics).
+// count. Each wavefront reduces in registers and issues one vector-memory
+// atomic per counter.
 __global__ __launch_bounds__(kBlock) void stageStatsKernel(int64_t n, const double* __restrict__ mask,
                                                            const double* __restrict__ l2,
                                                            const int* __restrict__ iters, double atol,
                                                            unsigned long long* __restrict__ out) {
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (z >= n || mask[z] == 0.) return;
-  if (l2[z] > atol) atomicAdd(&out[0], 1ull);
-  atomicMax(&out[1], (unsigned long long)iters[z]);
+  bool above = false;
+  int it = 0;
+  if (z < n && mask[z] != 0.) {
+    above = l2[z] > atol;
+    it = iters[z];
+  }
+  const unsigned long long ballot = __ballot(above);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) it = max(it, __shfl_xor(it, off, 64));
+  if ((threadIdx.x & 63) == 0) {
+    if (ballot) atomicAdd(&out[0], (unsigned long long)__popcll(ballot));
+    if (it > 0) atomicMax(&out[1], (unsigned long long)it);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1053,6 +1064,7 @@ struct grim_ctx {
   int* iters;
   double* l2;
   unsigned long long* counters;
+  void* newtonWs;   // compacting Newton work list (grim_newton_workspace_bytes)
   double* hostBuf;  // pinned: dt / stats readback
   // torus problem hooks (grim_init_torus)
   int torus;
@@ -1401,6 +1413,7 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
       {(void**)&x->iters, sizeof(int) * (size_t)n},
       {(void**)&x->l2, sizeof(double) * (size_t)n},
       {(void**)&x->counters, 2 * sizeof(unsigned long long)},
+      {(void**)&x->newtonWs, (size_t)grim_newton_workspace_bytes(n)},
   };
   for (auto& r : reqs) {
     hipError_t e = hipMalloc(r.ptr, r.bytes);
@@ -1410,6 +1423,8 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
       return GRIM_ERR_OOM;
     }
   }
+  x->opts.workspace = x->newtonWs;
+  x->opts.workspace_bytes = grim_newton_workspace_bytes(n);
   if (hipHostMalloc((void**)&x->hostBuf, 64) != hipSuccess) {
     setError("grim_create: hipHostMalloc failed");
     grim_destroy(x);
@@ -1452,7 +1467,8 @@ int grim_destroy(grim_ctx* x) {
   if (x->s) (void)hipStreamSynchronize(x->s);
   void* ptrs[] = {x->primOld, x->primHalf, x->primNew, x->primDt, x->consOld, x->fluxes, x->emf,
                   x->sExp,    x->sImp,     x->uT,      x->rhs,    x->mask,    x->geomC,  x->geomX1,
-                  x->geomX2,  x->gamma,    x->red,     x->iters,  x->l2,      x->counters};
+                  x->geomX2,  x->gamma,    x->red,     x->iters,  x->l2,      x->counters,
+                  x->newtonWs};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (x->hostBuf) (void)hipHostFree(x->hostBuf);
