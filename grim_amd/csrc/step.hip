@@ -291,6 +291,94 @@ __global__ __launch_bounds__(kBlock) void faceFluxKernel(StepArgs a, Box box,
   }
 }
 
+// The same face flux with the two states of a face on different wavefronts:
+// a 256-thread workgroup takes 128 faces; wavefronts 0-1 build the left
+// states (right face value of zone z - s), wavefronts 2-3 the right states
+// (left face value of zone z).  Each thread reconstructs only its own side
+// and evaluates one element, so the branch is uniform per wavefront and a
+// thread holds half the state of the one-thread-per-face form.  Fluxes,
+// conserved variables and speeds meet in LDS; then each thread writes half
+// of its face's variables.  Arithmetic per face is that of faceFluxKernel.
+constexpr int kFacesPerBlock = kBlock / 2;
+
+template <class L, int DIR, int RECON, int SIDE>
+__device__ __forceinline__ void faceState(const StepArgs& a, int64_t z, const double* __restrict__ prim,
+                                          const double* __restrict__ geomFace, double (*sh)[kFacesPerBlock],
+                                          int lf) {
+  const int64_t n = a.D.n, s = a.D.s[DIR];
+  double P[L::DOF];
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) {
+    double y[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) y[q] = prim[v * n + z + (q - 3 + SIDE) * s];
+    double l, r;
+    reconstructZone<RECON>(y, a.theta, l, r);
+    P[v] = SIDE ? l : r;
+  }
+  GeomFull G;
+  loadGeomFull(geomFace, n, z, G);
+  Kinematics kk;
+  Closure cc;
+  kinematics<L>(P, G.zg, a.prm, kk);
+  elementClosure<L>(P, kk, a.prm, G.r, cc);
+  double F[L::DOF];
+  fluxes<L, DIR + 1>(P, kk, cc, G.zg.g, F);
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) sh[v][lf] = F[v];
+  fluxes<L, 0>(P, kk, cc, G.zg.g, F);
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) sh[L::DOF + v][lf] = F[v];
+  double mn, mx;
+  charSpeeds<L, DIR>(kk, cc, G, a.prm, mn, mx);
+  sh[2 * L::DOF][lf] = mn;
+  sh[2 * L::DOF + 1][lf] = mx;
+}
+
+template <class L, int DIR, int RECON>
+__global__ __launch_bounds__(kBlock) void faceFluxSplitKernel(StepArgs a, Box box,
+                                                              const double* __restrict__ prim,
+                                                              const double* __restrict__ geomFace,
+                                                              double* __restrict__ flux) {
+  __shared__ double sh[2][2 * L::DOF + 2][kFacesPerBlock];
+  const int side = threadIdx.x / kFacesPerBlock;
+  const int lf = threadIdx.x % kFacesPerBlock;
+  const int64_t f = (int64_t)blockIdx.x * kFacesPerBlock + lf;
+  const bool active = f < box.count();
+  const Dims& D = a.D;
+  const int64_t n = D.n;
+  int64_t z = 0;
+  if (active) {
+    int i, j, k;
+    box.at(f, i, j, k);
+    z = flatIndex(D, i, j, k);
+    if (side == 0)
+      faceState<L, DIR, RECON, 0>(a, z, prim, geomFace, sh[0], lf);
+    else
+      faceState<L, DIR, RECON, 1>(a, z, prim, geomFace, sh[1], lf);
+  }
+  __syncthreads();
+  if (!active) return;
+  const double minSpeed = stdmin(sh[0][2 * L::DOF][lf], sh[1][2 * L::DOF][lf]);
+  const double maxSpeed = stdmax(sh[0][2 * L::DOF + 1][lf], sh[1][2 * L::DOF + 1][lf]);
+  constexpr int H = (L::DOF + 1) / 2;
+  const int v0 = side * H, v1 = side ? L::DOF : H;
+  if (a.riemann == GRIM_RIEMANN_HLL) {
+    for (int v = v0; v < v1; v++) {
+      const double FL = sh[0][v][lf], FR = sh[1][v][lf];
+      const double UL = sh[0][L::DOF + v][lf], UR = sh[1][L::DOF + v][lf];
+      flux[v * n + z] = (maxSpeed * FL - minSpeed * FR + minSpeed * maxSpeed * (UR - UL)) / (maxSpeed - minSpeed);
+    }
+  } else {
+    const double c = stdmax(maxSpeed, -minSpeed);
+    for (int v = v0; v < v1; v++) {
+      const double FL = sh[0][v][lf], FR = sh[1][v][lf];
+      const double UL = sh[0][L::DOF + v][lf], UR = sh[1][L::DOF + v][lf];
+      flux[v * n + z] = 0.5 * (FL + FR - c * (UR - UL));
+    }
+  }
+}
+
 // timeStepper::computeEMF (constrainedtransport.cpp:49-83) on the edges the
 // bulk divergence reads.
 template <class L>
@@ -1155,6 +1243,21 @@ void launchFace(grim_ctx* x, const double* prim, double* flux) {
   const Box b = faceBox(x->D, DIR);
   const double* gf = DIR == 0 ? x->geomX1 : (DIR == 1 ? x->geomX2 : x->geomC);
   const unsigned nb = blocksFor(b.count());
+  static const bool split = getenv("GRIM_FACE_SPLIT") && getenv("GRIM_FACE_SPLIT")[0] == '1';
+  if (split) {
+    const unsigned nbs = (unsigned)((b.count() + kFacesPerBlock - 1) / kFacesPerBlock);
+    switch (x->c.reconstruction) {
+      case GRIM_RECON_WENO5:
+        faceFluxSplitKernel<L, DIR, GRIM_RECON_WENO5><<<nbs, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);
+        break;
+      case GRIM_RECON_PPM:
+        faceFluxSplitKernel<L, DIR, GRIM_RECON_PPM><<<nbs, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);
+        break;
+      default:
+        faceFluxSplitKernel<L, DIR, GRIM_RECON_MINMOD><<<nbs, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);
+    }
+    return;
+  }
   switch (x->c.reconstruction) {
     case GRIM_RECON_WENO5:
       faceFluxKernel<L, DIR, GRIM_RECON_WENO5><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);

@@ -21,6 +21,16 @@ except the boundary identities and the EMHD linear-mode error table):
   EMHD linear mode matches the reference notebook's error table
   (src/problem/linear_modes/Linear modes.ipynb, cell 1) to a factor of 5 and
   converges.
+* geometric sources in Kerr: the unmagnetised Fishbone-Moncrief torus is a
+  discrete equilibrium (flux divergence cancels the source to truncation
+  error, converging at second order or better).
+* torus set-up and hooks: density maximum 1 and beta_min = 100 after the
+  field normalisation, floor bounds and idempotence, equatorial symmetry,
+  the fixPoles copy/flip identities and the inflowCheck bound.
+* parity unpinned (no independent check beyond the restatement): the torus
+  closure limiter's exact form (problem.cpp:7-55), the inflowCheck velocity
+  rescaling, the flux filter near the poles, and the splitmix64 noise that
+  replaces PETSc's RAND48 stream.
 """
 import json
 import os
@@ -534,6 +544,41 @@ def test_torus_boundary_hook_identities():
     ur = out[2][:, :, il + 1:]
     lf = np.sqrt(1 + 0)  # rescaled by the new Lorentz factor, sign preserved
     assert np.all(ur * lf >= (gc[11][:, :, il] * gc[14][:, :, il])[..., None] - 1e-12)
+
+
+def test_hydrostatic_torus_is_a_discrete_equilibrium():
+    """The unmagnetised Fishbone-Moncrief torus is a stationary solution of
+    ideal GR hydrodynamics in Kerr: inside it the flux divergence (dominated,
+    in the r and theta momentum rows, by the pressure gradient) must cancel
+    against the geometric source -g T^k_l Gamma^l_k nu, leaving only the
+    truncation error.  Pins computeExplicitSources and the connection (a
+    dropped term, wrong sign or swapped index leaves an O(1) imbalance), and
+    the Riemann fluxes in curved spacetime: the imbalance shrinks at least
+    at second order under refinement."""
+    a = 0.9375
+    Rin = 0.85 * (1 + np.sqrt(1 - a * a))
+    out = {}
+    for N in (64, 128):
+        g = O.grid(N1=N, N2=N, N3=1, dim=2, metric=O.METRIC_MKS, blackHoleSpin=a, X1Start=np.log(Rin),
+                   X1End=np.log(3000.), X2Start=1e-8, X2End=1 - 1e-8, boundaryLeft=O.BC_OUTFLOW,
+                   boundaryRight=O.BC_OUTFLOW, boundaryBottom=O.BC_MIRROR, boundaryTop=O.BC_MIRROR,
+                   reconstruction=O.RECON_WENO5, riemannSolver=O.RIEMANN_LLF)
+        p = O.default_params(False)
+        p.adiabaticIndex = 5. / 3
+        gm = O.Geometry(g)
+        t = O.torus_params(MinPlasmaBeta=1e30, InitialPerturbationAmplitude=0.)
+        P = O.apply_boundaries(g, O.torus_ic(g, p, t, gm))
+        div = planes(g, O.div_fluxes(g, p, gm, P))[:5][(slice(None),) + g.bulk()]
+        src = planes(g, O.explicit_sources(g, p, gm, P))[:5][(slice(None),) + g.bulk()]
+        inside = planes(g, P)[0][g.bulk()] > 0.3
+        out[N] = [(np.abs(div[v] + src[v])[inside].mean(), np.abs(div[v])[inside].mean()) for v in range(5)]
+    for v in (2, 3):  # r and theta momentum: pressure gradient against gravity and rotation
+        imb64, div64 = out[64][v]
+        imb128, div128 = out[128][v]
+        assert imb64 < 1e-2 * div64 and imb128 < 1e-2 * div128, v
+        assert imb128 < imb64 / 3.5, v
+    for v in (0, 1, 4):  # no geometric source on these rows: the divergence itself vanishes
+        assert out[128][v][0] < out[64][v][0] / 10., v
 
 
 @pytest.mark.parametrize("emhd", [False, True])
