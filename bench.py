@@ -283,18 +283,36 @@ def torus_config(N1, N2, N3):
                        reconstruction=capi.RECON_WENO5, riemannSolver=capi.RIEMANN_LLF, residualScaling=1)
 
 
-def time_steps(ts, steps, warmup):
-    """Wall time of `steps` grim_time_step calls after `warmup` (each step syncs
-    once for dt and once per stage for its statistics)."""
+def time_steps(ts, steps, warmup, graph=False):
+    """Wall time per step of `steps` grim_time_step calls after `warmup`.  The
+    timed steps run without statistics (no host synchronisation: dt and time
+    advance on the device); with graph=True one step is captured into a HIP
+    graph and replayed.  One more step with statistics follows the timed
+    region and is returned."""
     import torch
 
     for _ in range(warmup):
         ts.timeStep()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    stats = [ts.timeStep() for _ in range(steps)]
-    torch.cuda.synchronize()
-    return (time.perf_counter() - t0) * 1e3 / steps, stats
+    if graph:
+        s = ts._stream
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            ts.timeStep(stats=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.replay()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        del g
+    else:
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ts.timeStep(stats=False)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+    return ms, ts.timeStep()
 
 
 def full_step_bench(args):
@@ -322,13 +340,13 @@ def full_step_bench(args):
         ts = TimeStepper(p, cfg, opts=opts)
         ts.initTorus(capi.torus_params())
         ts.set_time(0., 0.005)
-        ms, stats = time_steps(ts, args.torus_steps, 2)
+        ms, last = time_steps(ts, args.torus_steps, 2)
         torus[name] = {
             "ms_per_step": ms,
             "zone_updates_per_s": zones / (ms * 1e-3),
-            "unconverged_last_step": [int(stats[-1].unconverged[0]), int(stats[-1].unconverged[1])],
-            "max_iterations_last_step": [int(stats[-1].max_iterations[0]), int(stats[-1].max_iterations[1])],
-            "dt_last": stats[-1].dt,
+            "unconverged_next_step": [int(last.unconverged[0]), int(last.unconverged[1])],
+            "max_iterations_next_step": [int(last.max_iterations[0]), int(last.max_iterations[1])],
+            "dt_next_step": last.dt,
         }
         if name == "fd_fp64":
             fd_prim = ts.primOld
@@ -355,14 +373,18 @@ def full_step_bench(args):
 
     p = capi.default_params(False)
     p.maxNonLinearIter = 10
-    ts = TimeStepper(p, grid_config(N1=64, N2=64, reconstruction=capi.RECON_WENO5, riemannSolver=capi.RIEMANN_LLF))
-    ts.initLinearMode(0, 1e-4)
-    ts.set_time(0., 0.5 / 64)
-    ms, stats = time_steps(ts, 10, 2)
-    res["linear_wave_64x64"] = {"ms_per_step": ms, "zone_updates_per_s": 64 * 64 / (ms * 1e-3),
-                                "config": "2D Minkowski fast magnetosonic wave 64x64, WENO5+LLF, FD-FP64 Newton, "
-                                          "10 steps after 2 warm-up"}
-    ts.close()
+    wave = {"config": "2D Minkowski fast magnetosonic wave 64x64, WENO5+LLF, FD-FP64 Newton, 10 steps after 2 "
+                      "warm-up; eager = steps enqueued one by one, graph = one step captured in a HIP graph "
+                      "and replayed"}
+    for mode in ("eager", "graph"):
+        ts = TimeStepper(p, grid_config(N1=64, N2=64, reconstruction=capi.RECON_WENO5,
+                                        riemannSolver=capi.RIEMANN_LLF), stream=torch.cuda.Stream())
+        ts.initLinearMode(0, 1e-4)
+        ts.set_time(0., 0.5 / 64)
+        ms, _ = time_steps(ts, 10, 2, graph=(mode == "graph"))
+        wave[mode] = {"ms_per_step": ms, "zone_updates_per_s": 64 * 64 / (ms * 1e-3)}
+        ts.close()
+    res["linear_wave_64x64"] = wave
     return res
 
 

@@ -86,6 +86,7 @@ class ZoneBatch(C.Structure):
         ("radius", C.c_void_p),
         ("residual_mask", C.c_void_p),
         ("dt_prime", C.c_double),
+        ("dt_prime_device", C.c_void_p),
     ]
 
 

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonGlobalCheck(DevParams prm
     b.prim[RHO * n + z] = P[RHO];
     b.prim[UU * n + z] = P[UU];
     double R[L::NF];
-    residual<L>(P, G, c, prm, b.dt_prime, R);
+    residual<L>(P, G, c, prm, batchDt(b), R);
     const double l2 = sumSq<L::NF>(R);
 #pragma unroll
     for (int v = 0; v < L::NF; v++) Rbuf[v * n + z] = R[v];
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonGlobalUpdate(DevParams pr
   int evals = 0;
   ZoneScale zs{};
   EmhdWeights ew;
-  newtonUpdate<L, MODE_FD>(P, R, l2buf[z], G, c, prm, b.dt_prime, zs, ew, Rt, evals);
+  newtonUpdate<L, MODE_FD>(P, R, l2buf[z], G, c, prm, batchDt(b), zs, ew, Rt, evals);
 #pragma unroll
   for (int v = 0; v < L::NF; v++) b.prim[v * n + z] = P[v];
   evalsBuf[z] += evals;
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonGlobalFinish(DevParams pr
     double P[L::DOF], R[L::NF];
 #pragma unroll
     for (int v = 0; v < L::DOF; v++) P[v] = b.prim[v * n + z];
-    residual<L>(P, G, c, prm, b.dt_prime, R);
+    residual<L>(P, G, c, prm, batchDt(b), R);
     l2 = sumSq<L::NF>(R);
   }
   if (st.iterations) st.iterations[z] = updates;
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kDefaultBlock) void residualKernel(DevParams prm, g
   double P[L::DOF], R[L::NF];
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) P[v] = b.prim[v * n + z];
-  residual<L>(P, G, c, prm, b.dt_prime, R);
+  residual<L>(P, G, c, prm, batchDt(b), R);
 #pragma unroll
   for (int v = 0; v < L::NF; v++) Rout[v * n + z] = R[v];
 }
@@ -159,8 +159,8 @@ __global__ __launch_bounds__(kDefaultBlock) void fdJacobianKernel(DevParams prm,
   double P[L::DOF], R[L::NF], A[L::NF][L::NF];
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) P[v] = b.prim[v * n + z];
-  residual<L>(P, G, c, prm, b.dt_prime, R);
-  fdJacobian<L>(P, R, G, c, prm, b.dt_prime, A);
+  residual<L>(P, G, c, prm, batchDt(b), R);
+  fdJacobian<L>(P, R, G, c, prm, batchDt(b), A);
 #pragma unroll
   for (int i = 0; i < L::NF; i++)
 #pragma unroll
@@ -178,11 +178,11 @@ __global__ __launch_bounds__(kDefaultBlock) void analyticJacobianKernel(DevParam
   ZoneConst<L> c;
   loadZoneConst<L>(b, prm, G, z, c);
   EmhdWeights ew;
-  if constexpr (L::EMHD) emhdWeights<L>(G, c, prm, b.dt_prime, ew);
+  if constexpr (L::EMHD) emhdWeights<L>(G, c, prm, batchDt(b), ew);
   double P[L::DOF], A[L::NF][L::NF];
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) P[v] = b.prim[v * n + z];
-  analyticJacobian<double, L>(P, G, prm, c.mask / b.dt_prime, c, ew, b.dt_prime, A);
+  analyticJacobian<double, L>(P, G, prm, c.mask / batchDt(b), c, ew, batchDt(b), A);
 #pragma unroll
   for (int i = 0; i < L::NF; i++)
 #pragma unroll
@@ -216,7 +216,7 @@ int validBatch(const grim_params* p, const grim_zone_batch* b) {
   if ((p->conduction || p->viscosity) && !b->prim_old) return 0;
   if (p->closure == GRIM_CLOSURE_PER_ZONE && (p->conduction || p->viscosity) && !b->closure) return 0;
   if (p->closure == GRIM_CLOSURE_TORUS && (p->conduction || p->viscosity) && !b->radius) return 0;
-  if (!(b->dt_prime > 0.)) return 0;
+  if (!b->dt_prime_device && !(b->dt_prime > 0.)) return 0;
   return 1;
 }
 

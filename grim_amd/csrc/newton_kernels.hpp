@@ -26,6 +26,11 @@ namespace grim {
 constexpr int kDefaultBlock = 256;
 constexpr int kMaxGroup = 8;  // phase-1 masks per phase-2 wavefront, at most
 
+// dt' of the batch: from device memory when the caller computed it there.
+__device__ __forceinline__ double batchDt(const grim_zone_batch& b) {
+  return b.dt_prime_device ? *b.dt_prime_device : b.dt_prime;
+}
+
 // Jacobian / precision modes of the per-zone Newton kernel.
 //  MODE_FD:          grim's residual and FD Jacobian, FP64 LU (reference path)
 //  MODE_ANALYTIC64:  grim's residual, analytic Jacobian and LU in FP64
@@ -166,7 +171,7 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
                                           const grim_newton_stats& st, int64_t z, int budget,
                                           bool accumulate, bool lastPhase) {
   const int64_t n = b.n;
-  const double dt = b.dt_prime;
+  const double dt = batchDt(b);
   ZoneGeom G;
   loadGeom(b.geom, n, z, G);
   ZoneConst<L> c;

@@ -480,7 +480,8 @@ __global__ __launch_bounds__(kBlock) void foldKernel(StepArgs a, Box box, const 
                                                      const double* __restrict__ geomC,
                                                      const double* __restrict__ consOld,
                                                      const double* __restrict__ sExp,
-                                                     const double* __restrict__ sImp, double dtPrime,
+                                                     const double* __restrict__ sImp,
+                                                     const double* __restrict__ dtPrimeDev,
                                                      double* __restrict__ divOut,
                                                      double* __restrict__ rhs,
                                                      double* __restrict__ primNew,
@@ -491,6 +492,7 @@ __global__ __launch_bounds__(kBlock) void foldKernel(StepArgs a, Box box, const 
   box.at(t, i, j, k);
   const Dims& D = a.D;
   const int64_t n = D.n, z = flatIndex(D, i, j, k);
+  const double dtPrime = rhs ? *dtPrimeDev : 0.;
   double rmax = 0.;
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) {
@@ -733,6 +735,40 @@ __global__ __launch_bounds__(kBlock) void maxReduceKernel(int64_t count, const d
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// The step's clock lives in device memory so that a whole step can be
+// enqueued (or captured in a HIP graph) without a host round trip:
+// clock[0] = time, clock[1] = dt, clock[2] = dt/2 (the half stage's dt').
+enum { CLOCK_TIME = 0, CLOCK_DT = 1, CLOCK_HALF_DT = 2, CLOCK_WORDS = 4 };
+
+// The end of timeStepper::computeDt (timestep.cpp:393-399) from the reduced
+// maximum of the inverse time step.
+__global__ void dtFinishKernel(const double* __restrict__ maxInvDt, double courant, double maxIncrement,
+                               double* __restrict__ clock) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double newDt = courant / maxInvDt[0];
+  if (newDt > maxIncrement * clock[CLOCK_DT]) newDt = maxIncrement * clock[CLOCK_DT];
+  clock[CLOCK_DT] = newDt;
+  clock[CLOCK_HALF_DT] = newDt / 2.;
+}
+
+__global__ void setClockKernel(double* __restrict__ clock, double time, double dt) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  clock[CLOCK_TIME] = time;
+  clock[CLOCK_DT] = dt;
+  clock[CLOCK_HALF_DT] = dt / 2.;
+}
+
+__global__ void advanceTimeKernel(double* __restrict__ clock) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  clock[CLOCK_TIME] += clock[CLOCK_DT];
+}
+
+// Clears counters with a kernel (a memset node in a captured graph is not
+// safe to replay).
+__global__ void zeroCountersKernel(unsigned long long* __restrict__ c, int count) {
+  if (blockIdx.x == 0 && (int)threadIdx.x < count) c[threadIdx.x] = 0ull;
 }
 
 // Bulk-zone Newton statistics: count above atol and the largest iteration
@@ -1144,7 +1180,8 @@ struct grim_ctx {
   CoordGrid cg;
   CoordMap cm;
   int dof, nf;
-  double time, dt;
+  bool clockSet;  // grim_set_time was called
+  double* clock;  // device: time, dt, dt/2 (CLOCK_*)
   double *primOld, *primHalf, *primNew, *primDt, *consOld;
   double *fluxes, *emf, *sExp, *sImp, *uT, *rhs, *mask;
   double *geomC, *geomX1, *geomX2, *gamma;
@@ -1307,18 +1344,35 @@ int explicitSources(grim_ctx* x, const double* prim, const double* primOld, doub
   return checkLaunch("sources");
 }
 
+// max over zones of the inverse time step, into x->red[nb] (device).
 template <class L>
-int computeDt(grim_ctx* x, const double* prim, double dt, double* out) {
+const double* reduceInvDt(grim_ctx* x, const double* prim) {
   const unsigned nb = blocksFor(x->D.n);
   dtKernel<L><<<nb, kBlock, 0, x->s>>>(x->a, prim, x->geomC, x->red);
   maxReduceKernel<<<1, kBlock, 0, x->s>>>(nb, x->red, x->red + nb);
-  GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf, x->red + nb, sizeof(double), hipMemcpyDeviceToHost, x->s));
+  return x->red + nb;
+}
+
+// timeStepper::computeDt with the previous dt given by the caller; returns
+// the new dt to the host (synchronises the stream).
+template <class L>
+int computeDt(grim_ctx* x, const double* prim, double dt, double* out) {
+  const double* maxInv = reduceInvDt<L>(x, prim);
+  GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf, maxInv, sizeof(double), hipMemcpyDeviceToHost, x->s));
   GRIM_HIP_TRY(hipStreamSynchronize(x->s));
   const double maxInvDt = x->hostBuf[0];
   double newDt = x->c.CourantFactor / maxInvDt;
   if (newDt > x->c.maxDtIncrement * dt) newDt = x->c.maxDtIncrement * dt;
   *out = newDt;
   return GRIM_OK;
+}
+
+// The same on the device clock: clock[dt] becomes the new dt, no host read.
+template <class L>
+int computeDtOnDevice(grim_ctx* x, const double* prim) {
+  const double* maxInv = reduceInvDt<L>(x, prim);
+  dtFinishKernel<<<1, 64, 0, x->s>>>(maxInv, x->c.CourantFactor, x->c.maxDtIncrement, x->clock);
+  return checkLaunch("computeDt");
 }
 
 template <class L>
@@ -1329,7 +1383,7 @@ int applyFloor(grim_ctx* x, double* prim) {
 
 int scrubNaNs(grim_ctx* x, double* prim, int nvars) {
   const Box b = bulkBox(x->D);
-  GRIM_HIP_TRY(hipMemsetAsync(x->counters, 0, sizeof(unsigned long long), x->s));
+  zeroCountersKernel<<<1, 64, 0, x->s>>>(x->counters, 1);
   nanCountKernel<<<blocksFor(b.count()), kBlock, 0, x->s>>>(x->D, b, nvars, prim, x->counters);
   nanZeroKernel<<<blocksFor(x->D.n), kBlock, 0, x->s>>>(x->D.n, nvars, x->counters, prim);
   return checkLaunch("NaN scrub");
@@ -1359,11 +1413,11 @@ int torusBCs(grim_ctx* x, double* prim) {
 
 // One stage of timeStep (timestep.cpp:14-107 half, 183-255 full).
 template <class L>
-int stage(grim_ctx* x, int st, double dt, grim_step_stats* stats) {
+int stage(grim_ctx* x, int st, grim_step_stats* stats) {
   const Dims& D = x->D;
   const size_t bytes = sizeof(double) * (size_t)L::DOF * D.n;
   double* primStage = st == 0 ? x->primOld : x->primHalf;
-  const double dtPrime = st == 0 ? dt / 2. : dt;
+  const double* dtPrime = x->clock + (st == 0 ? CLOCK_HALF_DT : CLOCK_DT);
   if (int rc = applyBoundaries(x, primStage, L::DOF, false)) return rc;
   if (x->torus)
     if (int rc = torusBCs<L>(x, primStage)) return rc;
@@ -1388,11 +1442,11 @@ int stage(grim_ctx* x, int st, double dt, grim_step_stats* stats) {
   zb.closure = nullptr;
   zb.radius = x->geomC + (size_t)LOC_R * D.n;
   zb.residual_mask = x->mask;
-  zb.dt_prime = dtPrime;
+  zb.dt_prime_device = dtPrime;
   grim_newton_stats ns = {x->iters, nullptr, x->l2};
   if (int rc = grim_batched_newton(&x->p, &zb, &x->opts, &ns, x->s)) return rc;
   if (stats) {
-    GRIM_HIP_TRY(hipMemsetAsync(x->counters, 0, 2 * sizeof(unsigned long long), x->s));
+    zeroCountersKernel<<<1, 64, 0, x->s>>>(x->counters, 2);
     stageStatsKernel<<<blocksFor(D.n), kBlock, 0, x->s>>>(D.n, x->mask, x->l2, x->iters,
                                                           x->p.nonlinearsolve_atol, x->counters);
     GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf, x->counters, 2 * sizeof(unsigned long long),
@@ -1485,8 +1539,7 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
   x->cm.X1Start = c->X1Start;
   x->cm.X1cyl = c->X1cyl;
   x->cm.X2cyl = c->X2cyl;
-  x->time = 0.;
-  x->dt = 0.;
+  x->clockSet = false;
   const int64_t n = x->D.n;
   const size_t dofB = sizeof(double) * (size_t)x->dof * n;
   const size_t nfB = sizeof(double) * (size_t)x->nf * n;
@@ -1517,6 +1570,7 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
       {(void**)&x->l2, sizeof(double) * (size_t)n},
       {(void**)&x->counters, 2 * sizeof(unsigned long long)},
       {(void**)&x->newtonWs, (size_t)grim_newton_workspace_bytes(n)},
+      {(void**)&x->clock, sizeof(double) * CLOCK_WORDS},
   };
   for (auto& r : reqs) {
     hipError_t e = hipMalloc(r.ptr, r.bytes);
@@ -1571,7 +1625,7 @@ int grim_destroy(grim_ctx* x) {
   void* ptrs[] = {x->primOld, x->primHalf, x->primNew, x->primDt, x->consOld, x->fluxes, x->emf,
                   x->sExp,    x->sImp,     x->uT,      x->rhs,    x->mask,    x->geomC,  x->geomX1,
                   x->geomX2,  x->gamma,    x->red,     x->iters,  x->l2,      x->counters,
-                  x->newtonWs};
+                  x->newtonWs, x->clock};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (x->hostBuf) (void)hipHostFree(x->hostBuf);
@@ -1622,8 +1676,9 @@ int grim_get_geometry(grim_ctx* x, int which, double* dst, int on_device) {
 
 int grim_set_time(grim_ctx* x, double time, double dt) {
   GRIM_REQUIRE(x && dt > 0., "grim_set_time: bad argument");
-  x->time = time;
-  x->dt = dt;
+  setClockKernel<<<1, 64, 0, x->s>>>(x->clock, time, dt);
+  if (int rc = checkLaunch("grim_set_time")) return rc;
+  x->clockSet = true;
   // primDt = primOld: computeDt of the first step reads the initial state.
   GRIM_HIP_TRY(hipMemcpyAsync(x->primDt, x->primOld, sizeof(double) * (size_t)x->dof * x->D.n,
                               hipMemcpyDeviceToDevice, x->s));
@@ -1631,26 +1686,31 @@ int grim_set_time(grim_ctx* x, double time, double dt) {
 }
 
 int grim_get_time(grim_ctx* x, double* time, double* dt) {
-  GRIM_REQUIRE(x && time && dt, "grim_get_time: null argument");
-  *time = x->time;
-  *dt = x->dt;
+  GRIM_REQUIRE(x && time && dt && x->clockSet, "grim_get_time: null argument or time not set");
+  GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf + 4, x->clock, 2 * sizeof(double), hipMemcpyDeviceToHost, x->s));
+  GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+  *time = x->hostBuf[4 + CLOCK_TIME];
+  *dt = x->hostBuf[4 + CLOCK_DT];
   return GRIM_OK;
 }
 
 int grim_time_step(grim_ctx* x, grim_step_stats* stats) {
-  GRIM_REQUIRE(x && x->dt > 0., "grim_time_step: no context or dt not set (grim_set_time)");
+  GRIM_REQUIRE(x && x->clockSet, "grim_time_step: no context or dt not set (grim_set_time)");
   return withLayout(x, [&]<class L>() -> int {
-    double dt;
-    if (int rc = computeDt<L>(x, x->primDt, x->dt, &dt)) return rc;
-    if (stats) stats->dt = dt;
-    if (int rc = stage<L>(x, 0, dt, stats)) return rc;
-    if (int rc = stage<L>(x, 1, dt, stats)) return rc;
+    if (int rc = computeDtOnDevice<L>(x, x->primDt)) return rc;
+    if (int rc = stage<L>(x, 0, stats)) return rc;
+    if (int rc = stage<L>(x, 1, stats)) return rc;
     // elemOld of the next computeDt: the problem's fullStepDiagnostics set
     // it from this step's final primOld.
     GRIM_HIP_TRY(hipMemcpyAsync(x->primDt, x->primOld, sizeof(double) * (size_t)L::DOF * x->D.n,
                                 hipMemcpyDeviceToDevice, x->s));
-    x->dt = dt;
-    x->time += dt;
+    advanceTimeKernel<<<1, 64, 0, x->s>>>(x->clock);
+    if (int rc = checkLaunch("grim_time_step")) return rc;
+    if (stats) {
+      GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf + 4, x->clock, 2 * sizeof(double), hipMemcpyDeviceToHost, x->s));
+      GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+      stats->dt = x->hostBuf[4 + CLOCK_DT];
+    }
     return GRIM_OK;
   });
 }
@@ -1706,7 +1766,7 @@ int grim_div_fluxes(grim_ctx* x, const double* prim, double* div_f, double* flux
     if (int rc = fluxesAndEmf<L>(x, prim)) return rc;
     const Box b = bulkBox(x->D);
     foldKernel<L><<<blocksFor(b.count()), kBlock, 0, x->s>>>(x->a, b, x->fluxes, x->emf, x->geomC, nullptr,
-                                                             nullptr, nullptr, 0., div_f, nullptr, nullptr,
+                                                             nullptr, nullptr, nullptr, div_f, nullptr, nullptr,
                                                              nullptr);
     if (fluxes) {
       GRIM_HIP_TRY(hipMemcpyAsync(fluxes, x->fluxes, sizeof(double) * (size_t)L::DOF * x->D.dim * x->D.n,

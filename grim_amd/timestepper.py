@@ -119,9 +119,11 @@ class TimeStepper:
         return prim
 
     # -- the step -----------------------------------------------------------
-    def timeStep(self) -> capi.StepStats:
-        st = capi.StepStats()
-        capi.check(capi.lib().grim_time_step(self._ctx, C.byref(st)), "grim_time_step")
+    def timeStep(self, stats: bool = True) -> capi.StepStats | None:
+        """One step.  stats=False enqueues it without a host synchronisation
+        (dt and time advance on the device), e.g. under HIP graph capture."""
+        st = capi.StepStats() if stats else None
+        capi.check(capi.lib().grim_time_step(self._ctx, C.byref(st) if stats else None), "grim_time_step")
         return st
 
     # -- stage-level pieces on torch device tensors ------------------------

@@ -83,6 +83,11 @@ typedef struct grim_zone_batch {
   const double* radius;       /* 1 plane (torus closure) or NULL */
   const double* residual_mask;/* 1 plane or NULL (= 1) */
   double dt_prime;            /* dt/2 (half step) or dt (full step) */
+  const double* dt_prime_device; /* optional: device address holding dt_prime,
+                                    read by the kernels instead of the value
+                                    (steps whose dt is computed on the device,
+                                    e.g. under HIP graph capture); NULL = use
+                                    dt_prime */
 } grim_zone_batch;
 
 typedef enum grim_jacobian_kind { GRIM_JAC_FD = 0, GRIM_JAC_ANALYTIC = 1 } grim_jacobian_kind;
@@ -274,13 +279,17 @@ int grim_set_prim(grim_ctx* ctx, const double* src, int on_device);
 int grim_get_prim(grim_ctx* ctx, double* dst, int on_device);
 int grim_get_geometry(grim_ctx* ctx, int which, double* dst, int on_device);
 /* time and dt of the timeStepper (dt is the previous step's, which bounds
- * the next one through maxDtIncrement). */
+ * the next one through maxDtIncrement).  Both live in device memory;
+ * grim_get_time synchronises the stream to read them. */
 int grim_set_time(grim_ctx* ctx, double time, double dt);
 int grim_get_time(grim_ctx* ctx, double* time, double* dt);
 
 /* timeStepper::timeStep (src/timestepper/timestep.cpp:3-334): computeDt,
  * then the half and full stages (boundaries, conserved variables, sources,
- * divergence of fluxes with flux-CT, induction, Newton solve). */
+ * divergence of fluxes with flux-CT, induction, Newton solve).  dt and time
+ * advance on the device.  With stats == NULL the call enqueues the whole
+ * step without a host synchronisation or allocation, so it may be captured
+ * into a HIP graph and replayed; with stats it synchronises to fill them. */
 int grim_time_step(grim_ctx* ctx, grim_step_stats* stats);
 
 /* The pieces of the step on caller-provided device arrays (dof planes each):

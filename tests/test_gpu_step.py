@@ -406,3 +406,70 @@ def test_torus_emhd_steps_match(torch):
     assert st.unconverged[0] + st.unconverged[1] < 1e-2 * g.N1 * g.N2 * g.N3
     # (measured on MI355X: ideal p99 5e-8, q/dP p99 8e-11)
     assert np.percentile(rel[ideal], 99) < 5e-7 and np.percentile(scaled, 99) < 1e-9
+
+
+@pytest.mark.parametrize("case", ["fast_mode_fd", "torus_ideal_mixed"])
+def test_graph_captured_time_step_matches_eager(torch, case):
+    """grim_time_step without stats enqueues a whole step with dt and time
+    on the device: captured once into a HIP graph and replayed, it must give
+    the eager steps' states bit for bit (replays interleaved with eager work
+    and synchronisations)."""
+    from grim_amd import capi
+    from grim_amd.timestepper import TimeStepper
+
+    from grim_amd.timestepper import grid_config
+
+    p = capi.default_params(False)
+    p.maxNonLinearIter = 10
+    if case == "fast_mode_fd":
+        cfg = grid_config(N1=64, N2=64, reconstruction=1, riemannSolver=1)
+        opts = capi.newton_opts()
+    else:
+        a = 0.9375
+        Rin = 0.85 * (1 + np.sqrt(1 - a * a))
+        cfg = grid_config(N1=96, N2=48, N3=4, dim=3, metric=1, blackHoleSpin=a, X1Start=np.log(Rin),
+                          X1End=np.log(3000.), X2Start=1e-8, X2End=1 - 1e-8, X3Start=0., X3End=2 * np.pi,
+                          DerefineThetaHorizon=1, DoCylindrify=1, X1cyl=np.log(4 * Rin), X2cyl=1. / 48,
+                          boundaryLeft=1, boundaryRight=1, boundaryBottom=2, boundaryTop=2, boundaryBack=0,
+                          boundaryFront=0, reconstruction=1, riemannSolver=1, residualScaling=1)
+        p.adiabaticIndex = 5. / 3
+        p.closure = capi.CLOSURE_TORUS
+        opts = capi.newton_opts(capi.JAC_ANALYTIC, capi.LU_FP32, compaction=1)
+    s = torch.cuda.Stream()
+    steppers = []
+    for stream in (None, s):
+        ts = TimeStepper(p, cfg, opts=opts, stream=stream)
+        if case == "fast_mode_fd":
+            ts.initLinearMode(0, 1e-4)
+            ts.set_time(0., 0.5 / 64)
+        else:
+            ts.initTorus(capi.torus_params())
+            ts.set_time(0., 0.005)
+        steppers.append(ts)
+    eager, graphed = steppers
+    torch.cuda.synchronize()
+    eager.timeStep(stats=False)
+    with torch.cuda.stream(s):
+        graphed.timeStep(stats=False)  # warm-up on the capture stream
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(graphed.primOld, eager.primOld)
+    gr = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(gr, stream=s):
+        graphed.timeStep(stats=False)
+    torch.cuda.synchronize()
+    for r in range(3):
+        st = eager.timeStep()
+        gr.replay()
+        torch.cuda.synchronize()
+        a, b = graphed.primOld, eager.primOld
+        assert np.array_equal(a, b, equal_nan=True), f"replay {r}"
+        assert graphed.time == eager.time and graphed.dt == eager.dt == st.dt, f"replay {r}"
+        # eager work on the capture stream between replays
+        with torch.cuda.stream(s):
+            x = torch.ones(1 << 20, device="cuda", dtype=torch.float64).cumsum(0)
+        torch.cuda.synchronize()
+        assert float(x[-1]) == float(1 << 20)
+    print(f"{case}: 3 graph replays bitwise equal to eager, dt {eager.dt:.6e}")
+    del gr
+    eager.close()
+    graphed.close()
