@@ -232,73 +232,17 @@ __global__ __launch_bounds__(kBlock) void reconstructKernel(Dims D, Box box, int
 // (riemannsolver.cpp:202-328) for one face.  The left state is the right
 // face value of zone z - s, the right state the left face value of zone z;
 // both sit at the face geometry.
-template <class L, int DIR, int RECON>
-__global__ __launch_bounds__(kBlock) void faceFluxKernel(StepArgs a, Box box,
-                                                         const double* __restrict__ prim,
-                                                         const double* __restrict__ geomFace,
-                                                         double* __restrict__ flux) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= box.count()) return;
-  int i, j, k;
-  box.at(t, i, j, k);
-  const Dims& D = a.D;
-  const int64_t n = D.n, z = flatIndex(D, i, j, k), s = D.s[DIR];
-  double PL[L::DOF], PR[L::DOF];
-#pragma unroll
-  for (int v = 0; v < L::DOF; v++) {
-    double y[6];
-#pragma unroll
-    for (int q = 0; q < 6; q++) y[q] = prim[v * n + z + (q - 3) * s];
-    double l, r;
-    reconstructZone<RECON>(y, a.theta, l, r);
-    PL[v] = r;
-    reconstructZone<RECON>(y + 1, a.theta, l, r);
-    PR[v] = l;
-  }
-  GeomFull G;
-  loadGeomFull(geomFace, n, z, G);
-  double FL[L::DOF], UL[L::DOF], minL, maxL;
-  {
-    Kinematics kk;
-    Closure cc;
-    kinematics<L>(PL, G.zg, a.prm, kk);
-    elementClosure<L>(PL, kk, a.prm, G.r, cc);
-    fluxes<L, DIR + 1>(PL, kk, cc, G.zg.g, FL);
-    fluxes<L, 0>(PL, kk, cc, G.zg.g, UL);
-    charSpeeds<L, DIR>(kk, cc, G, a.prm, minL, maxL);
-  }
-  double FR[L::DOF], UR[L::DOF], minR, maxR;
-  {
-    Kinematics kk;
-    Closure cc;
-    kinematics<L>(PR, G.zg, a.prm, kk);
-    elementClosure<L>(PR, kk, a.prm, G.r, cc);
-    fluxes<L, DIR + 1>(PR, kk, cc, G.zg.g, FR);
-    fluxes<L, 0>(PR, kk, cc, G.zg.g, UR);
-    charSpeeds<L, DIR>(kk, cc, G, a.prm, minR, maxR);
-  }
-  const double minSpeed = stdmin(minL, minR);
-  const double maxSpeed = stdmax(maxL, maxR);
-  if (a.riemann == GRIM_RIEMANN_HLL) {
-#pragma unroll
-    for (int v = 0; v < L::DOF; v++)
-      flux[v * n + z] = (maxSpeed * FL[v] - minSpeed * FR[v] + minSpeed * maxSpeed * (UR[v] - UL[v])) /
-                        (maxSpeed - minSpeed);
-  } else {
-    const double c = stdmax(maxSpeed, -minSpeed);
-#pragma unroll
-    for (int v = 0; v < L::DOF; v++) flux[v * n + z] = 0.5 * (FL[v] + FR[v] - c * (UR[v] - UL[v]));
-  }
-}
-
-// The same face flux with the two states of a face on different wavefronts:
-// a 256-thread workgroup takes 128 faces; wavefronts 0-1 build the left
+//
+// The two states of a face are built on different wavefronts: a 256-thread
+// workgroup takes 128 faces; wavefronts 0-1 build the left
 // states (right face value of zone z - s), wavefronts 2-3 the right states
 // (left face value of zone z).  Each thread reconstructs only its own side
 // and evaluates one element, so the branch is uniform per wavefront and a
-// thread holds half the state of the one-thread-per-face form.  Fluxes,
+// thread holds half the state a one-thread-per-face kernel would (which
+// needs 256 VGPRs plus spills and runs one wavefront per SIMD).  Fluxes,
 // conserved variables and speeds meet in LDS; then each thread writes half
-// of its face's variables.  Arithmetic per face is that of faceFluxKernel.
+// of its face's variables.  The arithmetic follows grim's expression order
+// exactly.
 constexpr int kFacesPerBlock = kBlock / 2;
 
 template <class L, int DIR, int RECON, int SIDE>
@@ -765,6 +709,22 @@ __global__ void advanceTimeKernel(double* __restrict__ clock) {
   clock[CLOCK_TIME] += clock[CLOCK_DT];
 }
 
+// dst1 = dst2 = src over `count` doubles: the end of the step writes the new
+// state into primOld and into the state computeDt reads next, reading it once.
+__global__ __launch_bounds__(kBlock) void copy2Kernel(int64_t count, const double* __restrict__ src,
+                                                      double* __restrict__ dst1, double* __restrict__ dst2) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t pairs = count / 2;
+  if (t < pairs) {
+    const double2 v = reinterpret_cast<const double2*>(src)[t];
+    reinterpret_cast<double2*>(dst1)[t] = v;
+    reinterpret_cast<double2*>(dst2)[t] = v;
+  } else if (t == pairs && (count & 1)) {
+    dst1[count - 1] = src[count - 1];
+    dst2[count - 1] = src[count - 1];
+  }
+}
+
 // Clears counters with a kernel (a memset node in a captured graph is not
 // safe to replay).
 __global__ void zeroCountersKernel(unsigned long long* __restrict__ c, int count) {
@@ -778,21 +738,38 @@ __global__ __launch_bounds__(kBlock) void stageStatsKernel(int64_t n, const doub
                                                            const double* __restrict__ l2,
                                                            const int* __restrict__ iters, double atol,
                                                            unsigned long long* __restrict__ out) {
-  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool above = false;
+  // grid-stride over the zones (a fixed, small grid: same-address atomics
+  // serialise, so each workgroup issues only one per counter)
+  unsigned long long count = 0;
   int it = 0;
-  if (z < n && mask[z] != 0.) {
-    above = l2[z] > atol;
-    it = iters[z];
+  for (int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; z < n; z += (int64_t)gridDim.x * blockDim.x) {
+    if (mask[z] != 0.) {
+      count += l2[z] > atol ? 1ull : 0ull;
+      it = max(it, iters[z]);
+    }
   }
-  const unsigned long long ballot = __ballot(above);
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) it = max(it, __shfl_xor(it, off, 64));
+  for (int off = 32; off > 0; off >>= 1) {
+    count += __shfl_xor(count, off, 64);
+    it = max(it, __shfl_xor(it, off, 64));
+  }
+  __shared__ unsigned long long sc[kBlock / 64];
+  __shared__ int si[kBlock / 64];
   if ((threadIdx.x & 63) == 0) {
-    if (ballot) atomicAdd(&out[0], (unsigned long long)__popcll(ballot));
+    sc[threadIdx.x / 64] = count;
+    si[threadIdx.x / 64] = it;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; w++) {
+      count += sc[w];
+      it = max(it, si[w]);
+    }
+    if (count) atomicAdd(&out[0], count);
     if (it > 0) atomicMax(&out[1], (unsigned long long)it);
   }
 }
+constexpr unsigned kStatsBlocks = 512;
 
 // ---------------------------------------------------------------------------
 // Linear-mode initial conditions (src/problem/linear_modes/problem.cpp:10-111)
@@ -1279,31 +1256,16 @@ template <class L, int DIR>
 void launchFace(grim_ctx* x, const double* prim, double* flux) {
   const Box b = faceBox(x->D, DIR);
   const double* gf = DIR == 0 ? x->geomX1 : (DIR == 1 ? x->geomX2 : x->geomC);
-  const unsigned nb = blocksFor(b.count());
-  static const bool split = getenv("GRIM_FACE_SPLIT") && getenv("GRIM_FACE_SPLIT")[0] == '1';
-  if (split) {
-    const unsigned nbs = (unsigned)((b.count() + kFacesPerBlock - 1) / kFacesPerBlock);
-    switch (x->c.reconstruction) {
-      case GRIM_RECON_WENO5:
-        faceFluxSplitKernel<L, DIR, GRIM_RECON_WENO5><<<nbs, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);
-        break;
-      case GRIM_RECON_PPM:
-        faceFluxSplitKernel<L, DIR, GRIM_RECON_PPM><<<nbs, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);
-        break;
-      default:
-        faceFluxSplitKernel<L, DIR, GRIM_RECON_MINMOD><<<nbs, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);
-    }
-    return;
-  }
+  const unsigned nb = (unsigned)((b.count() + kFacesPerBlock - 1) / kFacesPerBlock);
   switch (x->c.reconstruction) {
     case GRIM_RECON_WENO5:
-      faceFluxKernel<L, DIR, GRIM_RECON_WENO5><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);
+      faceFluxSplitKernel<L, DIR, GRIM_RECON_WENO5><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);
       break;
     case GRIM_RECON_PPM:
-      faceFluxKernel<L, DIR, GRIM_RECON_PPM><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);
+      faceFluxSplitKernel<L, DIR, GRIM_RECON_PPM><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);
       break;
     default:
-      faceFluxKernel<L, DIR, GRIM_RECON_MINMOD><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);
+      faceFluxSplitKernel<L, DIR, GRIM_RECON_MINMOD><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, gf, flux);
   }
 }
 
@@ -1416,7 +1378,12 @@ template <class L>
 int stage(grim_ctx* x, int st, grim_step_stats* stats) {
   const Dims& D = x->D;
   const size_t bytes = sizeof(double) * (size_t)L::DOF * D.n;
+  // Fixed buffer roles (no pointer swaps), so that a captured step reads and
+  // writes the same addresses on every replay: the half stage advances
+  // primOld into primHalf, the full stage primHalf into primNew, which
+  // grim_time_step then copies back into primOld.
   double* primStage = st == 0 ? x->primOld : x->primHalf;
+  double* out = st == 0 ? x->primHalf : x->primNew;
   const double* dtPrime = x->clock + (st == 0 ? CLOCK_HALF_DT : CLOCK_DT);
   if (int rc = applyBoundaries(x, primStage, L::DOF, false)) return rc;
   if (x->torus)
@@ -1425,16 +1392,16 @@ int stage(grim_ctx* x, int st, grim_step_stats* stats) {
                                   st == 0 ? x->consOld : nullptr))
     return rc;
   if (int rc = fluxesAndEmf<L>(x, primStage)) return rc;
-  GRIM_HIP_TRY(hipMemcpyAsync(x->primNew, primStage, bytes, hipMemcpyDeviceToDevice, x->s));
+  GRIM_HIP_TRY(hipMemcpyAsync(out, primStage, bytes, hipMemcpyDeviceToDevice, x->s));
   const Box b = bulkBox(D);
   foldKernel<L><<<blocksFor(b.count()), kBlock, 0, x->s>>>(x->a, b, x->fluxes, x->emf, x->geomC, x->consOld,
                                                            x->sExp, x->sImp, dtPrime, nullptr, x->rhs,
-                                                           x->primNew, x->c.residualScaling ? x->mask : nullptr);
+                                                           out, x->c.residualScaling ? x->mask : nullptr);
   if (int rc = checkLaunch("fold")) return rc;
 
   grim_zone_batch zb = {};
   zb.n = D.n;
-  zb.prim = x->primNew;
+  zb.prim = out;
   zb.geom = x->geomC;
   zb.rhs = x->rhs;
   zb.prim_old = x->primOld;
@@ -1447,7 +1414,7 @@ int stage(grim_ctx* x, int st, grim_step_stats* stats) {
   if (int rc = grim_batched_newton(&x->p, &zb, &x->opts, &ns, x->s)) return rc;
   if (stats) {
     zeroCountersKernel<<<1, 64, 0, x->s>>>(x->counters, 2);
-    stageStatsKernel<<<blocksFor(D.n), kBlock, 0, x->s>>>(D.n, x->mask, x->l2, x->iters,
+    stageStatsKernel<<<kStatsBlocks, kBlock, 0, x->s>>>(D.n, x->mask, x->l2, x->iters,
                                                           x->p.nonlinearsolve_atol, x->counters);
     GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf, x->counters, 2 * sizeof(unsigned long long),
                                 hipMemcpyDeviceToHost, x->s));
@@ -1457,17 +1424,13 @@ int stage(grim_ctx* x, int st, grim_step_stats* stats) {
     stats->unconverged[st] = (int64_t)c[0];
     stats->max_iterations[st] = (int32_t)c[1];
   }
-  if (int rc = communicate(x, x->primNew, L::DOF)) return rc;
+  if (int rc = communicate(x, out, L::DOF)) return rc;
   // halfStepDiagnostics / fullStepDiagnostics of the torus (:714-741)
   if (x->torus) {
     if (st == 1)
-      if (int rc = scrubNaNs(x, x->primNew, L::DOF)) return rc;
-    if (int rc = applyFloor<L>(x, x->primNew)) return rc;
+      if (int rc = scrubNaNs(x, out, L::DOF)) return rc;
+    if (int rc = applyFloor<L>(x, out)) return rc;
   }
-  if (st == 0)
-    std::swap(x->primHalf, x->primNew);
-  else
-    std::swap(x->primOld, x->primNew);
   return GRIM_OK;
 }
 
@@ -1700,10 +1663,10 @@ int grim_time_step(grim_ctx* x, grim_step_stats* stats) {
     if (int rc = computeDtOnDevice<L>(x, x->primDt)) return rc;
     if (int rc = stage<L>(x, 0, stats)) return rc;
     if (int rc = stage<L>(x, 1, stats)) return rc;
-    // elemOld of the next computeDt: the problem's fullStepDiagnostics set
-    // it from this step's final primOld.
-    GRIM_HIP_TRY(hipMemcpyAsync(x->primDt, x->primOld, sizeof(double) * (size_t)L::DOF * x->D.n,
-                                hipMemcpyDeviceToDevice, x->s));
+    // primOld = the full stage's result, and elemOld of the next computeDt
+    // (the problem's fullStepDiagnostics set it from this step's primOld).
+    const int64_t count = (int64_t)L::DOF * x->D.n;
+    copy2Kernel<<<blocksFor(count / 2 + 1), kBlock, 0, x->s>>>(count, x->primNew, x->primOld, x->primDt);
     advanceTimeKernel<<<1, 64, 0, x->s>>>(x->clock);
     if (int rc = checkLaunch("grim_time_step")) return rc;
     if (stats) {
