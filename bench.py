@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--no-emhd-batch", action="store_true", help="skip the EMHD uneven-convergence batch")
     ap.add_argument("--torus", type=int, nargs=3, default=[192, 192, 128], metavar=("N1", "N2", "N3"))
     ap.add_argument("--torus-steps", type=int, default=20)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one elementwise negation of this many MiB first: reference bytes that calibrate "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
     ap.add_argument("--paths", default=",".join(["fd_fp64", "fd_fp64_fused", "analytic_fp64", "mixed",
                                                  "mixed_compacted"]),
                     help="comma-separated subset of the Newton paths to time (profiling)")
@@ -121,6 +124,13 @@ def main():
 
     assert torch.cuda.is_available(), "no GPU"
     dev = capi.device_info()
+    if args.copy_probe_mib:
+        src = torch.ones(args.copy_probe_mib << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.neg(src, out=dst)  # reads and writes copy_probe_mib MiB once each
+        torch.cuda.synchronize()
+        del src, dst
     n = args.zones
     p = capi.default_params(False)
     p.maxNonLinearIter = 30

@@ -189,10 +189,13 @@ __device__ __forceinline__ void closure(const double* P, const Kinematics& k,
     c.chi = alphaC * cs * cs * c.tau;
     c.nu = alphaV * cs * cs * c.tau;
   } else {
-    const double dyn = pow(radius, 1.5);
+    // pow(r, 1.5) and pow(c_s, 3) of problem.cpp:7-55 as r sqrt(r) and
+    // c_s c_s c_s: equal to the last bit, reproducible on host and device,
+    // and a fraction of the cost of two library pow calls per element.
+    const double dyn = radius * sqrt(radius);
     double tau = dyn;
     if constexpr (L::COND_ON) {
-      const double Qmax = prm.condClosureFactor * k.rho * pow(cs, 3.);
+      const double Qmax = prm.condClosureFactor * k.rho * (cs * cs * cs);
       double y = fabs(c.q) / Qmax;
       y = exp(-(y - 1.) / 0.01);
       const double fd = y / (y + 1.) + 1.e-5;

@@ -101,12 +101,15 @@ struct FluidElement {
       chi_emhd = zc.conductionAlpha * soundSpeed * soundSpeed * tau;
       nu_emhd = zc.viscosityAlpha * soundSpeed * soundSpeed * tau;
     } else {
-      // torus closure, src/problem/torus/problem.cpp:7-55
+      // torus closure, src/problem/torus/problem.cpp:7-55.  pow(r, 1.5) and
+      // pow(c_s, 3) are evaluated as r sqrt(r) and c_s c_s c_s (equal up to
+      // the last bit; sqrt and products round identically on host and device,
+      // a library pow does not).
       T Radius = geom->r;
-      T DynamicalTimescale = pow(Radius, 1.5);
+      T DynamicalTimescale = Radius * sqrt(Radius);
       tau = DynamicalTimescale;
       if (p.conduction) {
-        T Qmax = p.ConductionClosureFactor * rho * pow(soundSpeed, 3.);
+        T Qmax = p.ConductionClosureFactor * rho * (soundSpeed * soundSpeed * soundSpeed);
         double lambda = 0.01;
         T yCon = abs(q) / Qmax;
         yCon = exp(-(yCon - 1.) / lambda);

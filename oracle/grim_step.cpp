@@ -597,7 +597,7 @@ void explicitSources(const grim_oracle_grid& c, const grim_oracle_params& p, con
     int lo[3] = {0, 0, 0}, hi[3] = {D.NT[0], D.NT[1], D.NT[2]};
     forRange(lo, hi, [&](int i, int j, int k) {
       const int64_t z = D.idx(i, j, k);
-      double P[GRIM_ORACLE_MAX_VARS];
+      double P[GRIM_ORACLE_MAX_VARS] = {};
       loadPrim(prim, nv, n, z, P);
       ZoneGeom<double> G;
       loadLocation(gm.center, n, z, &G);
