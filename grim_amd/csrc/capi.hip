@@ -14,6 +14,7 @@ namespace {
 thread_local std::string g_lastError;
 std::mutex g_devMutex;
 int g_devChecked[64] = {0};  // 0 unknown, 1 ok, -1 not gfx950
+int g_devCUs[64] = {0};      // compute units of each checked device
 }  // namespace
 
 void setError(const std::string& msg) { g_lastError = msg; }
@@ -34,12 +35,21 @@ int checkDevice() {
     return GRIM_ERR_NO_DEVICE;
   }
   const bool ok = std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
-  if (dev < 64) g_devChecked[dev] = ok ? 1 : -1;
+  if (dev < 64) {
+    g_devCUs[dev] = prop.multiProcessorCount;
+    g_devChecked[dev] = ok ? 1 : -1;
+  }
   if (!ok) {
     setError(std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
     return GRIM_ERR_NO_DEVICE;
   }
   return GRIM_OK;
+}
+
+int deviceCUs() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || g_devCUs[dev] <= 0) return 256;
+  return g_devCUs[dev];
 }
 
 }  // namespace grim

@@ -326,9 +326,8 @@ int grim_batch_linear_solve(int64_t n, int nvar, const double* A, const double* 
 }
 
 int64_t grim_newton_workspace_bytes(int64_t n) {
-  // one 64-bit mask per 64 zones, rounded up to 256 bytes
-  const int64_t bytes = ((n + 63) / 64) * (int64_t)sizeof(unsigned long long);
-  return ((bytes + 255) / 256) * 256;
+  // phase-1 masks, their prefix sums, the work list and two counters
+  return n < 0 ? 0 : compactionBytes(n);
 }
 
 int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
@@ -366,11 +365,14 @@ int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
   cfg.block = o.block_size ? o.block_size : kDefaultBlock;
   cfg.compaction = o.compaction != 0;
   cfg.budget1 = o.phase1_iters ? o.phase1_iters : 2;
-  cfg.group = o.phase2_group ? o.phase2_group : 4;
-  cfg.masks = nullptr;
+  cfg.group = o.phase2_group ? o.phase2_group : 2;
+  // persistent phase 2: 8 wavefronts per compute unit
+  cfg.persistentWaves = 8 * deviceCUs();
+  cfg.ws = CompactionWorkspace{};
   void* owned = nullptr;
   const int64_t need = grim_newton_workspace_bytes(b->n);
   if (cfg.compaction) {
+    GRIM_REQUIRE(b->n <= (int64_t(1) << 30), "grim_batched_newton: compaction supports at most 2^30 zones");
     void* ws = o.workspace;
     if (ws) {
       GRIM_REQUIRE(o.workspace_bytes >= need, "grim_batched_newton: workspace too small");
@@ -378,7 +380,7 @@ int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
       GRIM_HIP_TRY(hipMallocAsync(&owned, (size_t)need, s));
       ws = owned;
     }
-    cfg.masks = (unsigned long long*)ws;
+    cfg.ws = carveCompaction(ws, b->n);
   }
   const DevParams prm = makeDevParams(*p);
   const grim_zone_batch bb = *b;

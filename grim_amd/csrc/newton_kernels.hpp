@@ -17,6 +17,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "analytic.hpp"
 #include "grim_hip.h"
 #include "zone_physics.hpp"
@@ -24,7 +26,7 @@
 namespace grim {
 
 constexpr int kDefaultBlock = 256;
-constexpr int kMaxGroup = 8;  // phase-1 masks per phase-2 wavefront, at most
+constexpr int kMaxGroup = 8;  // phase-2 grab, in chunks of 64 list entries, at most
 
 // dt' of the batch: from device memory when the caller computed it there.
 __device__ __forceinline__ double batchDt(const grim_zone_batch& b) {
@@ -265,54 +267,102 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonPhase1(DevParams prm, gri
   if ((threadIdx.x & 63) == 0 && z < b.n) masks[z >> 6] = ballot;
 }
 
-// Position of the k-th (0-based) set bit of m, by popcount bisection.
-__device__ __forceinline__ int selectBit(unsigned long long m, int k) {
-  int pos = 0;
-#pragma unroll
-  for (int step = 32; step >= 1; step >>= 1) {
-    const unsigned long long low = m & ((1ull << step) - 1ull);
-    const int c = __popcll(low);
-    if (k >= c) {
-      k -= c;
-      m >>= step;
-      pos += step;
-    } else {
-      m = low;
-    }
+// The work list between the phases.  Phase 1 leaves one 64-bit mask of
+// unconverged lanes per wavefront; an exclusive prefix sum of their
+// popcounts numbers the unconverged zones, which are then scattered, in
+// ascending zone order, into a dense list of zone indices.
+constexpr int kScanBlock = 256;  // masks per scan workgroup
+
+// Prefix sums of the popcounts within each workgroup's kScanBlock masks, and
+// each workgroup's total.
+static __global__ __launch_bounds__(kScanBlock) void scanMasksKernel(int64_t nMasks,
+                                                              const unsigned long long* __restrict__ masks,
+                                                              int* __restrict__ maskOff,
+                                                              int* __restrict__ blockSum) {
+  __shared__ int sh[kScanBlock];
+  const int64_t m = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+  const int c = m < nMasks ? __popcll(masks[m]) : 0;
+  sh[threadIdx.x] = c;
+  __syncthreads();
+  // Hillis-Steele inclusive scan in LDS
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    const int v = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += v;
+    __syncthreads();
   }
-  return pos;
+  if (m < nMasks) maskOff[m] = sh[threadIdx.x] - c;
+  if (threadIdx.x == kScanBlock - 1) blockSum[blockIdx.x] = sh[threadIdx.x];
 }
 
-// Compacting solve, phase 2: wavefront q takes the `group` phase-1 masks
-// [q*group, (q+1)*group), numbers their set bits in order, and its lanes
-// continue those zones for the remaining maxNonLinearIter - budget1 updates
-// (lane j takes bits j, j+64, ...).  The zones of one wavefront lie within
-// 64*group consecutive zones, so plane reads stay local.
+// Exclusive prefix over the workgroup totals (one workgroup, in chunks of
+// kScanBlock), the list length into ctrl[0], and the grab counter ctrl[1]
+// cleared for the persistent phase.
+static __global__ __launch_bounds__(kScanBlock) void scanBlocksKernel(int64_t nBlocks, int* __restrict__ blockSum,
+                                                               int* __restrict__ ctrl) {
+  __shared__ int sh[kScanBlock];
+  int carry = 0;
+  for (int64_t base = 0; base < nBlocks; base += kScanBlock) {
+    const int64_t q = base + threadIdx.x;
+    const int c = q < nBlocks ? blockSum[q] : 0;
+    sh[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+      const int v = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += v;
+      __syncthreads();
+    }
+    if (q < nBlocks) blockSum[q] = carry + sh[threadIdx.x] - c;
+    const int chunk = sh[kScanBlock - 1];
+    __syncthreads();
+    carry += chunk;
+  }
+  if (threadIdx.x == 0) {
+    ctrl[0] = carry;
+    ctrl[1] = 0;
+  }
+}
+
+// Each mask's unconverged zones into the list at their global rank.
+static __global__ __launch_bounds__(kScanBlock) void scatterListKernel(int64_t nMasks,
+                                                                const unsigned long long* __restrict__ masks,
+                                                                const int* __restrict__ maskOff,
+                                                                const int* __restrict__ blockOff,
+                                                                int* __restrict__ list) {
+  const int64_t m = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+  if (m >= nMasks) return;
+  unsigned long long bits = masks[m];
+  int at = blockOff[m / kScanBlock] + maskOff[m];
+  while (bits) {
+    const int bit = __ffsll((long long)bits) - 1;
+    list[at++] = (int)(m * 64 + bit);
+    bits &= bits - 1ull;
+  }
+}
+
+// Compacting solve, phase 2: a persistent grid drains the list.  Each
+// wavefront takes `grab` chunks of 64 list entries at a time (one vector
+// atomic on the grab counter, by lane 0) and continues those zones for the
+// remaining maxNonLinearIter - budget1 updates; it exits once the counter
+// has passed the list length, which every wavefront reaches because the
+// counter only grows and the length is fixed.
 template <class L, int MODE>
 __global__ __launch_bounds__(kDefaultBlock) void newtonPhase2(DevParams prm, grim_zone_batch b,
                                                               grim_newton_stats st, int budget2,
-                                                              const unsigned long long* __restrict__ masks,
-                                                              int group) {
+                                                              const int* __restrict__ list,
+                                                              int* __restrict__ ctrl, int grab) {
   const int lane = threadIdx.x & 63;
-  const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nMasks = (b.n + 63) >> 6;
-  const int64_t first = q * group;
-  if (first >= nMasks) return;
-  int prefix[kMaxGroup + 1];
-  unsigned long long m[kMaxGroup];
-  prefix[0] = 0;
-#pragma unroll
-  for (int r = 0; r < kMaxGroup; r++) {
-    m[r] = (r < group && first + r < nMasks) ? masks[first + r] : 0ull;
-    prefix[r + 1] = prefix[r] + __popcll(m[r]);
-  }
-  const int total = prefix[kMaxGroup];
-  for (int j = lane; j < total; j += 64) {
-    int64_t zone = 0;
-#pragma unroll
-    for (int r = 0; r < kMaxGroup; r++)
-      if (j >= prefix[r] && j < prefix[r + 1]) zone = (first + r) * 64 + selectBit(m[r], j - prefix[r]);
-    solveZone<L, MODE>(prm, b, st, zone, budget2, true, true);
+  const int total = ctrl[0];
+  for (;;) {
+    int base = 0;
+    if (lane == 0) base = atomicAdd(&ctrl[1], 64 * grab);
+    base = __shfl(base, 0, 64);
+    if (base >= total) break;
+    for (int j = 0; j < grab; j++) {
+      const int idx = base + j * 64 + lane;
+      if (idx < total) solveZone<L, MODE>(prm, b, st, (int64_t)list[idx], budget2, true, true);
+    }
   }
 }
 
@@ -338,14 +388,48 @@ __device__ __forceinline__ void fdJacobianFast(const double* P, const double* R,
   }
 }
 
+// Carve-up of the caller's (or the library's) compaction workspace.
+struct CompactionWorkspace {
+  unsigned long long* masks;  // one per phase-1 wavefront
+  int* maskOff;               // popcount prefix within each scan workgroup
+  int* blockSum;              // scan workgroup totals, then their prefix
+  int* list;                  // unconverged zones, ascending
+  int* ctrl;                  // [0] list length, [1] grab counter
+};
+
+inline int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
+
+// Bytes of the workspace for n zones, and the carve-up of a buffer of that size.
+inline int64_t compactionBytes(int64_t n) {
+  const int64_t nMasks = (n + 63) / 64, nScan = (nMasks + kScanBlock - 1) / kScanBlock;
+  return align256(nMasks * 8) + align256(nMasks * 4) + align256(nScan * 4) + align256(n * 4) + 256;
+}
+
+inline CompactionWorkspace carveCompaction(void* base, int64_t n) {
+  const int64_t nMasks = (n + 63) / 64, nScan = (nMasks + kScanBlock - 1) / kScanBlock;
+  char* p = (char*)base;
+  CompactionWorkspace w;
+  w.masks = (unsigned long long*)p;
+  p += align256(nMasks * 8);
+  w.maskOff = (int*)p;
+  p += align256(nMasks * 4);
+  w.blockSum = (int*)p;
+  p += align256(nScan * 4);
+  w.list = (int*)p;
+  p += align256(n * 4);
+  w.ctrl = (int*)p;
+  return w;
+}
+
 // Launch configuration of the per-zone solve (chosen by the autotuner or the
 // caller).
 struct LaunchCfg {
   int block;                 // threads per workgroup: 64, 128 or 256
   bool compaction;           // two-phase compacting solve
   int budget1;               // phase-1 update budget
-  int group;                 // phase-1 masks per phase-2 wavefront (1..kMaxGroup)
-  unsigned long long* masks; // workspace: one 64-bit mask per phase-1 wavefront
+  int group;                 // phase-2 grab: chunks of 64 list entries per atomic (1..kMaxGroup)
+  int persistentWaves;       // phase-2 grid, in wavefronts
+  CompactionWorkspace ws;    // masks, prefix sums, work list, counters
 };
 
 // Enqueue the per-zone solve of mode MODE on stream s.
@@ -357,11 +441,16 @@ void launchSolve(const DevParams& prm, const grim_zone_batch& b, const grim_newt
     newtonPerZone<L, MODE><<<grid, cfg.block, 0, s>>>(prm, b, st);
     return;
   }
-  newtonPhase1<L, MODE><<<grid, cfg.block, 0, s>>>(prm, b, st, cfg.budget1, cfg.masks);
+  const CompactionWorkspace& w = cfg.ws;
+  newtonPhase1<L, MODE><<<grid, cfg.block, 0, s>>>(prm, b, st, cfg.budget1, w.masks);
   const int64_t nMasks = (b.n + 63) / 64;
-  const int64_t waves2 = (nMasks + cfg.group - 1) / cfg.group;
-  const unsigned grid2 = (unsigned)((waves2 * 64 + cfg.block - 1) / cfg.block);
-  newtonPhase2<L, MODE><<<grid2, cfg.block, 0, s>>>(prm, b, st, prm.maxIter - cfg.budget1, cfg.masks,
+  const int64_t nScan = (nMasks + kScanBlock - 1) / kScanBlock;
+  scanMasksKernel<<<(unsigned)nScan, kScanBlock, 0, s>>>(nMasks, w.masks, w.maskOff, w.blockSum);
+  scanBlocksKernel<<<1, kScanBlock, 0, s>>>(nScan, w.blockSum, w.ctrl);
+  scatterListKernel<<<(unsigned)nScan, kScanBlock, 0, s>>>(nMasks, w.masks, w.maskOff, w.blockSum, w.list);
+  const int64_t waves = std::min<int64_t>(cfg.persistentWaves, nMasks);
+  const unsigned grid2 = (unsigned)((waves * 64 + cfg.block - 1) / cfg.block);
+  newtonPhase2<L, MODE><<<grid2, cfg.block, 0, s>>>(prm, b, st, prm.maxIter - cfg.budget1, w.list, w.ctrl,
                                                     cfg.group);
 }
 

@@ -18,6 +18,9 @@ void setError(const std::string& msg);
 // device per process), GRIM_ERR_NO_DEVICE otherwise.
 int checkDevice();
 
+// Compute units of the current device (after checkDevice; 256 if unknown).
+int deviceCUs();
+
 #define GRIM_HIP_TRY(expr)                                                       \
   do {                                                                           \
     hipError_t e_ = (expr);                                                      \

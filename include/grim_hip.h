@@ -101,7 +101,10 @@ typedef struct grim_newton_opts {
   int jacobian;      /* grim_jacobian_kind */
   int lu_precision;  /* grim_lu_precision */
   int semantics;     /* grim_semantics */
-  int compaction;    /* 1: persistent kernel over a compacted work list */
+  int compaction;    /* 1: every zone gets phase1_iters updates, then the
+                        still unconverged zones are compacted into a dense
+                        work list that a persistent kernel drains (at most
+                        2^30 zones) */
   int block_size;    /* threads per workgroup (0 = library default) */
   int fused;         /* FD path only: 0 = grim's operation order, no fused
                         multiply-add (bit-reproducible against a CPU FP64
@@ -109,8 +112,8 @@ typedef struct grim_newton_opts {
                         forms.  The mixed path is always fused. */
   int phase1_iters;  /* compaction: Newton updates before the work list is
                         compacted (0 = library default) */
-  int phase2_group;  /* compaction: phase-1 wavefronts whose unconverged zones
-                        one phase-2 wavefront takes over, 1..8 (0 = default) */
+  int phase2_group;  /* compaction: chunks of 64 work-list zones a phase-2
+                        wavefront takes per grab, 1..8 (0 = default) */
   void* workspace;   /* compaction: device scratch of at least
                         grim_newton_workspace_bytes(n) bytes, or NULL to let
                         the library allocate it on the stream */
