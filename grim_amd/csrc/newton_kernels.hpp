@@ -243,9 +243,13 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
 
 // Per-zone semantics, one pass: each thread iterates its own zone until
 // sum R^2 <= atol or maxNonLinearIter updates.
+//
+// The EMHD solves need more than 256 registers per lane; asking for two
+// wavefronts per SIMD (instead of one with AGPR spills) measured 8-10 %
+// faster on the 3D torus, for the one-pass and the phase-1 kernel alike.
 template <class L, int MODE>
-__global__ __launch_bounds__(kDefaultBlock) void newtonPerZone(DevParams prm, grim_zone_batch b,
-                                                               grim_newton_stats st) {
+__global__ __launch_bounds__(kDefaultBlock) __attribute__((amdgpu_waves_per_eu(2))) void newtonPerZone(
+    DevParams prm, grim_zone_batch b, grim_newton_stats st) {
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= b.n) return;
   solveZone<L, MODE>(prm, b, st, z, prm.maxIter, false, true);
@@ -255,9 +259,8 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonPerZone(DevParams prm, gr
 // wavefront stores the 64-bit ballot of its lanes left unconverged (no
 // atomics: a single global counter serialises 65536 wavefronts).
 template <class L, int MODE>
-__global__ __launch_bounds__(kDefaultBlock) void newtonPhase1(DevParams prm, grim_zone_batch b,
-                                                              grim_newton_stats st, int budget1,
-                                                              unsigned long long* __restrict__ masks) {
+__global__ __launch_bounds__(kDefaultBlock) __attribute__((amdgpu_waves_per_eu(2))) void newtonPhase1(
+    DevParams prm, grim_zone_batch b, grim_newton_stats st, int budget1, unsigned long long* __restrict__ masks) {
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool more = false;
   if (z < b.n) more = solveZone<L, MODE>(prm, b, st, z, budget1, false, false);

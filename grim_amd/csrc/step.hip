@@ -239,16 +239,15 @@ __global__ __launch_bounds__(kBlock) void reconstructKernel(Dims D, Box box, int
 // (left face value of zone z).  Each thread reconstructs only its own side
 // and evaluates one element, so the branch is uniform per wavefront and a
 // thread holds half the state a one-thread-per-face kernel would (which
-// needs 256 VGPRs plus spills and runs one wavefront per SIMD).  Fluxes,
-// conserved variables and speeds meet in LDS; then each thread writes half
-// of its face's variables.  The arithmetic follows grim's expression order
-// exactly.
+// needs 256 VGPRs plus spills and runs one wavefront per SIMD).  The right
+// states' fluxes, conserved variables and speeds meet the left states in
+// LDS.  The arithmetic follows grim's expression order exactly.
 constexpr int kFacesPerBlock = kBlock / 2;
 
 template <class L, int DIR, int RECON, int SIDE>
 __device__ __forceinline__ void faceState(const StepArgs& a, int64_t z, const double* __restrict__ prim,
-                                          const double* __restrict__ geomFace, double (*sh)[kFacesPerBlock],
-                                          int lf) {
+                                          const double* __restrict__ geomFace, double (&F)[L::DOF],
+                                          double (&U)[L::DOF], double& mn, double& mx) {
   const int64_t n = a.D.n, s = a.D.s[DIR];
   double P[L::DOF];
 #pragma unroll
@@ -266,25 +265,19 @@ __device__ __forceinline__ void faceState(const StepArgs& a, int64_t z, const do
   Closure cc;
   kinematics<L>(P, G.zg, a.prm, kk);
   elementClosure<L>(P, kk, a.prm, G.r, cc);
-  double F[L::DOF];
   fluxes<L, DIR + 1>(P, kk, cc, G.zg.g, F);
-#pragma unroll
-  for (int v = 0; v < L::DOF; v++) sh[v][lf] = F[v];
-  fluxes<L, 0>(P, kk, cc, G.zg.g, F);
-#pragma unroll
-  for (int v = 0; v < L::DOF; v++) sh[L::DOF + v][lf] = F[v];
-  double mn, mx;
+  fluxes<L, 0>(P, kk, cc, G.zg.g, U);
   charSpeeds<L, DIR>(kk, cc, G, a.prm, mn, mx);
-  sh[2 * L::DOF][lf] = mn;
-  sh[2 * L::DOF + 1][lf] = mx;
 }
 
+// Only the right states pass through LDS (22.5 KB per workgroup), so LDS
+// does not cap the occupancy; the left-state wavefronts combine and write
+// every variable of their faces.
 template <class L, int DIR, int RECON>
-__global__ __launch_bounds__(kBlock) void faceFluxSplitKernel(StepArgs a, Box box,
-                                                              const double* __restrict__ prim,
-                                                              const double* __restrict__ geomFace,
-                                                              double* __restrict__ flux) {
-  __shared__ double sh[2][2 * L::DOF + 2][kFacesPerBlock];
+__global__ __launch_bounds__(kBlock) void faceFluxSplitKernel(
+    StepArgs a, Box box, const double* __restrict__ prim, const double* __restrict__ geomFace,
+    double* __restrict__ flux) {
+  __shared__ double sh[2 * L::DOF + 2][kFacesPerBlock];
   const int side = threadIdx.x / kFacesPerBlock;
   const int lf = threadIdx.x % kFacesPerBlock;
   const int64_t f = (int64_t)blockIdx.x * kFacesPerBlock + lf;
@@ -292,33 +285,40 @@ __global__ __launch_bounds__(kBlock) void faceFluxSplitKernel(StepArgs a, Box bo
   const Dims& D = a.D;
   const int64_t n = D.n;
   int64_t z = 0;
+  double F[L::DOF], U[L::DOF], mn = 0., mx = 0.;
   if (active) {
     int i, j, k;
     box.at(f, i, j, k);
     z = flatIndex(D, i, j, k);
-    if (side == 0)
-      faceState<L, DIR, RECON, 0>(a, z, prim, geomFace, sh[0], lf);
-    else
-      faceState<L, DIR, RECON, 1>(a, z, prim, geomFace, sh[1], lf);
+    if (side == 0) {
+      faceState<L, DIR, RECON, 0>(a, z, prim, geomFace, F, U, mn, mx);
+    } else {
+      faceState<L, DIR, RECON, 1>(a, z, prim, geomFace, F, U, mn, mx);
+#pragma unroll
+      for (int v = 0; v < L::DOF; v++) {
+        sh[v][lf] = F[v];
+        sh[L::DOF + v][lf] = U[v];
+      }
+      sh[2 * L::DOF][lf] = mn;
+      sh[2 * L::DOF + 1][lf] = mx;
+    }
   }
   __syncthreads();
-  if (!active) return;
-  const double minSpeed = stdmin(sh[0][2 * L::DOF][lf], sh[1][2 * L::DOF][lf]);
-  const double maxSpeed = stdmax(sh[0][2 * L::DOF + 1][lf], sh[1][2 * L::DOF + 1][lf]);
-  constexpr int H = (L::DOF + 1) / 2;
-  const int v0 = side * H, v1 = side ? L::DOF : H;
+  if (!active || side != 0) return;
+  const double minSpeed = stdmin(mn, sh[2 * L::DOF][lf]);
+  const double maxSpeed = stdmax(mx, sh[2 * L::DOF + 1][lf]);
   if (a.riemann == GRIM_RIEMANN_HLL) {
-    for (int v = v0; v < v1; v++) {
-      const double FL = sh[0][v][lf], FR = sh[1][v][lf];
-      const double UL = sh[0][L::DOF + v][lf], UR = sh[1][L::DOF + v][lf];
-      flux[v * n + z] = (maxSpeed * FL - minSpeed * FR + minSpeed * maxSpeed * (UR - UL)) / (maxSpeed - minSpeed);
+#pragma unroll
+    for (int v = 0; v < L::DOF; v++) {
+      const double FR = sh[v][lf], UR = sh[L::DOF + v][lf];
+      flux[v * n + z] = (maxSpeed * F[v] - minSpeed * FR + minSpeed * maxSpeed * (UR - U[v])) / (maxSpeed - minSpeed);
     }
   } else {
     const double c = stdmax(maxSpeed, -minSpeed);
-    for (int v = v0; v < v1; v++) {
-      const double FL = sh[0][v][lf], FR = sh[1][v][lf];
-      const double UL = sh[0][L::DOF + v][lf], UR = sh[1][L::DOF + v][lf];
-      flux[v * n + z] = 0.5 * (FL + FR - c * (UR - UL));
+#pragma unroll
+    for (int v = 0; v < L::DOF; v++) {
+      const double FR = sh[v][lf], UR = sh[L::DOF + v][lf];
+      flux[v * n + z] = 0.5 * (F[v] + FR - c * (UR - U[v]));
     }
   }
 }
