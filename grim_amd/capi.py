@@ -27,6 +27,7 @@ JAC_FD = 0
 JAC_ANALYTIC = 1
 LU_FP64 = 0
 LU_FP32 = 1
+LU_FP32_FP64 = 2
 NEWTON_PER_ZONE = 0
 NEWTON_GLOBAL = 1
 GEOM_PLANES = 16
@@ -157,7 +158,8 @@ class GridConfig(C.Structure):
 
 
 class StepStats(C.Structure):
-    _fields_ = [("dt", C.c_double), ("unconverged", C.c_int64 * 2), ("max_iterations", C.c_int32 * 2)]
+    _fields_ = [("dt", C.c_double), ("unconverged", C.c_int64 * 2), ("max_iterations", C.c_int32 * 2),
+                ("nonfinite", C.c_int64 * 2)]
 
 
 class TorusParams(C.Structure):

@@ -347,6 +347,8 @@ int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
   if (o.jacobian == GRIM_JAC_FD && o.lu_precision == GRIM_LU_FP64) mode = o.fused ? MODE_FD_FAST : MODE_FD;
   if (o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP64) mode = MODE_ANALYTIC64;
   if (o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP32) mode = MODE_MIXED;
+  const bool escalate = o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP32_FP64;
+  if (escalate) mode = MODE_MIXED;
   GRIM_REQUIRE(mode >= 0, "grim_batched_newton: unsupported Jacobian / LU precision pair");
   GRIM_REQUIRE(mode == MODE_FD || o.semantics == GRIM_NEWTON_PER_ZONE,
                "grim_batched_newton: lock-step semantics run grim's exact FD-FP64 algorithm only");
@@ -363,7 +365,8 @@ int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
 
   LaunchCfg cfg;
   cfg.block = o.block_size ? o.block_size : kDefaultBlock;
-  cfg.compaction = o.compaction != 0;
+  cfg.compaction = o.compaction != 0 || escalate;
+  cfg.escalate = escalate;
   cfg.budget1 = o.phase1_iters ? o.phase1_iters : 2;
   cfg.group = o.phase2_group ? o.phase2_group : 2;
   // persistent phase 2: 8 wavefronts per compute unit

@@ -18,6 +18,8 @@ int launchNewtonFast(int mode, const grim_params& p, const grim_zone_batch& b,
   return dispatchLayout(p, [&]<class L>() -> int {
     if (mode == MODE_FD_FAST)
       launchSolve<L, MODE_FD_FAST>(prm, b, st, cfg, s);
+    else if (cfg.escalate)
+      launchSolve<L, MODE_MIXED, MODE_ANALYTIC64F>(prm, b, st, cfg, s);
     else
       launchSolve<L, MODE_MIXED>(prm, b, st, cfg, s);
     return checkLaunch("grim_batched_newton (fused)");

@@ -41,11 +41,15 @@ __device__ __forceinline__ double batchDt(const grim_zone_batch& b) {
 //  MODE_FD_FAST:     grim's FD Jacobian rule on the reciprocal-form FP64
 //                    residual, one reciprocal per FD column, FP64 LU (the
 //                    fastest FD-FP64 variant, for a like-for-like comparison)
-enum { MODE_FD = 0, MODE_ANALYTIC64 = 1, MODE_MIXED = 2, MODE_FD_FAST = 3 };
+//  MODE_ANALYTIC64F: the mixed path's FP64 escalation: MODE_MIXED's residual
+//                    with the analytic Jacobian and LU in FP64 (phase 2 of
+//                    GRIM_LU_FP32_FP64, compiled in the fused unit)
+enum { MODE_FD = 0, MODE_ANALYTIC64 = 1, MODE_MIXED = 2, MODE_FD_FAST = 3, MODE_ANALYTIC64F = 4 };
 
 template <int MODE>
 struct ModeTraits {
-  static constexpr bool fastResidual = (MODE == MODE_MIXED || MODE == MODE_FD_FAST);
+  static constexpr bool fastResidual = (MODE == MODE_MIXED || MODE == MODE_FD_FAST || MODE == MODE_ANALYTIC64F);
+  static constexpr bool analytic = (MODE == MODE_ANALYTIC64 || MODE == MODE_MIXED || MODE == MODE_ANALYTIC64F);
 };
 
 // Per-zone quantities of the residual formed once per zone.
@@ -119,7 +123,7 @@ __device__ __forceinline__ bool newtonUpdate(double* P, const double* R, double 
 #pragma unroll
     for (int v = 0; v < L::NF; v++) d[v] = -R[v];
     luSolve<double, L::NF>(A, d);
-  } else if constexpr (MODE == MODE_ANALYTIC64) {
+  } else if constexpr (MODE == MODE_ANALYTIC64 || MODE == MODE_ANALYTIC64F) {
     double A[L::NF][L::NF];
     analyticJacobian<double, L>(P, G, prm, c.mask / dt, c, ew, dt, A);
 #pragma unroll
@@ -142,20 +146,40 @@ __device__ __forceinline__ bool newtonUpdate(double* P, const double* R, double 
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) Pt[v] = P[v];
   bool accepted = false;
+  bool finiteTrial = true;
   for (int ls = 0; ls < prm.maxLS; ls++) {
 #pragma unroll
     for (int v = 0; v < L::NF; v++) Pt[v] = P[v] + lambda * d[v];
     evalResidual<L, MODE>(Pt, G, c, prm, dt, zs, Rt);
     evals += 1;
     const double f1 = 0.5 * sumSq<L::NF>(Rt);
-    const bool condition = f1 > f0 * (1. - 1e-4 * lambda) + prm.lsFloor;
-    if (!condition) {
-      accepted = true;
-      break;
+    if constexpr (MODE == MODE_FD) {
+      // grim's test: a non-finite trial compares false and is accepted
+      const bool condition = f1 > f0 * (1. - 1e-4 * lambda) + prm.lsFloor;
+      if (!condition) {
+        accepted = true;
+        break;
+      }
+    } else {
+      // this build's solvers reject a non-finite trial and halve the step
+      // (an FP32 Jacobian of a near-singular zone can overshoot into
+      // states where the residual overflows)
+      finiteTrial = isfinite(f1);
+      if (f1 <= f0 * (1. - 1e-4 * lambda) + prm.lsFloor) {
+        accepted = true;
+        break;
+      }
+      if (!finiteTrial) {
+        lambda = 0.5 * lambda;
+        continue;
+      }
     }
     const double denom = f1 - f0 - fPrime0 * lambda;
     lambda = -fPrime0 * lambda * lambda / denom / 2.;
   }
+  // grim applies the last step length even when no trial passed; after a
+  // non-finite last trial this build's solvers keep the zone where it is
+  if (MODE != MODE_FD && !accepted && !finiteTrial) return false;
 #pragma unroll
   for (int v = 0; v < L::NF; v++) P[v] = P[v] + lambda * d[v];
   return accepted;
@@ -184,7 +208,7 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
     zs.scale = c.mask / dt;
   }
   EmhdWeights ew;
-  if constexpr (L::EMHD && (MODE == MODE_ANALYTIC64 || MODE == MODE_MIXED))
+  if constexpr (L::EMHD && ModeTraits<MODE>::analytic)
     emhdWeights<L>(G, c, prm, dt, ew);
   double P[L::DOF];
 #pragma unroll
@@ -432,11 +456,13 @@ struct LaunchCfg {
   int budget1;               // phase-1 update budget
   int group;                 // phase-2 grab: chunks of 64 list entries per atomic (1..kMaxGroup)
   int persistentWaves;       // phase-2 grid, in wavefronts
+  bool escalate;             // mixed path: phase 2 in FP64 (GRIM_LU_FP32_FP64)
   CompactionWorkspace ws;    // masks, prefix sums, work list, counters
 };
 
-// Enqueue the per-zone solve of mode MODE on stream s.
-template <class L, int MODE>
+// Enqueue the per-zone solve of mode MODE on stream s (phase 2 of the
+// compacting solve in MODE2).
+template <class L, int MODE, int MODE2 = MODE>
 void launchSolve(const DevParams& prm, const grim_zone_batch& b, const grim_newton_stats& st,
                  const LaunchCfg& cfg, hipStream_t s) {
   const unsigned grid = (unsigned)((b.n + cfg.block - 1) / cfg.block);
@@ -444,6 +470,7 @@ void launchSolve(const DevParams& prm, const grim_zone_batch& b, const grim_newt
     newtonPerZone<L, MODE><<<grid, cfg.block, 0, s>>>(prm, b, st);
     return;
   }
+  static_assert(MODE2 == MODE || (MODE == MODE_MIXED && MODE2 == MODE_ANALYTIC64F), "phase-2 mode");
   const CompactionWorkspace& w = cfg.ws;
   newtonPhase1<L, MODE><<<grid, cfg.block, 0, s>>>(prm, b, st, cfg.budget1, w.masks);
   const int64_t nMasks = (b.n + 63) / 64;
@@ -453,8 +480,8 @@ void launchSolve(const DevParams& prm, const grim_zone_batch& b, const grim_newt
   scatterListKernel<<<(unsigned)nScan, kScanBlock, 0, s>>>(nMasks, w.masks, w.maskOff, w.blockSum, w.list);
   const int64_t waves = std::min<int64_t>(cfg.persistentWaves, nMasks);
   const unsigned grid2 = (unsigned)((waves * 64 + cfg.block - 1) / cfg.block);
-  newtonPhase2<L, MODE><<<grid2, cfg.block, 0, s>>>(prm, b, st, prm.maxIter - cfg.budget1, w.list, w.ctrl,
-                                                    cfg.group);
+  newtonPhase2<L, MODE2><<<grid2, cfg.block, 0, s>>>(prm, b, st, prm.maxIter - cfg.budget1, w.list, w.ctrl,
+                                                     cfg.group);
 }
 
 // Host launcher of the fast translation unit (newton_fast.hip).

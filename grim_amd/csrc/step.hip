@@ -740,33 +740,38 @@ __global__ __launch_bounds__(kBlock) void stageStatsKernel(int64_t n, const doub
                                                            unsigned long long* __restrict__ out) {
   // grid-stride over the zones (a fixed, small grid: same-address atomics
   // serialise, so each workgroup issues only one per counter)
-  unsigned long long count = 0;
+  unsigned long long count = 0, bad = 0;
   int it = 0;
   for (int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; z < n; z += (int64_t)gridDim.x * blockDim.x) {
     if (mask[z] != 0.) {
       count += l2[z] > atol ? 1ull : 0ull;
+      bad += isfinite(l2[z]) ? 0ull : 1ull;
       it = max(it, iters[z]);
     }
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     count += __shfl_xor(count, off, 64);
+    bad += __shfl_xor(bad, off, 64);
     it = max(it, __shfl_xor(it, off, 64));
   }
-  __shared__ unsigned long long sc[kBlock / 64];
+  __shared__ unsigned long long sc[kBlock / 64], sb[kBlock / 64];
   __shared__ int si[kBlock / 64];
   if ((threadIdx.x & 63) == 0) {
     sc[threadIdx.x / 64] = count;
+    sb[threadIdx.x / 64] = bad;
     si[threadIdx.x / 64] = it;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < kBlock / 64; w++) {
       count += sc[w];
+      bad += sb[w];
       it = max(it, si[w]);
     }
     if (count) atomicAdd(&out[0], count);
     if (it > 0) atomicMax(&out[1], (unsigned long long)it);
+    if (bad) atomicAdd(&out[2], bad);
   }
 }
 constexpr unsigned kStatsBlocks = 512;
@@ -1413,16 +1418,17 @@ int stage(grim_ctx* x, int st, grim_step_stats* stats) {
   grim_newton_stats ns = {x->iters, nullptr, x->l2};
   if (int rc = grim_batched_newton(&x->p, &zb, &x->opts, &ns, x->s)) return rc;
   if (stats) {
-    zeroCountersKernel<<<1, 64, 0, x->s>>>(x->counters, 2);
+    zeroCountersKernel<<<1, 64, 0, x->s>>>(x->counters, 3);
     stageStatsKernel<<<kStatsBlocks, kBlock, 0, x->s>>>(D.n, x->mask, x->l2, x->iters,
                                                           x->p.nonlinearsolve_atol, x->counters);
-    GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf, x->counters, 2 * sizeof(unsigned long long),
+    GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf, x->counters, 3 * sizeof(unsigned long long),
                                 hipMemcpyDeviceToHost, x->s));
     GRIM_HIP_TRY(hipStreamSynchronize(x->s));
-    unsigned long long c[2];
+    unsigned long long c[3];
     std::memcpy(c, x->hostBuf, sizeof(c));
     stats->unconverged[st] = (int64_t)c[0];
     stats->max_iterations[st] = (int32_t)c[1];
+    stats->nonfinite[st] = (int64_t)c[2];
   }
   if (int rc = communicate(x, out, L::DOF)) return rc;
   // halfStepDiagnostics / fullStepDiagnostics of the torus (:714-741)
@@ -1531,7 +1537,7 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
       {(void**)&x->red, sizeof(double) * ((size_t)nb + 1)},
       {(void**)&x->iters, sizeof(int) * (size_t)n},
       {(void**)&x->l2, sizeof(double) * (size_t)n},
-      {(void**)&x->counters, 2 * sizeof(unsigned long long)},
+      {(void**)&x->counters, 4 * sizeof(unsigned long long)},
       {(void**)&x->newtonWs, (size_t)grim_newton_workspace_bytes(n)},
       {(void**)&x->clock, sizeof(double) * CLOCK_WORDS},
   };

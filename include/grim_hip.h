@@ -91,7 +91,13 @@ typedef struct grim_zone_batch {
 } grim_zone_batch;
 
 typedef enum grim_jacobian_kind { GRIM_JAC_FD = 0, GRIM_JAC_ANALYTIC = 1 } grim_jacobian_kind;
-typedef enum grim_lu_precision { GRIM_LU_FP64 = 0, GRIM_LU_FP32 = 1 } grim_lu_precision;
+typedef enum grim_lu_precision {
+  GRIM_LU_FP64 = 0,
+  GRIM_LU_FP32 = 1,
+  GRIM_LU_FP32_FP64 = 2 /* analytic Jacobian only: FP32 Jacobian and LU for the
+                           first phase1_iters updates of each zone, FP64 for
+                           the rest (always run as the compacting solve) */
+} grim_lu_precision;
 typedef enum grim_semantics {
   GRIM_NEWTON_PER_ZONE = 0, /* each zone stops at its own convergence */
   GRIM_NEWTON_GLOBAL = 1    /* grim's lock-step loop: all zones iterate while any is unconverged */
@@ -256,6 +262,8 @@ typedef struct grim_step_stats {
   double dt;                  /* dt taken by the step */
   int64_t unconverged[2];     /* bulk zones above nonlinearsolve_atol after the half / full stage */
   int32_t max_iterations[2];  /* largest per-zone Newton iteration count of each stage */
+  int64_t nonfinite[2];       /* bulk zones whose final residual is not finite (grim's
+                                 test counts them as converged) */
 } grim_step_stats;
 
 /* Number of zones (N1T*N2T*N3T) of a grid. */

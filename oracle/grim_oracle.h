@@ -288,6 +288,7 @@ typedef struct grim_oracle_step_stats {
   double dt;
   int64_t unconverged[2];   /* bulk zones above atol after each stage */
   int32_t maxIterations[2]; /* per stage */
+  int64_t nonfinite[2];     /* bulk zones whose final residual is not finite */
 } grim_oracle_step_stats;
 
 /* timeStepper::timeStep (src/timestepper/timestep.cpp:3-334): computeDt from

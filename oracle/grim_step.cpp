@@ -1208,15 +1208,17 @@ void stage(const grim_oracle_grid& c, const grim_oracle_params& p, const Dims& D
   std::vector<double> l2(n, 0.);
   grim_oracle_newton(&p, &b, semantics, iters.data(), l2.data(), nullptr, 0);
   if (stats) {
-    int64_t unconv = 0;
+    int64_t unconv = 0, nonfinite = 0;
     int32_t mx = 0;
     for (int64_t z = 0; z < n; z++) {
       if (mask[z] == 0.) continue;
       if (l2[z] > p.nonlinearsolve_atol) unconv++;
+      if (!std::isfinite(l2[z])) nonfinite++;
       mx = std::max(mx, iters[z]);
     }
     stats->unconverged[st] = unconv;
     stats->maxIterations[st] = mx;
+    stats->nonfinite[st] = nonfinite;
   }
   communicate(c, D, nv, primNew);
   // halfStepDiagnostics / fullStepDiagnostics (torus/problem.cpp:714-741)

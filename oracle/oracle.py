@@ -133,7 +133,8 @@ class _Geometry(C.Structure):
 
 
 class StepStats(C.Structure):
-    _fields_ = [("dt", C.c_double), ("unconverged", C.c_int64 * 2), ("maxIterations", C.c_int32 * 2)]
+    _fields_ = [("dt", C.c_double), ("unconverged", C.c_int64 * 2), ("maxIterations", C.c_int32 * 2),
+                ("nonfinite", C.c_int64 * 2)]
 
 
 def build() -> str:

@@ -250,20 +250,23 @@ def test_analytic_jacobian_matches_dual_numbers(capi, torch_cuda, oracle, emhd):
 
 
 @pytest.mark.parametrize("emhd", [False, True])
-@pytest.mark.parametrize("mode", ["analytic64", "mixed"])
+@pytest.mark.parametrize("mode", ["analytic64", "mixed", "mixed_escalating"])
 def test_analytic_newton_converges_to_fd_solution(capi, torch_cuda, oracle, mode, emhd):
+    """The analytic Jacobian in FP64, in FP32, and in FP32 for the first two
+    updates of each zone then FP64 (GRIM_LU_FP32_FP64) reach grim's FD-FP64
+    solution."""
     torch = torch_cuda
     n = 65536
     p, b, rhs, mask = make_problem(capi, torch, n, emhd)
     results = {}
-    for name, jac, lu in (("fd", capi.JAC_FD, capi.LU_FP64),
-                          (mode, capi.JAC_ANALYTIC, capi.LU_FP64 if mode == "analytic64" else capi.LU_FP32)):
+    lu_of = {"analytic64": capi.LU_FP64, "mixed": capi.LU_FP32, "mixed_escalating": capi.LU_FP32_FP64}
+    for name, jac, lu in (("fd", capi.JAC_FD, capi.LU_FP64), (mode, capi.JAC_ANALYTIC, lu_of[mode])):
         prim = b["guess"].clone()
         it = torch.zeros(n, dtype=torch.int32, device="cuda")
         l2 = torch.zeros(n, dtype=torch.float64, device="cuda")
         capi.batched_newton(p, capi.make_batch(prim, b["geom"], rhs, b["prim_old"], None, b["closure"],
                                                residual_mask=mask), jac, lu, iterations=it, final_l2=l2,
-                            fused=1 if lu == capi.LU_FP32 else 0)
+                            fused=1 if lu != capi.LU_FP64 else 0)
         torch.cuda.synchronize()
         results[name] = (host(prim), host(it), host(l2))
     fd, an = results["fd"], results[mode]
