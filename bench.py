@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--no-emhd-batch", action="store_true", help="skip the EMHD uneven-convergence batch")
     ap.add_argument("--torus", type=int, nargs=3, default=[192, 192, 128], metavar=("N1", "N2", "N3"))
     ap.add_argument("--torus-steps", type=int, default=20)
+    ap.add_argument("--torus-paths", default="fd_fp64_lockstep,fd_fp64,mixed",
+                    help="comma-separated subset of the torus Newton paths (profiling)")
+    ap.add_argument("--no-wave", action="store_true", help="skip the 64x64 linear-wave workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one elementwise negation of this many MiB first: reference bytes that calibrate "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -110,11 +113,29 @@ HEADLINE = "mixed"
 FP32_PEAK_TFLOPS = 157.3
 
 
+TRAFFIC_FILE = "profiles/newton_mixed_ks2e22.md"
+
+
+def measured_traffic():
+    """HBM bytes per launch of the headline kernel from the calibrated
+    FETCH_SIZE + WRITE_SIZE counter passes recorded in TRAFFIC_FILE (counters
+    cannot be read from inside a timed run); None when the summary is absent."""
+    import re
+
+    try:
+        text = open(os.path.join(ROOT, TRAFFIC_FILE)).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)\s*$", text, re.M)
+    return int(m.group(1)) if m else None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the isolated-zone batch shards by zone range with no exchange; see DESIGN.md)",
+              "(the isolated-zone batch shards by zone range with no exchange; the grid step by slabs with "
+              "one halo exchange per stage)",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np
@@ -248,7 +269,8 @@ def main():
             "peak": mixed_peak,
             "unit": "TFLOP/s",
             "frac": head["valu_frac"],
-            "traffic": None,
+            "traffic": measured_traffic(),
+            "traffic_source": TRAFFIC_FILE,
             "kernel": head["kernel"],
             "kernel_ms": head["kernel_ms"],
             "hbm_gbs_algorithmic": bytes_per_zone * n / kern_s / 1e9,
@@ -343,6 +365,8 @@ def full_step_bench(args):
     for name, opts in (("fd_fp64_lockstep", capi.newton_opts(semantics=capi.NEWTON_GLOBAL)),
                        ("fd_fp64", capi.newton_opts()),
                        ("mixed", capi.newton_opts(jacobian=capi.JAC_ANALYTIC, lu_precision=capi.LU_FP32_FP64))):
+        if name not in args.torus_paths.split(","):
+            continue
         p = capi.default_params(True)
         p.adiabaticIndex = 5. / 3
         p.closure = capi.CLOSURE_TORUS
@@ -367,11 +391,11 @@ def full_step_bench(args):
 
     g = N1 + 6, N2 + 6, N3 + 6
     bulk = lambda a: a.reshape(p.dof, g[2], g[1], g[0])[:, 3:-3, 3:-3, 3:-3].reshape(p.dof, -1)  # noqa: E731
-    ref = bulk(states["fd_fp64"])
+    ref = bulk(states["fd_fp64"]) if "fd_fp64" in states else None
     for name in states:
         mine = bulk(states[name])
         torus[name]["nonfinite_zones"] = int((~np.isfinite(mine).all(axis=0)).sum())
-        if name == "fd_fp64":
+        if ref is None or name == "fd_fp64":
             continue
         # rho and u (positive) relative to themselves; every variable also
         # relative to its largest value (velocities, field components, q and
@@ -385,10 +409,10 @@ def full_step_bench(args):
         torus[name]["vs_fd_fp64_rho_u_rel_median"] = float(np.median(rel))
         torus[name]["vs_fd_fp64_scaled_p99"] = float(np.percentile(scaled, 99))
         torus[name]["vs_fd_fp64_scaled_max"] = float(scaled.max())
-    torus["speedup_mixed_vs_fd_fp64_lockstep"] = (torus["mixed"]["zone_updates_per_s"]
-                                                  / torus["fd_fp64_lockstep"]["zone_updates_per_s"])
-    torus["speedup_mixed_vs_fd_fp64_per_zone"] = (torus["mixed"]["zone_updates_per_s"]
-                                                  / torus["fd_fp64"]["zone_updates_per_s"])
+    for base, key in (("fd_fp64_lockstep", "speedup_mixed_vs_fd_fp64_lockstep"),
+                      ("fd_fp64", "speedup_mixed_vs_fd_fp64_per_zone")):
+        if "mixed" in torus and base in torus:
+            torus[key] = torus["mixed"]["zone_updates_per_s"] / torus[base]["zone_updates_per_s"]
     torus["config"] = (f"3D MKS torus {N1}x{N2}x{N3} (a=0.9375, r_in=6, r_max=12, SANE beta_min=100), EMHD "
                        f"(torus closure), WENO5+LLF, {args.torus_steps} steps after 2 warm-up, Newton to "
                        "relative residual 1e-10 (maxNonLinearIter 10); fd_fp64_lockstep = grim's solve loop "
@@ -397,6 +421,8 @@ def full_step_bench(args):
                        "with FP32 assembly and LU for the first two updates of each zone and FP64 for the "
                        "compacted stragglers (FP64 residual and update throughout)")
     res["torus_emhd"] = torus
+    if args.no_wave:
+        return res
 
     p = capi.default_params(False)
     p.maxNonLinearIter = 10
