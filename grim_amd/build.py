@@ -7,6 +7,11 @@ Flags that matter:
   -ffp-contract=off   no fused multiply-add contraction, so FP64 results are
                       rounded operation by operation as written (every file
                       except *_fast.hip, which use -ffp-contract=fast);
+  -fno-hip-fp32-correctly-rounded-divide-sqrt
+                      *_fast.hip only: FP32 division and square root by the
+                      hardware reciprocal sequences (the FP32 Jacobian and LU
+                      of the mixed path need no correctly rounded quotients;
+                      FP64 arithmetic is unaffected);
   -Xclang -target-feature -Xclang -packed-fp32-ops
                       keeps the compiler from emitting packed FP32 math.
 """
@@ -57,6 +62,7 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
         if src.endswith("_fast.hip"):
             # fused translation units: a*b+c may become one FMA
             flags[flags.index("-ffp-contract=off")] = "-ffp-contract=fast"
+            flags.append("-fno-hip-fp32-correctly-rounded-divide-sqrt")
         cmd = [hipcc(), *flags, *inc, "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -351,6 +351,60 @@ __device__ __forceinline__ void residualFastIdeal(const double* P, const ZoneGeo
   }
 }
 
+// The EMHD residual of the fused paths: the same rows as residual<L>, with
+// the per-zone constants folded once per zone.  The time-derivative sources
+// are affine in the new element's u_mu and T with the weights of
+// emhdWeights (S_td = sum_mu w_mu (u_mu - u_mu,old) + w_T (T - T_old), the
+// divisions by dt, tau_c, |b|_c and b^2_c taken once), the implicit source is
+// g qTilde / tau_c, and the rows are scaled by 1/dt, the normalisation and
+// the mask by products.  Equal to residual<L> up to rounding.
+template <class L>
+__device__ __forceinline__ void residualFastEmhd(const double* P, const ZoneGeom& G, const ZoneConst<L>& c,
+                                                 const DevParams& prm, const EmhdWeights& ew, double invDt,
+                                                 double* R) {
+  static_assert(L::EMHD, "EMHD residual");
+  Kinematics k;
+  kinematics<L>(P, G, prm, k);
+  Closure cl;
+  closure<L>(P, k, prm, c.alphaC, c.alphaV, c.radius, cl);
+  const double w = k.rho + k.u + k.pressure + k.bSqr;
+  const double qOverB = L::COND_ON ? cl.q / cl.bNorm : 0.;
+  const double dPOverB2 = L::VISC_ON ? cl.deltaP / k.bSqr : 0.;
+  double T0[4];
+#pragma unroll
+  for (int nu = 0; nu < 4; nu++) {
+    double t = w * k.uCon[0] * k.uCov[nu] - k.bCon[0] * k.bCov[nu];
+    if (nu == 0) t += k.pressure + 0.5 * k.bSqr;
+    if constexpr (L::COND_ON) t += qOverB * (k.uCon[0] * k.bCov[nu] + k.bCon[0] * k.uCov[nu]);
+    if constexpr (L::VISC_ON)
+      t -= dPOverB2 * k.bCon[0] * k.bCov[nu] - (cl.deltaP / 3.) * ((nu == 0 ? 1. : 0.) + k.uCon[0] * k.uCov[nu]);
+    T0[nu] = t;
+  }
+  const double s = c.mask * invDt;
+  const double Urho = G.g * (k.rho * k.uCon[0]);
+  R[RHO] = (Urho - c.rhs[RHO]) * s;
+  R[UU] = (G.g * T0[0] + Urho - c.rhs[UU]) * s;
+#pragma unroll
+  for (int i = 1; i < 4; i++) R[UU + i] = (G.g * T0[i] - c.rhs[UU + i]) * s;
+  double du[4];
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) du[mu] = k.uCov[mu] - c.oldUCov[mu];
+  if constexpr (L::COND_ON) {
+    double std = ew.wQT * (k.temperature - c.oldT);
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) std += ew.wQ[mu] * du[mu];
+    const double U = G.g * (k.uCon[0] * P[L::Q]);
+    R[L::Q] = ((U - c.rhs[L::Q]) * invDt + 0.5 * ew.gOverTau * P[L::Q] + std) * ew.sQ;
+  }
+  if constexpr (L::VISC_ON) {
+    double std = 0.;
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) std += ew.wDP[mu] * du[mu];
+    const double U = G.g * (k.uCon[0] * P[L::DP]);
+    R[L::DP] = ((U - c.rhs[L::DP]) * invDt + 0.5 * ew.gOverTau * P[L::DP] + std) * ew.sDP;
+  }
+}
+
 // Row and column equilibration around the FP32 LU: solve (Dr A Dc) y = Dr b,
 // x = Dc y, with Dc = |P_k| (the natural scale of each primitive) and Dr the
 // inverse of each scaled row's largest entry.

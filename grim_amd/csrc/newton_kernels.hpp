@@ -56,14 +56,18 @@ struct ModeTraits {
 struct ZoneScale {
   double invAlpha;  // 1/alpha
   double scale;     // mask/dt
+  double invDt;     // 1/dt
 };
 
 template <class L, int MODE>
 __device__ __forceinline__ void evalResidual(const double* P, const ZoneGeom& G,
                                              const ZoneConst<L>& c, const DevParams& prm,
-                                             double dt, const ZoneScale& zs, double* R) {
+                                             double dt, const ZoneScale& zs, const EmhdWeights& ew,
+                                             double* R) {
   if constexpr (ModeTraits<MODE>::fastResidual && !L::EMHD)
     residualFastIdeal<L>(P, G, c, prm, zs.invAlpha, zs.scale, R);
+  else if constexpr (ModeTraits<MODE>::fastResidual)
+    residualFastEmhd<L>(P, G, c, prm, ew, zs.invDt, R);
   else
     residual<L>(P, G, c, prm, dt, R);
 }
@@ -95,7 +99,7 @@ __device__ __forceinline__ void fdJacobian(const double* P, const double* R, con
 template <class L, int MODE>
 __device__ __forceinline__ void fdJacobianFast(const double* P, const double* R, const ZoneGeom& G,
                                                const ZoneConst<L>& c, const DevParams& prm,
-                                               double dt, const ZoneScale& zs,
+                                               double dt, const ZoneScale& zs, const EmhdWeights& ew,
                                                double (&A)[L::NF][L::NF]);
 
 // One Newton update after the convergence test (solve.cpp:80-198): Jacobian,
@@ -118,7 +122,7 @@ __device__ __forceinline__ bool newtonUpdate(double* P, const double* R, double 
     luSolve<double, L::NF>(A, d);
   } else if constexpr (MODE == MODE_FD_FAST) {
     double A[L::NF][L::NF];
-    fdJacobianFast<L, MODE>(P, R, G, c, prm, dt, zs, A);
+    fdJacobianFast<L, MODE>(P, R, G, c, prm, dt, zs, ew, A);
     evals += L::NF;
 #pragma unroll
     for (int v = 0; v < L::NF; v++) d[v] = -R[v];
@@ -150,7 +154,7 @@ __device__ __forceinline__ bool newtonUpdate(double* P, const double* R, double 
   for (int ls = 0; ls < prm.maxLS; ls++) {
 #pragma unroll
     for (int v = 0; v < L::NF; v++) Pt[v] = P[v] + lambda * d[v];
-    evalResidual<L, MODE>(Pt, G, c, prm, dt, zs, Rt);
+    evalResidual<L, MODE>(Pt, G, c, prm, dt, zs, ew, Rt);
     evals += 1;
     const double f1 = 0.5 * sumSq<L::NF>(Rt);
     if constexpr (MODE == MODE_FD) {
@@ -206,9 +210,10 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
   if constexpr (ModeTraits<MODE>::fastResidual) {
     zs.invAlpha = 1. / G.alpha;
     zs.scale = c.mask / dt;
+    zs.invDt = 1. / dt;
   }
   EmhdWeights ew;
-  if constexpr (L::EMHD && ModeTraits<MODE>::analytic)
+  if constexpr (L::EMHD && (ModeTraits<MODE>::analytic || ModeTraits<MODE>::fastResidual))
     emhdWeights<L>(G, c, prm, dt, ew);
   double P[L::DOF];
 #pragma unroll
@@ -227,7 +232,7 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
     P[RHO] = p0;
     P[UU] = p1;
     if (!haveR) {
-      evalResidual<L, MODE>(P, G, c, prm, dt, zs, R);
+      evalResidual<L, MODE>(P, G, c, prm, dt, zs, ew, R);
       evals++;
     }
     l2 = sumSq<L::NF>(R);
@@ -245,7 +250,7 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
   }
   if (!converged) {
     if (!haveR) {
-      evalResidual<L, MODE>(P, G, c, prm, dt, zs, R);
+      evalResidual<L, MODE>(P, G, c, prm, dt, zs, ew, R);
       if (!lastPhase) evals++;
     }
     l2 = sumSq<L::NF>(R);
@@ -396,7 +401,7 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonPhase2(DevParams prm, gri
 template <class L, int MODE>
 __device__ __forceinline__ void fdJacobianFast(const double* P, const double* R, const ZoneGeom& G,
                                                const ZoneConst<L>& c, const DevParams& prm,
-                                               double dt, const ZoneScale& zs,
+                                               double dt, const ZoneScale& zs, const EmhdWeights& ew,
                                                double (&A)[L::NF][L::NF]) {
   double Pp[L::DOF];
 #pragma unroll
@@ -407,7 +412,7 @@ __device__ __forceinline__ void fdJacobianFast(const double* P, const double* R,
     const bool small = fabs(P[row]) < .5 * eps;
     Pp[row] = small ? P[row] + eps : P[row] + eps * P[row];
     double Rp[L::NF];
-    evalResidual<L, MODE>(Pp, G, c, prm, dt, zs, Rp);
+    evalResidual<L, MODE>(Pp, G, c, prm, dt, zs, ew, Rp);
     const double inv = 1. / (Pp[row] - P[row]);
 #pragma unroll
     for (int col = 0; col < L::NF; col++) A[col][row] = (Rp[col] - R[col]) * inv;
