@@ -347,6 +347,50 @@ def time_steps(ts, steps, warmup, graph=False):
     return ms, ts.timeStep()
 
 
+def torus_params_emhd():
+    from grim_amd import capi
+
+    p = capi.default_params(True)
+    p.adiabaticIndex = 5. / 3
+    p.closure = capi.CLOSURE_TORUS
+    p.maxNonLinearIter = 10
+    return p
+
+
+def torus_one_step_error(cfg, N1, N2, N3):
+    """Primitive error of one mixed-precision step against one FD-FP64 step
+    taken from the same initial state (per-zone semantics for both)."""
+    import numpy as np
+    import torch
+
+    from grim_amd import capi
+    from grim_amd.timestepper import TimeStepper
+
+    out = {}
+    for name, opts in (("fd_fp64", capi.newton_opts()),
+                       ("mixed", capi.newton_opts(jacobian=capi.JAC_ANALYTIC, lu_precision=capi.LU_FP32_FP64))):
+        p = torus_params_emhd()
+        ts = TimeStepper(p, cfg, opts=opts)
+        ts.initTorus(capi.torus_params())
+        ts.set_time(0., 0.005)
+        st = ts.timeStep()
+        g = N1 + 6, N2 + 6, N3 + 6
+        out[name] = (ts.primOld.reshape(p.dof, g[2], g[1], g[0])[:, 3:-3, 3:-3, 3:-3].reshape(p.dof, -1),
+                     [int(st.unconverged[0]), int(st.unconverged[1])])
+        ts.close()
+        del ts
+        torch.cuda.empty_cache()
+    ref, mine = out["fd_fp64"][0], out["mixed"][0]
+    diff = np.abs(mine - ref)
+    rel = diff[:2] / np.abs(ref[:2])
+    scaled = diff / np.abs(ref).max(axis=1, keepdims=True)
+    return {"rho_u_rel_p99": float(np.percentile(rel, 99)), "rho_u_rel_median": float(np.median(rel)),
+            "rho_u_rel_max": float(rel.max()), "scaled_p99": float(np.percentile(scaled, 99)),
+            "scaled_max": float(scaled.max()), "unconverged_fd_fp64": out["fd_fp64"][1],
+            "unconverged_mixed": out["mixed"][1],
+            "note": "rho, u relative to themselves; every variable relative to its largest value"}
+
+
 def full_step_bench(args):
     """The whole time step (timeStepper::timeStep) on two workloads:
     the 3D EMHD Kerr torus at 192x192x128 with both Newton paths, and the 2D
@@ -361,6 +405,8 @@ def full_step_bench(args):
     cfg = torus_config(N1, N2, N3)
     zones = N1 * N2 * N3
     torus = {}
+    if "mixed" in args.torus_paths.split(","):
+        torus["one_step_error_mixed_vs_fd_fp64"] = torus_one_step_error(cfg, N1, N2, N3)
     states = {}
     for name, opts in (("fd_fp64_lockstep", capi.newton_opts(semantics=capi.NEWTON_GLOBAL)),
                        ("fd_fp64", capi.newton_opts()),
