@@ -87,7 +87,26 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_driver(verbose)
     return LIB
+
+
+DRIVER = os.path.join(HERE, "grim_step")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The C++ driver grim_amd/grim_step (host code only: g++ against the C
+    ABI and the C++ interface grim_amd/host/timestepper.hpp)."""
+    tmp = DRIVER + ".tmp"
+    cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-Wall", "-Wextra",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host"),
+           os.path.join(HERE, "host", "grim_main.cpp"), "-L", HERE, "-lgrim_hip",
+           "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

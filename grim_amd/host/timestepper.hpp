@@ -1,0 +1,188 @@
+// C++ host interface of the gfx950 time step, with the names and argument
+// meaning of grim's timeStepper (src/timestepper/timestepper.hpp:19-132) and
+// the parameter names of its `namespace params` (src/params.hpp:96-210).
+//
+// grim calls timeStepper member functions linked statically; this header
+// gives a grim driver (src/grim.cpp:10-84) the same calls on top of the C ABI
+// of include/grim_hip.h.  The grid arrays live on the GPU inside the
+// library's context; the host sees them through getPrim/setPrim (grim's
+// gridPy setVars/getVars index order, src/grid/gridPy.pyx:185-230).
+//
+// Errors: grim's member functions return void and fail through NaNs or
+// MPI_Abort; here a failing C call throws std::runtime_error with the
+// library's message (exceptions never cross the C ABI itself).  Numerical
+// non-convergence is not an error: it is reported in lastStats().
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "grim_hip.h"
+
+namespace grim {
+
+// The members of grim's `namespace params` that the time step reads.
+struct Params {
+  // grid and geometry (src/problem/*/params.cpp)
+  int metric = GRIM_METRIC_MINKOWSKI;
+  double blackHoleSpin = 0.9375, hSlope = 0.3;
+  int DerefineThetaHorizon = 0, DoCylindrify = 0;
+  double X1cyl = 0., X2cyl = 0.;
+  // scheme
+  int reconstruction = GRIM_RECON_WENO5, riemannSolver = GRIM_RIEMANN_LLF;
+  double slopeLimTheta = 2., CourantFactor = 0.9, maxDtIncrement = 1.3;
+  // physics (grim_params)
+  grim_params physics{};
+  // Newton path of the two solves and the relative residual test
+  grim_newton_opts newton{};
+  int residualScaling = 0;
+
+  explicit Params(bool emhd = false) {
+    grim_default_params(&physics, emhd ? 1 : 0);
+    newton.jacobian = GRIM_JAC_FD;
+    newton.lu_precision = GRIM_LU_FP64;
+    newton.semantics = GRIM_NEWTON_PER_ZONE;
+  }
+};
+
+inline void check(int rc, const char* what) {
+  if (rc != GRIM_OK) throw std::runtime_error(std::string(what) + ": " + grim_last_error());
+}
+
+class timeStepper {
+  grim_ctx* ctx = nullptr;
+  grim_step_stats stats{};
+  grim_grid_config cfg{};
+
+ public:
+  double dt = 0., time = 0.;
+  int N1, N2, N3, numGhost, dim;
+  int numVars;
+  int boundaryLeft, boundaryRight;
+  int boundaryTop, boundaryBottom;
+  int boundaryFront, boundaryBack;
+
+  // timeStepper::timeStepper (src/timestepper/timestepper.cpp:3-386): the
+  // grid arrays and geometry (zone centres, lower faces, connection).  grim
+  // passes top/bottom in that order; they map to X2's upper/lower faces.
+  timeStepper(const int N1, const int N2, const int N3, const int dim, const int numVars,
+              const int numGhost, const double time, const double dt, const int boundaryLeft,
+              const int boundaryRight, const int boundaryTop, const int boundaryBottom,
+              const int boundaryFront, const int boundaryBack, const int metric,
+              const double blackHoleSpin, const double hSlope, const double X1Start,
+              const double X1End, const double X2Start, const double X2End, const double X3Start,
+              const double X3End, const Params& params = Params(), void* stream = nullptr)
+      : dt(dt), time(time), N1(N1), N2(N2), N3(N3), numGhost(numGhost), dim(dim), numVars(numVars),
+        boundaryLeft(boundaryLeft), boundaryRight(boundaryRight), boundaryTop(boundaryTop),
+        boundaryBottom(boundaryBottom), boundaryFront(boundaryFront), boundaryBack(boundaryBack) {
+    if (numVars != 8 + params.physics.conduction + params.physics.viscosity)
+      throw std::runtime_error("timeStepper: numVars does not match the conduction/viscosity switches");
+    cfg.N1 = N1;
+    cfg.N2 = N2;
+    cfg.N3 = N3;
+    cfg.dim = dim;
+    cfg.numGhost = numGhost;
+    cfg.X1Start = X1Start;
+    cfg.X1End = X1End;
+    cfg.X2Start = X2Start;
+    cfg.X2End = X2End;
+    cfg.X3Start = X3Start;
+    cfg.X3End = X3End;
+    cfg.metric = metric;
+    cfg.blackHoleSpin = blackHoleSpin;
+    cfg.hSlope = hSlope;
+    cfg.DerefineThetaHorizon = params.DerefineThetaHorizon;
+    cfg.DoCylindrify = params.DoCylindrify;
+    cfg.X1cyl = params.X1cyl;
+    cfg.X2cyl = params.X2cyl;
+    cfg.boundaryLeft = boundaryLeft;
+    cfg.boundaryRight = boundaryRight;
+    cfg.boundaryBottom = boundaryBottom;
+    cfg.boundaryTop = boundaryTop;
+    cfg.boundaryBack = boundaryBack;
+    cfg.boundaryFront = boundaryFront;
+    cfg.reconstruction = params.reconstruction;
+    cfg.riemannSolver = params.riemannSolver;
+    cfg.slopeLimTheta = params.slopeLimTheta;
+    cfg.CourantFactor = params.CourantFactor;
+    cfg.maxDtIncrement = params.maxDtIncrement;
+    cfg.residualScaling = params.residualScaling;
+    check(grim_create(&params.physics, &cfg, &params.newton, (grim_stream_t)stream, &ctx), "grim_create");
+  }
+  ~timeStepper() { grim_destroy(ctx); }
+  timeStepper(const timeStepper&) = delete;
+  timeStepper& operator=(const timeStepper&) = delete;
+
+  // Zones of one variable plane, ghosts included (N1T * N2T * N3T).
+  int64_t zones() const { return grim_grid_zones(&cfg); }
+
+  // timeStepper::initialConditions of the two problems this path serves
+  // (src/problem/linear_modes/problem.cpp:10-111, src/problem/torus/problem.cpp:145-540);
+  // a caller with its own initial state uses setPrim.  Either is followed by
+  // the start time and dt (the constructor's time and dt by default).
+  void initialConditionsLinearMode(int mode, double amplitude) {
+    check(grim_init_linear_mode(ctx, mode, amplitude), "grim_init_linear_mode");
+    setTime(time, dt);
+  }
+  void initialConditionsTorus(const grim_torus_params& torus) {
+    check(grim_init_torus(ctx, &torus), "grim_init_torus");
+    setTime(time, dt);
+  }
+
+  // grid::setVars / getVars: numVars planes of zones() doubles on the host.
+  void setPrim(const std::vector<double>& prim) {
+    if ((int64_t)prim.size() != (int64_t)numVars * zones()) throw std::runtime_error("setPrim: wrong size");
+    check(grim_set_prim(ctx, prim.data(), 0), "grim_set_prim");
+  }
+  std::vector<double> getPrim() const {
+    std::vector<double> out((size_t)numVars * (size_t)zones());
+    check(grim_get_prim(ctx, out.data(), 0), "grim_get_prim");
+    return out;
+  }
+  // Device address of primOld (numVars planes), for on-device analysis.
+  double* primOldDevice() const {
+    double* p = nullptr;
+    check(grim_prim(ctx, &p), "grim_prim");
+    return p;
+  }
+
+  void setTime(double t, double d) {
+    check(grim_set_time(ctx, t, d), "grim_set_time");
+    time = t;
+    dt = d;
+  }
+
+  // timeStepper::timeStep (src/timestepper/timestep.cpp:3-334).  grim fills
+  // numReads/numWrites with array-access counts for its bandwidth report;
+  // kernel traffic here is measured with rocprofv3 instead, so both are 0.
+  void timeStep(int& numReads, int& numWrites) {
+    check(grim_time_step(ctx, &stats), "grim_time_step");
+    check(grim_get_time(ctx, &time, &dt), "grim_get_time");
+    numReads = 0;
+    numWrites = 0;
+  }
+
+  // The same step without statistics or host synchronisation (dt and time
+  // advance on the device; call syncTime() to read them).
+  void timeStepAsync() { check(grim_time_step(ctx, nullptr), "grim_time_step"); }
+  void syncTime() { check(grim_get_time(ctx, &time, &dt), "grim_get_time"); }
+
+  // timeStepper::computeDt (timestep.cpp:336-400) from the current primOld:
+  // the next dt, bounded by maxDtIncrement times the current one.
+  void computeDt(int& numReads, int& numWrites) {
+    double next = 0.;
+    check(grim_compute_dt(ctx, primOldDevice(), dt, &next), "grim_compute_dt");
+    numReads = 0;
+    numWrites = 0;
+    dt = next;
+  }
+
+  // Statistics of the last timeStep: dt, unconverged zones and largest
+  // Newton iteration count per stage, zones with a non-finite residual.
+  const grim_step_stats& lastStats() const { return stats; }
+  grim_ctx* context() const { return ctx; }
+};
+
+}  // namespace grim
