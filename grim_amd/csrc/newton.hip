@@ -368,7 +368,7 @@ int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
   cfg.compaction = o.compaction != 0 || escalate;
   cfg.escalate = escalate;
   cfg.budget1 = o.phase1_iters ? o.phase1_iters : 2;
-  cfg.group = o.phase2_group ? o.phase2_group : 2;
+  cfg.group = o.phase2_group ? o.phase2_group : 1;
   // persistent phase 2: 8 wavefronts per compute unit
   cfg.persistentWaves = 8 * deviceCUs();
   cfg.ws = CompactionWorkspace{};

@@ -201,6 +201,16 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
                                           const grim_newton_stats& st, int64_t z, int budget,
                                           bool accumulate, bool lastPhase) {
   const int64_t n = b.n;
+  if (b.residual_mask && b.residual_mask[z] == 0.) {
+    // outside the residual mask (ghost zones of a grid) R = 0: the zone
+    // takes its floors and stops at the first test, as in the full path
+    b.prim[RHO * n + z] = smax(b.prim[RHO * n + z], prm.rhoFloor);
+    b.prim[UU * n + z] = smax(b.prim[UU * n + z], prm.uFloor);
+    if (st.iterations) st.iterations[z] = accumulate ? st.iterations[z] : 0;
+    if (st.residual_evals) st.residual_evals[z] = (accumulate ? st.residual_evals[z] : 0) + 1;
+    if (st.final_l2) st.final_l2[z] = 0.;
+    return false;
+  }
   const double dt = batchDt(b);
   ZoneGeom G;
   loadGeom(b.geom, n, z, G);

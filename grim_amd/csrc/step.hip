@@ -1257,9 +1257,23 @@ int communicate(grim_ctx* x, double* prim, int nvars) {
   return checkLaunch("communicate");
 }
 
+// Faces the stage reads: along DIR the faces of the bulk zones, across it
+// one zone beyond the bulk (the flux-CT EMFs read the transverse
+// neighbours' fluxes); the whole face range when the caller asks for the
+// flux arrays themselves.
+Box stageFaceBox(const Dims& D, int dir) {
+  Box b = faceBox(D, dir);
+  for (int d = 0; d < 3; d++) {
+    if (d == dir || d >= D.dim) continue;
+    b.lo[d] = D.g[d] - 1;
+    b.hi[d] = D.g[d] + D.N[d] + 1;
+  }
+  return b;
+}
+
 template <class L, int DIR>
-void launchFace(grim_ctx* x, const double* prim, double* flux) {
-  const Box b = faceBox(x->D, DIR);
+void launchFace(grim_ctx* x, const double* prim, double* flux, bool allFaces = true) {
+  const Box b = allFaces ? faceBox(x->D, DIR) : stageFaceBox(x->D, DIR);
   const double* gf = DIR == 0 ? x->geomX1 : (DIR == 1 ? x->geomX2 : x->geomC);
   const unsigned nb = (unsigned)((b.count() + kFacesPerBlock - 1) / kFacesPerBlock);
   switch (x->c.reconstruction) {
@@ -1276,11 +1290,11 @@ void launchFace(grim_ctx* x, const double* prim, double* flux) {
 
 // Riemann fluxes of every active direction into x->fluxes, then the EMFs.
 template <class L>
-int fluxesAndEmf(grim_ctx* x, const double* prim) {
+int fluxesAndEmf(grim_ctx* x, const double* prim, bool allFaces) {
   const size_t plane = (size_t)L::DOF * x->D.n;
-  launchFace<L, 0>(x, prim, x->fluxes);
-  if (x->D.dim > 1) launchFace<L, 1>(x, prim, x->fluxes + plane);
-  if (x->D.dim > 2) launchFace<L, 2>(x, prim, x->fluxes + 2 * plane);
+  launchFace<L, 0>(x, prim, x->fluxes, allFaces);
+  if (x->D.dim > 1) launchFace<L, 1>(x, prim, x->fluxes + plane, allFaces);
+  if (x->D.dim > 2) launchFace<L, 2>(x, prim, x->fluxes + 2 * plane, allFaces);
   if (x->D.dim > 1) {
     const Box e = edgeBox(x->D);
     emfKernel<L><<<blocksFor(e.count()), kBlock, 0, x->s>>>(x->D, e, x->fluxes, x->emf);
@@ -1396,7 +1410,7 @@ int stage(grim_ctx* x, int st, grim_step_stats* stats) {
   if (int rc = explicitSources<L>(x, primStage, x->primOld, x->sExp, L::EMHD ? x->sImp : nullptr,
                                   st == 0 ? x->consOld : nullptr))
     return rc;
-  if (int rc = fluxesAndEmf<L>(x, primStage)) return rc;
+  if (int rc = fluxesAndEmf<L>(x, primStage, false)) return rc;
   GRIM_HIP_TRY(hipMemcpyAsync(out, primStage, bytes, hipMemcpyDeviceToDevice, x->s));
   const Box b = bulkBox(D);
   foldKernel<L><<<blocksFor(b.count()), kBlock, 0, x->s>>>(x->a, b, x->fluxes, x->emf, x->geomC, x->consOld,
@@ -1732,7 +1746,7 @@ int grim_riemann_flux(grim_ctx* x, const double* prim, int dir, double* flux) {
 int grim_div_fluxes(grim_ctx* x, const double* prim, double* div_f, double* fluxes) {
   GRIM_REQUIRE(x && prim && div_f, "grim_div_fluxes: bad argument");
   return withLayout(x, [&]<class L>() -> int {
-    if (int rc = fluxesAndEmf<L>(x, prim)) return rc;
+    if (int rc = fluxesAndEmf<L>(x, prim, fluxes != nullptr)) return rc;
     const Box b = bulkBox(x->D);
     foldKernel<L><<<blocksFor(b.count()), kBlock, 0, x->s>>>(x->a, b, x->fluxes, x->emf, x->geomC, nullptr,
                                                              nullptr, nullptr, nullptr, div_f, nullptr, nullptr,
