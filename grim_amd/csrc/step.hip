@@ -534,18 +534,36 @@ __global__ __launch_bounds__(kBlock) void sourcesKernel(StepArgs a, Box box,
 #pragma unroll
   for (int v = 0; v < L::NF; v++) src[v] = 0.;
   if (a.metric != GRIM_METRIC_MINKOWSKI) {
-    double T[4][4];
-    stressRow<L, 0>(kk, cc, T[0]);
-    stressRow<L, 1>(kk, cc, T[1]);
-    stressRow<L, 2>(kk, cc, T[2]);
-    stressRow<L, 3>(kk, cc, T[3]);
+    // One row of T^kappa_lambda at a time: every src[UU + nu] still receives
+    // its 16 terms in grim's (kappa, lambda) order, while only four stress
+    // values and 16 connection planes are live at once.
+    auto contract = [&](const double (&Tk)[4], int kappa) {
 #pragma unroll
-    for (int nu = 0; nu < 4; nu++)
+      for (int lamda = 0; lamda < 4; lamda++)
 #pragma unroll
-      for (int kappa = 0; kappa < 4; kappa++)
-#pragma unroll
-        for (int lamda = 0; lamda < 4; lamda++)
-          src[UU + nu] -= g * T[kappa][lamda] * gamma[(lamda * 16 + kappa * 4 + nu) * n + z];
+        for (int nu = 0; nu < 4; nu++)
+          src[UU + nu] -= g * Tk[lamda] * gamma[(lamda * 16 + kappa * 4 + nu) * n + z];
+    };
+    {
+      double T[4];
+      stressRow<L, 0>(kk, cc, T);
+      contract(T, 0);
+    }
+    {
+      double T[4];
+      stressRow<L, 1>(kk, cc, T);
+      contract(T, 1);
+    }
+    {
+      double T[4];
+      stressRow<L, 2>(kk, cc, T);
+      contract(T, 2);
+    }
+    {
+      double T[4];
+      stressRow<L, 3>(kk, cc, T);
+      contract(T, 3);
+    }
   }
   if constexpr (L::EMHD) {
     auto gam = [&](int l, int p, int q) { return gamma[(l * 16 + p * 4 + q) * n + z]; };
