@@ -400,6 +400,33 @@ def torus_one_step_error(cfg, N1, N2, N3):
             "note": "rho, u relative to themselves; every variable relative to its largest value"}
 
 
+OBS_BYTES_PER_ZONE = (8 + 16) * 8  # rho, u, u~^i, B^i + gCov (10), gCon row 0, alpha, g
+
+
+def observers_bench(ts, zones, calls=20):
+    """ComputeEnergyIntegrals + ComputeMinMaxVariables + ComputeBoundaryFluxes
+    (grim_torus_observers) on primOld: HIP events on the context's stream
+    around each call (both reduction kernels and the 72-byte readback)."""
+    import torch
+
+    s = ts._stream
+    for _ in range(2):
+        ts.observers()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(calls)]
+    t0 = time.perf_counter()
+    for e0, e1 in ev:
+        e0.record(s)
+        obs = ts.observers()
+        e1.record(s)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / calls
+    ms = sum(e0.elapsed_time(e1) for e0, e1 in ev) / calls
+    return {"ms_per_call": ms, "wall_ms_per_call": wall, "bytes_per_zone": OBS_BYTES_PER_ZONE,
+            "achieved_GBps": OBS_BYTES_PER_ZONE * zones / (ms * 1e-3) / 1e9,
+            "frac_of_hbm_peak": OBS_BYTES_PER_ZONE * zones / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "values_after_steps": obs}
+
+
 def full_step_bench(args):
     """The whole time step (timeStepper::timeStep) on two workloads:
     the 3D EMHD Kerr torus at 192x192x128 with both Newton paths, and the 2D
@@ -439,6 +466,8 @@ def full_step_bench(args):
             "dt_next_step": last.dt,
         }
         states[name] = ts.primOld
+        if name == "mixed":
+            torus["observers"] = observers_bench(ts, zones)
         ts.close()
         del ts
         torch.cuda.empty_cache()

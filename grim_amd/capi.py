@@ -162,6 +162,17 @@ class StepStats(C.Structure):
                 ("nonfinite", C.c_int64 * 2)]
 
 
+class TorusObservables(C.Structure):
+    """grim_torus_observables (ObserversTorus.hpp:6-251)."""
+
+    _fields_ = [(name, C.c_double) for name in (
+        "baryon_mass", "magnetic_energy", "thermal_energy", "mdot_in", "mdot_out", "mdot_out_unbound",
+        "mdot_out_relativistic", "rho_max", "beta_min")]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class TorusParams(C.Structure):
     """grim_torus_params (src/problem/torus/params.cpp:38-80)."""
 
@@ -229,6 +240,7 @@ SIGNATURES = {
     "grim_init_torus": (_i, [_vp, C.POINTER(TorusParams)]),
     "grim_apply_floor": (_i, [_vp, _vp]),
     "grim_torus_bcs": (_i, [_vp, _vp]),
+    "grim_torus_observers": (_i, [_vp, _vp, _vp]),
 }
 
 _lib = None

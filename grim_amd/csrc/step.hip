@@ -1075,6 +1075,132 @@ __global__ __launch_bounds__(kBlock) void nanZeroKernel(int64_t n, int nvars, co
     for (int v = 0; v < nvars; v++) prim[v * n + z] = 0.;
 }
 
+// On-the-fly observers of fullStepDiagnostics (:757-776), defined in
+// src/problem/torus/ObserversTorus.hpp, of the element set at prim:
+//  ComputeEnergyIntegrals (:6-81)   sums over the bulk of rho, b^2 and u
+//                                   times volElem*g*gamma/alpha
+//  ComputeMinMaxVariables (:83-139) max rho, min 2(p+1e-13)/(b^2+1e-18)
+//  ComputeBoundaryFluxes (:141-251) sums of rho*volElem*g*u^1 over the first
+//                                   and last bulk X1 planes; on the last, the
+//                                   unbound part ((1+u/rho*Gamma)u_0 < -1) and
+//                                   its relativistic part (gamma > 2)
+// Each workgroup strides over a fixed share of the bulk and stores its
+// partial sums, max and min; obsFinishKernel combines the partials in a
+// fixed order, so repeated calls give identical results.
+enum {
+  OBS_MASS,
+  OBS_EMAG,
+  OBS_ETH,
+  OBS_MDOT_IN,
+  OBS_MDOT_OUT,
+  OBS_MDOT_UNBOUND,
+  OBS_MDOT_REL,
+  OBS_SUMS,
+  OBS_RHO_MAX = OBS_SUMS,
+  OBS_BETA_MIN,
+  OBS_WORDS
+};
+constexpr unsigned kObsBlocks = 2048;
+
+// word q of two partials combined: sums add, the maximum and minimum as such
+__device__ __forceinline__ double obsCombine(int q, double a, double b) {
+  return q < OBS_SUMS ? a + b : (q == OBS_RHO_MAX ? fmax(a, b) : fmin(a, b));
+}
+
+template <class L>
+__global__ __launch_bounds__(kBlock) void observersKernel(StepArgs a, Box box, double volElem,
+                                                          const double* __restrict__ prim,
+                                                          const double* __restrict__ geomC,
+                                                          double* __restrict__ partial) {
+  const int64_t n = a.D.n;
+  const int iIn = box.lo[0], iOut = box.hi[0] - 1;
+  double s[OBS_SUMS];
+#pragma unroll
+  for (int q = 0; q < OBS_SUMS; q++) s[q] = 0.;
+  double rhoMax = -INFINITY, betaMin = INFINITY;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < box.count();
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    box.at(t, i, j, k);
+    const int64_t z = flatIndex(a.D, i, j, k);
+    double P[L::DOF];
+#pragma unroll
+    for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
+    ZoneGeom G;
+    loadGeom(geomC, n, z, G);
+    Kinematics e;
+    kinematics<L>(P, G, a.prm, e);
+    const double rho = P[RHO], u = P[UU];
+    s[OBS_MASS] += rho * volElem * G.g * e.gammaLF / G.alpha;
+    s[OBS_EMAG] += volElem * e.bSqr * G.g * e.gammaLF / G.alpha;
+    s[OBS_ETH] += volElem * u * G.g * e.gammaLF / G.alpha;
+    rhoMax = fmax(rhoMax, rho);
+    betaMin = fmin(betaMin, 2. * (e.pressure + 1.e-13) / (e.bSqr + 1.e-18));
+    if (i == iIn || i == iOut) {
+      const double mdot = rho * volElem * G.g * e.uCon[1];
+      if (i == iIn) s[OBS_MDOT_IN] += mdot;
+      if (i == iOut) {
+        const double bernoulli = (1. + u / rho * a.prm.gam) * e.uCov[0];
+        const double unbound = mdot * (bernoulli < -1. ? 1. : 0.);
+        s[OBS_MDOT_OUT] += mdot;
+        s[OBS_MDOT_UNBOUND] += unbound;
+        s[OBS_MDOT_REL] += unbound * (e.gammaLF > 2. ? 1. : 0.);
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < OBS_SUMS; q++) s[q] += __shfl_xor(s[q], off, 64);
+    rhoMax = fmax(rhoMax, __shfl_xor(rhoMax, off, 64));
+    betaMin = fmin(betaMin, __shfl_xor(betaMin, off, 64));
+  }
+  __shared__ double sh[kBlock / 64][OBS_WORDS];
+  const int w = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < OBS_SUMS; q++) sh[w][q] = s[q];
+    sh[w][OBS_RHO_MAX] = rhoMax;
+    sh[w][OBS_BETA_MIN] = betaMin;
+  }
+  __syncthreads();
+  if (threadIdx.x < OBS_WORDS) {
+    const int q = threadIdx.x;
+    double r = sh[0][q];
+    for (int v = 1; v < kBlock / 64; v++)
+      r = obsCombine(q, r, sh[v][q]);
+    partial[(int64_t)blockIdx.x * OBS_WORDS + q] = r;
+  }
+}
+
+// One workgroup combines the kObsBlocks partials of every word in a fixed
+// order (a strided pass with all words in flight, then a tree in LDS),
+// writing out[q].
+__global__ __launch_bounds__(kBlock) void obsFinishKernel(const double* __restrict__ partial,
+                                                          double* __restrict__ out) {
+  static_assert(kObsBlocks % kBlock == 0, "every thread starts from one partial");
+  __shared__ double sh[OBS_WORDS][kBlock];
+  const int t = threadIdx.x;
+  double r[OBS_WORDS];
+#pragma unroll
+  for (int q = 0; q < OBS_WORDS; q++) r[q] = partial[(int64_t)t * OBS_WORDS + q];
+  for (unsigned b = t + kBlock; b < kObsBlocks; b += kBlock) {
+#pragma unroll
+    for (int q = 0; q < OBS_WORDS; q++) r[q] = obsCombine(q, r[q], partial[(int64_t)b * OBS_WORDS + q]);
+  }
+#pragma unroll
+  for (int q = 0; q < OBS_WORDS; q++) sh[q][t] = r[q];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (t < w) {
+#pragma unroll
+      for (int q = 0; q < OBS_WORDS; q++) sh[q][t] = obsCombine(q, sh[q][t], sh[q][t + w]);
+    }
+    __syncthreads();
+  }
+  if (t < OBS_WORDS) out[t] = sh[t][0];
+}
+
 // inflowCheck (:941-1044) on the radial ghost slabs.
 template <class L>
 __global__ __launch_bounds__(kBlock) void inflowCheckKernel(StepArgs a, grim_torus_params t, Box box, int side,
@@ -1186,6 +1312,7 @@ struct grim_ctx {
   double *fluxes, *emf, *sExp, *sImp, *uT, *rhs, *mask;
   double *geomC, *geomX1, *geomX2, *gamma;
   double* red;
+  double* obs;  // observers: per-workgroup partials, then the result
   int* iters;
   double* l2;
   unsigned long long* counters;
@@ -1388,6 +1515,20 @@ int scrubNaNs(grim_ctx* x, double* prim, int nvars) {
   return checkLaunch("NaN scrub");
 }
 
+// The observers of prim into x->obs[kObsBlocks * OBS_WORDS + q] (device);
+// volElem = dX1 (* dX2 from dim 2, * dX3 from dim 3) as in
+// fullStepDiagnostics (torus/problem.cpp:764-774).
+template <class L>
+int observers(grim_ctx* x, const double* prim) {
+  const Box b = bulkBox(x->D);
+  double volElem = x->a.dX[0];
+  if (x->c.dim > 1) volElem *= x->a.dX[1];
+  if (x->c.dim > 2) volElem *= x->a.dX[2];
+  observersKernel<L><<<kObsBlocks, kBlock, 0, x->s>>>(x->a, b, volElem, prim, x->geomC, x->obs);
+  obsFinishKernel<<<1, kBlock, 0, x->s>>>(x->obs, x->obs + (size_t)kObsBlocks * OBS_WORDS);
+  return checkLaunch("observers");
+}
+
 // setProblemSpecificBCs of the torus: inflowCheck on both radial ghost
 // slabs, then fixPoles (problem.cpp:1217-1241).
 template <class L>
@@ -1567,6 +1708,7 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
       {(void**)&x->geomX2, locB},
       {(void**)&x->gamma, sizeof(double) * 64 * (size_t)n},
       {(void**)&x->red, sizeof(double) * ((size_t)nb + 1)},
+      {(void**)&x->obs, sizeof(double) * ((size_t)kObsBlocks + 1) * OBS_WORDS},
       {(void**)&x->iters, sizeof(int) * (size_t)n},
       {(void**)&x->l2, sizeof(double) * (size_t)n},
       {(void**)&x->counters, 4 * sizeof(unsigned long long)},
@@ -1583,7 +1725,7 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
   }
   x->opts.workspace = x->newtonWs;
   x->opts.workspace_bytes = grim_newton_workspace_bytes(n);
-  if (hipHostMalloc((void**)&x->hostBuf, 64) != hipSuccess) {
+  if (hipHostMalloc((void**)&x->hostBuf, 128) != hipSuccess) {
     setError("grim_create: hipHostMalloc failed");
     grim_destroy(x);
     return GRIM_ERR_OOM;
@@ -1626,7 +1768,7 @@ int grim_destroy(grim_ctx* x) {
   void* ptrs[] = {x->primOld, x->primHalf, x->primNew, x->primDt, x->consOld, x->fluxes, x->emf,
                   x->sExp,    x->sImp,     x->uT,      x->rhs,    x->mask,    x->geomC,  x->geomX1,
                   x->geomX2,  x->gamma,    x->red,     x->iters,  x->l2,      x->counters,
-                  x->newtonWs, x->clock};
+                  x->newtonWs, x->clock,  x->obs};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (x->hostBuf) (void)hipHostFree(x->hostBuf);
@@ -1839,6 +1981,26 @@ int grim_apply_floor(grim_ctx* x, double* prim) {
 int grim_torus_bcs(grim_ctx* x, double* prim) {
   GRIM_REQUIRE(x && prim && x->torus, "grim_torus_bcs: no torus (grim_init_torus) or null prim");
   return withLayout(x, [&]<class L>() -> int { return torusBCs<L>(x, prim); });
+}
+
+int grim_torus_observers(grim_ctx* x, const double* prim, grim_torus_observables* out) {
+  GRIM_REQUIRE(x && out, "grim_torus_observers: null argument");
+  if (!prim) prim = x->primOld;
+  if (int rc = withLayout(x, [&]<class L>() -> int { return observers<L>(x, prim); })) return rc;
+  GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf, x->obs + (size_t)kObsBlocks * OBS_WORDS,
+                              sizeof(double) * OBS_WORDS, hipMemcpyDeviceToHost, x->s));
+  GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+  const double* r = x->hostBuf;
+  out->baryon_mass = r[OBS_MASS];
+  out->magnetic_energy = r[OBS_EMAG];
+  out->thermal_energy = r[OBS_ETH];
+  out->mdot_in = r[OBS_MDOT_IN];
+  out->mdot_out = r[OBS_MDOT_OUT];
+  out->mdot_out_unbound = r[OBS_MDOT_UNBOUND];
+  out->mdot_out_relativistic = r[OBS_MDOT_REL];
+  out->rho_max = r[OBS_RHO_MAX];
+  out->beta_min = r[OBS_BETA_MIN];
+  return GRIM_OK;
 }
 
 int grim_init_linear_mode(grim_ctx* x, int mode, double amplitude) {

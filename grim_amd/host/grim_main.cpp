@@ -5,6 +5,7 @@
 //
 //   grim_step [--problem fastmode|emhdmode|torus] [--N N1 N2 N3] [--steps K]
 //             [--solver fd|fd_lockstep|mixed] [--maxiter M] [--dump FILE]
+//             [--observe-every DT]
 //
 // fastmode: 2D Minkowski ideal MHD, 64x64 periodic, WENO5 + LLF, the fast
 //           magnetosonic eigenmode at amplitude 1e-4, dt0 = 0.5/N1;
@@ -12,7 +13,9 @@
 // torus:    3D MKS torus (a = 0.9375, r_in = 6, r_max = 12, SANE beta 100),
 //           EMHD with the torus closure, WENO5 + LLF, relative residual test.
 // --dump writes primOld (numVars planes of N1T*N2T*N3T doubles, X1 fastest)
-// after the last step.  Exit status 1 on any library error.
+// after the last step.  --observe-every DT prints the torus observers
+// whenever time crosses a multiple of DT (fullStepDiagnostics,
+// torus/problem.cpp:752-776).  Exit status 1 on any library error.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -32,6 +35,7 @@ struct Options {
   std::string solver = "fd";
   int maxIter = 10;
   std::string dump;
+  double observeEvery = 0.;
 };
 
 Options parse(int argc, char** argv) {
@@ -54,6 +58,8 @@ Options parse(int argc, char** argv) {
       o.maxIter = std::atoi(next());
     } else if (a == "--dump") {
       o.dump = next();
+    } else if (a == "--observe-every") {
+      o.observeEvery = std::atof(next());
     } else {
       throw std::runtime_error("unknown argument " + a);
     }
@@ -158,6 +164,16 @@ int run(const Options& o) {
                 "zone cycles/sec = %.4e\n",
                 ts.time, st.dt, (long long)st.unconverged[0], (long long)st.unconverged[1],
                 st.max_iterations[0], st.max_iterations[1], zones / s);
+    // ObserveData (torus/problem.cpp:752): time crossed a multiple of the
+    // observation interval during this step
+    if (o.observeEvery > 0. &&
+        std::floor(ts.time / o.observeEvery) != std::floor((ts.time - st.dt) / o.observeEvery)) {
+      std::printf("Global quantities at t = %e\n", ts.time);
+      const grim_torus_observables obs = ts.observe();
+      grim::timeStepper::ComputeMinMaxVariables(obs);
+      grim::timeStepper::ComputeEnergyIntegrals(obs);
+      grim::timeStepper::ComputeBoundaryFluxes(obs);
+    }
   }
   std::printf("%d steps of %s (%s): %.4e zone cycles/sec\n", o.steps, o.problem.c_str(), o.solver.c_str(),
               zones * o.steps / total);

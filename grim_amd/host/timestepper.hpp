@@ -15,6 +15,7 @@
 #pragma once
 
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -177,6 +178,25 @@ class timeStepper {
     numReads = 0;
     numWrites = 0;
     dt = next;
+  }
+
+  // The torus's on-the-fly observers of primOld (ObserversTorus.hpp:6-251),
+  // all three from one pass over the grid; each prints grim's line.
+  grim_torus_observables observe() {
+    grim_torus_observables o{};
+    check(grim_torus_observers(ctx, nullptr, &o), "grim_torus_observers");
+    return o;
+  }
+  static void ComputeMinMaxVariables(const grim_torus_observables& o) {
+    std::printf("rhoMax = %e; betaMin = %e;\n", o.rho_max, o.beta_min);
+  }
+  static void ComputeEnergyIntegrals(const grim_torus_observables& o) {
+    std::printf("Baryon Mass = %e; Magnetic Energy = %e; Thermal Energy = %e;\n", o.baryon_mass,
+                o.magnetic_energy, o.thermal_energy);
+  }
+  static void ComputeBoundaryFluxes(const grim_torus_observables& o) {
+    std::printf("MdotIn = %e; MdotOut = %e; ( Unbound = %e; Relativistic = %e)\n", o.mdot_in, o.mdot_out,
+                o.mdot_out_unbound, o.mdot_out_relativistic);
   }
 
   // Statistics of the last timeStep: dt, unconverged zones and largest

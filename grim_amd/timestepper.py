@@ -118,6 +118,15 @@ class TimeStepper:
         capi.check(capi.lib().grim_torus_bcs(self._ctx, capi._ptr(prim)), "grim_torus_bcs")
         return prim
 
+    def observers(self, prim=None) -> dict:
+        """ComputeEnergyIntegrals, ComputeMinMaxVariables and
+        ComputeBoundaryFluxes (ObserversTorus.hpp:6-251) of prim (a device
+        tensor of dof planes; None: primOld), as a dict."""
+        out = capi.TorusObservables()
+        capi.check(capi.lib().grim_torus_observers(self._ctx, None if prim is None else capi._ptr(prim),
+                                                   C.byref(out)), "grim_torus_observers")
+        return out.as_dict()
+
     # -- the step -----------------------------------------------------------
     def timeStep(self, stats: bool = True) -> capi.StepStats | None:
         """One step.  stats=False enqueues it without a host synchronisation

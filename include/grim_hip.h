@@ -356,6 +356,32 @@ int grim_apply_floor(grim_ctx* ctx, double* prim);
 /* setProblemSpecificBCs of the torus on caller-provided primitive planes. */
 int grim_torus_bcs(grim_ctx* ctx, double* prim);
 
+/* The torus's on-the-fly observers (src/problem/torus/ObserversTorus.hpp,
+ * called from fullStepDiagnostics, problem.cpp:757-776), of the element set
+ * at the given primitives, over the bulk zones; volElem = dX1 (* dX2 when
+ * dim > 1, * dX3 when dim > 2). */
+typedef struct grim_torus_observables {
+  /* ComputeEnergyIntegrals (:6-81): sums of rho, b^2 and u, each times
+   * volElem * g * gamma / alpha */
+  double baryon_mass, magnetic_energy, thermal_energy;
+  /* ComputeBoundaryFluxes (:141-251): sums of rho * volElem * g * u^1 over
+   * the first (in) and last (out) bulk X1 planes; of the outflow, the part
+   * with (1 + u/rho * Gamma) u_0 < -1 (unbound) and of that the part with
+   * gamma > 2 (relativistic) */
+  double mdot_in, mdot_out, mdot_out_unbound, mdot_out_relativistic;
+  /* ComputeMinMaxVariables (:83-139): max rho, min 2 (p + 1e-13) / (b^2 +
+   * 1e-18); NaN zones are skipped (fmax / fmin) */
+  double rho_max, beta_min;
+} grim_torus_observables;
+
+/* Replaces ComputeEnergyIntegrals, ComputeMinMaxVariables and
+ * ComputeBoundaryFluxes (ObserversTorus.hpp:6-251) on one rank.  prim:
+ * device primitive planes, or NULL for primOld.  Valid on every grid (the
+ * reference calls it for the torus).  Sums are combined in a fixed order:
+ * repeated calls on the same state return identical values.  Synchronises
+ * the context's stream. */
+int grim_torus_observers(grim_ctx* ctx, const double* prim, grim_torus_observables* out);
+
 #ifdef __cplusplus
 }
 #endif

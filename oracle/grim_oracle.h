@@ -284,6 +284,14 @@ void grim_oracle_torus_bcs(const grim_oracle_grid* c, const grim_oracle_params* 
                            const grim_oracle_torus* t, const grim_oracle_geometry* gm,
                            double* prim);
 
+/* The torus's observers (src/problem/torus/ObserversTorus.hpp:6-251) of the
+ * element set at prim, over the bulk: out[9] = baryon mass, magnetic and
+ * thermal energy, mdot in, out, out unbound, out relativistic, rho max,
+ * beta min. */
+void grim_oracle_torus_observers(const grim_oracle_grid* c, const grim_oracle_params* p,
+                                 const grim_oracle_geometry* gm, const double* prim,
+                                 double* out);
+
 typedef struct grim_oracle_step_stats {
   double dt;
   int64_t unconverged[2];   /* bulk zones above atol after each stage */

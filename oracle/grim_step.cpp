@@ -1517,4 +1517,57 @@ void grim_oracle_torus_bcs(const grim_oracle_grid* c, const grim_oracle_params* 
   torusBCs(*c, *p, *t, D, *gm, prim);
 }
 
+// fullStepDiagnostics' observers (torus/problem.cpp:757-776) as written in
+// src/problem/torus/ObserversTorus.hpp, on one rank: the element is set at
+// prim (centre geometry); af::sum(af::flat(X(domain))) becomes a serial sum
+// over the bulk in flat order (X1 fastest); af::max / af::min over the
+// bulk become fmax / fmin from -inf / +inf.
+void grim_oracle_torus_observers(const grim_oracle_grid* c, const grim_oracle_params* p,
+                                 const grim_oracle_geometry* gm, const double* prim,
+                                 double* out) {
+  const Dims D(*c);
+  const Vars vars(*p);
+  const int64_t n = D.n;
+  // volElem (problem.cpp:764-774)
+  double volElem = dXof(*c, 0);
+  if (c->dim > 1) volElem *= dXof(*c, 1);
+  if (c->dim > 2) volElem *= dXof(*c, 2);
+  double mass = 0., eMag = 0., eTh = 0., rhoMax = -INFINITY, betaMin = INFINITY;
+  double mdotIn = 0., mdotOut = 0., unbound = 0., relativistic = 0.;
+  const int iIn = D.g[0], iOut = D.g[0] + D.N[0] - 1;  // MassIntegrand(3, ...), (N1Local+2, ...)
+  for (int k = D.g[2]; k < D.g[2] + D.N[2]; k++)
+    for (int j = D.g[1]; j < D.g[1] + D.N[1]; j++)
+      for (int i = D.g[0]; i < D.g[0] + D.N[0]; i++) {
+        const int64_t z = D.idx(i, j, k);
+        double P[GRIM_ORACLE_MAX_VARS];
+        loadPrim(prim, vars.dof, n, z, P);
+        ZoneGeom<double> G;
+        loadLocation(gm->center, n, z, &G);
+        FluidElement<double> e;
+        e.set(P, G, *p, ZoneClosure{1., 1.});
+        const double rho = P[0], u = P[1];
+        // ComputeEnergyIntegrals (:17, :40, :62)
+        mass += rho * volElem * G.g * e.gammaLorentzFactor / G.alpha;
+        eMag += volElem * e.bSqr * G.g * e.gammaLorentzFactor / G.alpha;
+        eTh += volElem * u * G.g * e.gammaLorentzFactor / G.alpha;
+        // ComputeMinMaxVariables (:93, :115-118)
+        rhoMax = std::fmax(rhoMax, rho);
+        betaMin = std::fmin(betaMin, 2. * (e.pressure + 1.e-13) / (e.bSqr + 1.e-18));
+        // ComputeBoundaryFluxes (:150-173)
+        if (i == iIn || i == iOut) {
+          const double mdot = rho * volElem * G.g * e.uCon[1];
+          if (i == iIn) mdotIn += mdot;
+          if (i == iOut) {
+            const double bernoulli = (1. + u / rho * p->adiabaticIndex) * e.uCov[0];
+            const double unb = mdot * (bernoulli < -1. ? 1. : 0.);
+            mdotOut += mdot;
+            unbound += unb;
+            relativistic += unb * (e.gammaLorentzFactor > 2. ? 1. : 0.);
+          }
+        }
+      }
+  const double r[9] = {mass, eMag, eTh, mdotIn, mdotOut, unbound, relativistic, rhoMax, betaMin};
+  std::copy(r, r + 9, out);
+}
+
 }  // extern "C"

@@ -190,6 +190,7 @@ def lib():
         L.grim_oracle_torus_ic.argtypes = [Gp, P, Tp, Gm, vp]
         L.grim_oracle_apply_floor.argtypes = [Gp, P, Tp, Gm, vp]
         L.grim_oracle_torus_bcs.argtypes = [Gp, P, Tp, Gm, vp]
+        L.grim_oracle_torus_observers.argtypes = [Gp, P, Gm, vp, vp]
         _lib = L
     return _lib
 
@@ -521,3 +522,15 @@ def torus_bcs(g: Grid, p: Params, t: Torus, gm: Geometry, prim):
     prim = _f64(prim).copy()
     lib().grim_oracle_torus_bcs(C.byref(g), C.byref(p), C.byref(t), C.byref(gm._c()), _ptr(prim))
     return prim
+
+
+OBSERVABLES = ("baryon_mass", "magnetic_energy", "thermal_energy", "mdot_in", "mdot_out", "mdot_out_unbound",
+               "mdot_out_relativistic", "rho_max", "beta_min")
+
+
+def torus_observers(g: Grid, p: Params, gm: Geometry, prim) -> dict:
+    """ComputeEnergyIntegrals, ComputeMinMaxVariables, ComputeBoundaryFluxes
+    (ObserversTorus.hpp) of the element set at prim, as a dict."""
+    out = np.zeros(9)
+    lib().grim_oracle_torus_observers(C.byref(g), C.byref(p), C.byref(gm._c()), _ptr(_f64(prim)), _ptr(out))
+    return dict(zip(OBSERVABLES, out.tolist()))

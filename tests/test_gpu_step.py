@@ -418,6 +418,33 @@ def test_torus_emhd_steps_match(torch, solver):
     assert np.percentile(rel[ideal], 99) < 5e-7 and np.percentile(scaled, 99) < 1e-9
 
 
+@pytest.mark.parametrize("emhd", [False, True])
+def test_torus_observers_match_oracle(torch, emhd):
+    """ObserversTorus.hpp on a torus state after one step (inflow through the
+    inner X1 plane): the maximum and minimum bit-identical to the oracle's;
+    each sum within 1e-12 of the sum of its absolute integrands (device and
+    host add in different orders).  Repeated calls return identical values
+    and prim = NULL reads primOld."""
+    import observers_numpy as ON
+
+    ts, g, p, po, gm, t, floors = torus_case(emhd, maxNonLinearIter=6)
+    ts.set_time(0., 0.005)
+    ts.timeStep()
+    prim = ts.primOld
+    got = ts.observers()
+    ref = O.torus_observers(g, po, gm, prim)
+    _, scale = ON.observers(g, po, gm.center, prim)
+    print(f"observers ({'EMHD' if emhd else 'ideal'}): " +
+          ", ".join(f"{k} {got[k]:.6e} (diff {got[k] - ref[k]:.1e})" for k in ON.NAMES), flush=True)
+    for k in ("rho_max", "beta_min"):
+        assert got[k] == ref[k], (k, got[k], ref[k])
+    for k, s in scale.items():
+        assert abs(got[k] - ref[k]) <= 1e-12 * s, (k, got[k], ref[k], s)
+    assert ref["mdot_in"] < 0.
+    assert ts.observers() == got
+    assert ts.observers(dev(torch, prim)) == got
+
+
 @pytest.mark.parametrize("case", ["fast_mode_fd", "torus_ideal_mixed"])
 def test_graph_captured_time_step_matches_eager(torch, case):
     """grim_time_step without stats enqueues a whole step with dt and time

@@ -7,7 +7,8 @@ and the library's message) instead of falling back to anything.
 GPU: the driver's steps equal, bit for bit, the same steps taken through the
 Python mirror of the interface (same library, same parameters), for the
 64x64 fast wave with grim's FD-FP64 solve and for a small EMHD torus with the
-escalating mixed solve.
+escalating mixed solve; the torus observers it prints after every step
+(--observe-every) match the Python interface's.
 """
 import os
 import subprocess
@@ -51,7 +52,8 @@ def test_driver_matches_python_interface(tmp_path, case):
         cfg = grid_config(N1=64, N2=64, reconstruction=capi.RECON_WENO5, riemannSolver=capi.RIEMANN_LLF)
         opts, steps = capi.newton_opts(), 5
     else:
-        args = ["--problem", "torus", "--N", "96", "48", "4", "--steps", "2", "--solver", "mixed"]
+        args = ["--problem", "torus", "--N", "96", "48", "4", "--steps", "2", "--solver", "mixed",
+                "--observe-every", "1e-9"]
         import bench
 
         p = bench.torus_params_emhd()
@@ -75,3 +77,13 @@ def test_driver_matches_python_interface(tmp_path, case):
     ts.close()
     assert got.shape == ref.shape
     assert np.array_equal(got, ref, equal_nan=True)
+    if case == "torus_mixed":
+        # the observers printed after the last step (grim's format, %e)
+        ts = TimeStepper(p, cfg, opts=opts)
+        ts.primOld = ref
+        obs = ts.observers()
+        ts.close()
+        line = [ln for ln in r.stdout.splitlines() if ln.startswith("Baryon Mass = ")][-1]
+        mass = float(line.split("=")[1].split(";")[0])
+        assert r.stdout.count("Global quantities at t = ") == steps
+        assert mass == pytest.approx(obs["baryon_mass"], rel=1e-5)

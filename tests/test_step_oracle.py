@@ -27,6 +27,9 @@ except the boundary identities and the EMHD linear-mode error table):
 * torus set-up and hooks: density maximum 1 and beta_min = 100 after the
   field normalisation, floor bounds and idempotence, equatorial symmetry,
   the fixPoles copy/flip identities and the inflowCheck bound.
+* torus observers (ObserversTorus.hpp): closed forms on uniform Minkowski
+  states, an independent vectorised numpy evaluation of every integrand on a
+  Kerr-Schild torus state, and the set-up's normalisation read back.
 * parity unpinned (no independent check beyond the restatement): the torus
   closure limiter's exact form (problem.cpp:7-55), the inflowCheck velocity
   rescaling, the flux filter near the poles, and the splitmix64 noise that
@@ -591,3 +594,61 @@ def test_torus_steps_run(emhd):
         st = sim.step()
     assert np.all(np.isfinite(sim.prim))
     assert st.dt > 0.005
+
+
+# ---------------------------------------------------------------------------
+# Torus observers (src/problem/torus/ObserversTorus.hpp)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("gamma_gt_2", [False, True])
+def test_observers_closed_form_uniform_minkowski(gamma_gt_2):
+    """A uniform state in Minkowski space: mass = rho gamma V, magnetic and
+    thermal energy b^2 gamma V and u gamma V with b^2 = (B^2 + (B.ut)^2) /
+    gamma^2, mdot through either X1 plane = N2 N3 dV rho ut^1, all of it
+    unbound ((1 + u Gamma / rho) u_0 = -(1 + u Gamma / rho) gamma < -1), and
+    relativistic exactly when gamma > 2."""
+    g = O.grid(N1=8, N2=6, N3=4, dim=3, X1Start=0., X1End=2., X2Start=0., X2End=1., X3Start=0., X3End=0.5)
+    p = O.default_params(False)
+    ut = np.array([0.4, -0.2, 0.1]) * (6. if gamma_gt_2 else 1.)
+    B = np.array([0.1, 0.2, -0.3])
+    rho, u = 0.7, 0.3
+    prim = np.zeros((p.dof, g.n))
+    prim[0], prim[1], prim[2:5] = rho, u, ut[:, None]
+    prim[5:8] = B[:, None]
+    got = O.torus_observers(g, p, O.Geometry(g), prim)
+    gam = np.sqrt(1. + ut @ ut)
+    assert (gam > 2.) == gamma_gt_2
+    V, dV = 1., 2. / 8 * 1. / 6 * 0.5 / 4
+    bsq = (B @ B + (B @ ut) ** 2) / gam ** 2 + p.bSqrFloorInFluidElement
+    mdot = 6 * 4 * dV * rho * ut[0]
+    want = dict(baryon_mass=rho * gam * V, magnetic_energy=bsq * gam * V, thermal_energy=u * gam * V,
+                mdot_in=mdot, mdot_out=mdot, mdot_out_unbound=mdot,
+                mdot_out_relativistic=mdot if gamma_gt_2 else 0., rho_max=rho,
+                beta_min=2. * ((p.adiabaticIndex - 1.) * u + 1e-13) / (bsq + 1e-18))
+    for k, v in want.items():
+        assert got[k] == pytest.approx(v, rel=1e-13, abs=1e-300), k
+
+
+@pytest.mark.parametrize("emhd", [False, True])
+def test_observers_match_independent_numpy_on_torus(emhd):
+    """The oracle's observers of a torus state (Kerr-Schild, cylindrified,
+    with inflow through the inner X1 plane) against a vectorised numpy
+    evaluation of every integrand from the primitive and metric planes."""
+    import observers_numpy as ON
+
+    g, p, gm, t = torus_setup(emhd, size=(48, 24, 4))
+    P = O.torus_ic(g, p, t, gm)
+    got = O.torus_observers(g, p, gm, P)
+    want, scale = ON.observers(g, p, gm.center, P)
+    assert want["mdot_in"] < 0. and want["baryon_mass"] > 0.
+    for k in ON.NAMES:
+        tol = 1e-12 * scale[k] if k in scale else 0.
+        assert abs(got[k] - want[k]) <= tol, (k, got[k], want[k])
+
+
+def test_observers_torus_normalisation():
+    """The torus set-up normalises the density maximum to 1 and the field to
+    beta_min = 100 (problem.cpp:343-504): the observers read them back."""
+    g, p, gm, t = torus_setup(False, size=(128, 64, 2), InitialPerturbationAmplitude=0.)
+    got = O.torus_observers(g, p, gm, O.torus_ic(g, p, t, gm))
+    assert got["rho_max"] == pytest.approx(1., rel=1e-12)
+    assert got["beta_min"] == pytest.approx(100., rel=1e-9)
