@@ -87,8 +87,26 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_io(verbose)
     build_driver(verbose)
     return LIB
+
+
+IO_LIB = os.path.join(HERE, "libgrim_io.so")
+
+
+def build_io(verbose: bool = False) -> str:
+    """grim_amd/libgrim_io.so: grim's HDF5 dumps and restart files
+    (include/grim_io.h), host code only; the HDF5 C library is loaded at run
+    time."""
+    tmp = IO_LIB + ".tmp"
+    cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-shared", "-fPIC",
+           "-I", os.path.join(ROOT, "include"), os.path.join(HERE, "host", "grim_io.cpp"), "-ldl", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
 
 
 DRIVER = os.path.join(HERE, "grim_step")
@@ -100,7 +118,7 @@ def build_driver(verbose: bool = False) -> str:
     tmp = DRIVER + ".tmp"
     cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-Wall", "-Wextra",
            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host"),
-           os.path.join(HERE, "host", "grim_main.cpp"), "-L", HERE, "-lgrim_hip",
+           os.path.join(HERE, "host", "grim_main.cpp"), "-L", HERE, "-lgrim_hip", "-lgrim_io",
            "-Wl,-rpath,$ORIGIN", "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)

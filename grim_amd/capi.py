@@ -252,6 +252,12 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise GrimError(GRIM_ERR_NO_DEVICE, f"loading {LIB_PATH} (not built: run __graft_entry__.build())")
+        # torch ships its own HIP runtime under the same sonames
+        # (libamdhip64.so.7, libhsa-runtime64.so.1): whichever loads first
+        # serves the whole process.  Load torch's first, so that torch and
+        # this library share one runtime whatever the import order.
+        import torch  # noqa: F401
+
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

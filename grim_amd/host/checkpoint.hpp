@@ -1,0 +1,101 @@
+// grim's HDF5 output and restart protocol on the C++ timeStepper interface
+// (timestepper.hpp), over include/grim_io.h (link with -lgrim_io).
+//
+//  dumpPrimitives   grid::dump of primOld as "primitives" (the torus's
+//                   WriteData dumps, torus/problem.cpp:778-840)
+//  loadPrimitives   grid::load into primOld (src/grid/grid.cpp:482-499)
+//  writeRestart     CheckWallClockTermination's restart dump and the two
+//                   text files naming it (torus/problem.cpp:880-906)
+//  restart          the timeStepper constructor's restart branch
+//                   (src/timestepper/timestepper.cpp:387-441)
+#pragma once
+
+#include <sys/stat.h>
+
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "grim_io.h"
+#include "timestepper.hpp"
+
+namespace grim {
+
+inline void checkIo(int rc, const char* what) {
+  if (rc != 0) throw std::runtime_error(std::string(what) + ": " + grim_io_last_error());
+}
+
+inline bool fileExists(const std::string& f) {
+  struct stat info;
+  return stat(f.c_str(), &info) == 0;
+}
+
+inline void dumpPrimitives(const timeStepper& ts, const std::string& file,
+                           const std::string& dataset = "primitives") {
+  const std::vector<double> prim = ts.getPrim();
+  checkIo(grim_io_dump(&ts.config(), ts.numVars, prim.data(), dataset.c_str(), file.c_str()), "grim_io_dump");
+}
+
+// The bulk from the file; ghost zones keep their values except the periodic
+// ones, which communicate refills (grid::load's DMGlobalToLocal).
+inline void loadPrimitives(timeStepper& ts, const std::string& file, const std::string& dataset = "primitives") {
+  std::vector<double> prim = ts.getPrim();
+  checkIo(grim_io_load(&ts.config(), ts.numVars, prim.data(), dataset.c_str(), file.c_str()), "grim_io_load");
+  ts.setPrim(prim);
+  ts.communicate();
+}
+
+// primVarsT%06d.h5, WriteIdx = floor(time / WriteDataEveryDt)
+inline std::string primVarsName(double time, double writeDataEveryDt) {
+  const std::string idx = std::to_string((long long)std::floor(time / writeDataEveryDt));
+  return "primVarsT" + std::string(idx.size() < 6 ? 6 - idx.size() : 0, '0') + idx + ".h5";
+}
+
+// restartVarsT%08d.h5 with index floor(100 time), then the file name and
+// the time (default stream precision, as grim writes it) in the two text
+// files.  Returns the dump's name.
+inline std::string writeRestart(const timeStepper& ts, const std::string& restartFileName = "restartFileName.txt",
+                                const std::string& restartFileTime = "restartFileTime.txt") {
+  const std::string idx = std::to_string((long long)std::floor(ts.time * 100));
+  const std::string file = "restartVarsT" + std::string(idx.size() < 8 ? 8 - idx.size() : 0, '0') + idx + ".h5";
+  dumpPrimitives(ts, file);
+  std::ofstream fName(restartFileName);
+  fName << file << std::endl;
+  std::ofstream fTime(restartFileTime);
+  fTime << ts.time << std::endl;
+  if (!fName || !fTime) throw std::runtime_error("writeRestart: cannot write " + restartFileName);
+  return file;
+}
+
+// When both text files exist: the time and the dump they name; otherwise,
+// when `force` (params::restart): restartFile.  Then the torus's floors
+// (fullStepDiagnostics "to reset the time step").  Returns whether a state
+// was loaded.
+inline bool restart(timeStepper& ts, bool force = false, const std::string& restartFile = "restartFile.h5",
+                    const std::string& restartFileName = "restartFileName.txt",
+                    const std::string& restartFileTime = "restartFileTime.txt") {
+  const bool haveFiles = fileExists(restartFileName) && fileExists(restartFileTime);
+  if (!force && !haveFiles) return false;
+  if (haveFiles) {
+    double restartTime = 0.;
+    std::ifstream fTime(restartFileTime);
+    fTime >> restartTime;
+    std::string file;
+    std::ifstream fName(restartFileName);
+    fName >> file;
+    if (!fTime || !fName) throw std::runtime_error("restart: cannot read " + restartFileName);
+    std::printf("\n Restarting from file %s at time %e\n\n", restartFileName.c_str(), restartTime);
+    loadPrimitives(ts, file);
+    ts.setTime(restartTime, ts.dt);
+  } else {
+    if (!fileExists(restartFile)) throw std::runtime_error("Restart file " + restartFile + " does not exist");
+    std::printf("\nFound restart file: %s\n\n", restartFile.c_str());
+    loadPrimitives(ts, restartFile);
+  }
+  if (ts.torus()) ts.applyFloor();
+  return true;
+}
+
+}  // namespace grim

@@ -5,7 +5,8 @@
 //
 //   grim_step [--problem fastmode|emhdmode|torus] [--N N1 N2 N3] [--steps K]
 //             [--solver fd|fd_lockstep|mixed] [--maxiter M] [--dump FILE]
-//             [--observe-every DT]
+//             [--observe-every DT] [--write-every DT] [--restart]
+//             [--max-wall-time SECONDS] [--dt0 DT]
 //
 // fastmode: 2D Minkowski ideal MHD, 64x64 periodic, WENO5 + LLF, the fast
 //           magnetosonic eigenmode at amplitude 1e-4, dt0 = 0.5/N1;
@@ -15,7 +16,15 @@
 // --dump writes primOld (numVars planes of N1T*N2T*N3T doubles, X1 fastest)
 // after the last step.  --observe-every DT prints the torus observers
 // whenever time crosses a multiple of DT (fullStepDiagnostics,
-// torus/problem.cpp:752-776).  Exit status 1 on any library error.
+// torus/problem.cpp:752-776).  --write-every DT writes primOld to
+// primVarsT%06d.h5 whenever time crosses a multiple of DT (WriteData,
+// :777-840, the primitives only).  A run restarts as grim's constructor does
+// (timestepper.cpp:387-441) when restartFileName.txt and restartFileTime.txt
+// exist in the working directory, or from restartFile.h5 with --restart;
+// --max-wall-time stops after the first step that ends past the limit and
+// writes the restart files (CheckWallClockTermination, :880-906).  --dt0
+// replaces the problem's initial dt (params::InitialDt).
+// Exit status 1 on any library error.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -24,6 +33,7 @@
 #include <string>
 #include <vector>
 
+#include "checkpoint.hpp"
 #include "timestepper.hpp"
 
 namespace {
@@ -36,6 +46,10 @@ struct Options {
   int maxIter = 10;
   std::string dump;
   double observeEvery = 0.;
+  double writeEvery = 0.;
+  bool restart = false;
+  double maxWallTime = -1.;
+  double dt0 = 0.;
 };
 
 Options parse(int argc, char** argv) {
@@ -60,6 +74,14 @@ Options parse(int argc, char** argv) {
       o.dump = next();
     } else if (a == "--observe-every") {
       o.observeEvery = std::atof(next());
+    } else if (a == "--write-every") {
+      o.writeEvery = std::atof(next());
+    } else if (a == "--restart") {
+      o.restart = true;
+    } else if (a == "--max-wall-time") {
+      o.maxWallTime = std::atof(next());
+    } else if (a == "--dt0") {
+      o.dt0 = std::atof(next());
     } else {
       throw std::runtime_error("unknown argument " + a);
     }
@@ -126,6 +148,9 @@ int run(const Options& o) {
     }
     dt0 = 0.5 / N1;
   }
+  if (o.dt0 > 0.) {
+    dt0 = o.dt0;
+  }
   const int numVars = 8 + prm.physics.conduction + prm.physics.viscosity;
   // grim's constructor order: ..., boundaryTop, boundaryBottom, boundaryFront, boundaryBack
   grim::timeStepper ts(N1, N2, N3, dim, numVars, 3, 0., dt0, bl, br, bt, bb, bf, bk, metric, a, 0.3, X1s,
@@ -151,8 +176,11 @@ int run(const Options& o) {
   } else {
     ts.initialConditionsLinearMode(o.problem == "emhdmode" ? 1 : 0, o.problem == "emhdmode" ? 1e-8 : 1e-4);
   }
+  grim::restart(ts, o.restart);
+  const auto start = std::chrono::steady_clock::now();
   const double zones = (double)N1 * N2 * N3;
   double total = 0.;
+  int done = 0;
   for (int n = 0; n < o.steps; n++) {
     int numReads = 0, numWrites = 0;
     const auto t0 = std::chrono::steady_clock::now();
@@ -174,9 +202,21 @@ int run(const Options& o) {
       grim::timeStepper::ComputeEnergyIntegrals(obs);
       grim::timeStepper::ComputeBoundaryFluxes(obs);
     }
+    // WriteData (:753, :777-840): the primitives only
+    if (o.writeEvery > 0. &&
+        std::floor(ts.time / o.writeEvery) != std::floor((ts.time - st.dt) / o.writeEvery))
+      grim::dumpPrimitives(ts, grim::primVarsName(ts.time, o.writeEvery));
+    done++;
+    // CheckWallClockTermination (torus/problem.cpp:880-906)
+    if (o.maxWallTime >= 0. &&
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count() >= o.maxWallTime) {
+      const std::string file = grim::writeRestart(ts);
+      std::printf("\n Termination reason: WallClock (restart file %s)\n", file.c_str());
+      break;
+    }
   }
-  std::printf("%d steps of %s (%s): %.4e zone cycles/sec\n", o.steps, o.problem.c_str(), o.solver.c_str(),
-              zones * o.steps / total);
+  std::printf("%d steps of %s (%s): %.4e zone cycles/sec\n", done, o.problem.c_str(), o.solver.c_str(),
+              zones * done / total);
   if (!o.dump.empty()) {
     const std::vector<double> prim = ts.getPrim();
     FILE* f = std::fopen(o.dump.c_str(), "wb");

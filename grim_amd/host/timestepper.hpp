@@ -56,6 +56,7 @@ class timeStepper {
   grim_ctx* ctx = nullptr;
   grim_step_stats stats{};
   grim_grid_config cfg{};
+  bool isTorus = false;
 
  public:
   double dt = 0., time = 0.;
@@ -118,6 +119,9 @@ class timeStepper {
 
   // Zones of one variable plane, ghosts included (N1T * N2T * N3T).
   int64_t zones() const { return grim_grid_zones(&cfg); }
+  const grim_grid_config& config() const { return cfg; }
+  // the torus problem's hooks are active (initialConditionsTorus)
+  bool torus() const { return isTorus; }
 
   // timeStepper::initialConditions of the two problems this path serves
   // (src/problem/linear_modes/problem.cpp:10-111, src/problem/torus/problem.cpp:145-540);
@@ -129,6 +133,7 @@ class timeStepper {
   }
   void initialConditionsTorus(const grim_torus_params& torus) {
     check(grim_init_torus(ctx, &torus), "grim_init_torus");
+    isTorus = true;
     setTime(time, dt);
   }
 
@@ -148,6 +153,11 @@ class timeStepper {
     check(grim_prim(ctx, &p), "grim_prim");
     return p;
   }
+
+  // grid::communicate on primOld (periodic ghost zones, one rank) and the
+  // torus's applyFloor on primOld.
+  void communicate() { check(grim_communicate(ctx, primOldDevice(), numVars), "grim_communicate"); }
+  void applyFloor() { check(grim_apply_floor(ctx, primOldDevice()), "grim_apply_floor"); }
 
   void setTime(double t, double d) {
     check(grim_set_time(ctx, t, d), "grim_set_time");

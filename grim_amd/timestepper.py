@@ -118,6 +118,24 @@ class TimeStepper:
         capi.check(capi.lib().grim_torus_bcs(self._ctx, capi._ptr(prim)), "grim_torus_bcs")
         return prim
 
+    def dump(self, file: str, dataset: str = "primitives"):
+        """grid::dump of primOld (grim's HDF5 layout, grim_amd/checkpoint.py)."""
+        from . import checkpoint
+
+        checkpoint.dump(self.grid, self.primOld, file, dataset)
+
+    def load(self, file: str, dataset: str = "primitives"):
+        """grid::load into primOld's bulk, then communicate (periodic ghosts)."""
+        import torch
+
+        from . import checkpoint
+
+        prim = checkpoint.load(self.grid, self.primOld, file, dataset)
+        dev = torch.from_numpy(prim).to(self.device)
+        self.communicate(dev)
+        torch.cuda.synchronize()
+        self.primOld = dev.cpu().numpy()
+
     def observers(self, prim=None) -> dict:
         """ComputeEnergyIntegrals, ComputeMinMaxVariables and
         ComputeBoundaryFluxes (ObserversTorus.hpp:6-251) of prim (a device

@@ -1,6 +1,7 @@
 """CPU checks of the C ABI: the HIP library loads and exports every function
-``include/grim_hip.h`` declares, and the Python binding covers them all.
-No compute entry point is called (there is no GPU here)."""
+``include/grim_hip.h`` declares, the host I/O library every function of
+``include/grim_io.h``, and the Python bindings cover them all.  No compute
+entry point is called (there is no GPU here)."""
 import ctypes
 import os
 import re
@@ -10,10 +11,10 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
+def declared_functions(header="grim_hip.h"):
     names = set()
     for fn in os.listdir(os.path.join(ROOT, "include")):
-        if not fn.endswith(".h"):
+        if fn != header:
             continue
         text = open(os.path.join(ROOT, "include", fn)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
@@ -49,6 +50,20 @@ def test_binding_covers_every_declared_symbol():
     from grim_amd import capi
 
     assert set(declared_functions()) == set(capi.SIGNATURES)
+
+
+def test_headers_are_the_two_libraries():
+    assert sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")) == \
+        ["grim_hip.h", "grim_io.h"]
+
+
+def test_io_library_exports_and_binding():
+    from grim_amd import checkpoint
+
+    io = checkpoint.lib()
+    names = declared_functions("grim_io.h")
+    assert names == ["grim_io_dump", "grim_io_last_error", "grim_io_load"]
+    assert all(hasattr(io, n) and getattr(io, n).argtypes is not None for n in names)
 
 
 def test_default_params_match_grim_linear_modes(lib):

@@ -87,3 +87,41 @@ def test_driver_matches_python_interface(tmp_path, case):
         mass = float(line.split("=")[1].split(";")[0])
         assert r.stdout.count("Global quantities at t = ") == steps
         assert mass == pytest.approx(obs["baryon_mass"], rel=1e-5)
+
+
+@pytest.mark.gpu
+def test_driver_restart_continues_the_run(tmp_path):
+    """CheckWallClockTermination + the constructor's restart branch: a run
+    stopped after its first step (restartVarsT*.h5, restartFileName.txt,
+    restartFileTime.txt written) and started again in the same directory
+    continues to the state of the uninterrupted run.  grim's restart keeps no
+    dt (the next computeDt is bounded by maxDtIncrement * InitialDt), so the
+    runs start from an InitialDt above the Courant limit: every step then
+    takes the Courant dt.  The restart file holds the bulk only, so ghost
+    zones (and through them computeDt) may differ in the last bits: the
+    comparison is to 1e-9 of each variable's scale."""
+    whole, part = tmp_path / "whole", tmp_path / "part"
+    whole.mkdir()
+    part.mkdir()
+    base = [DRIVER, "--problem", "fastmode", "--solver", "fd", "--dt0", "0.05"]
+    r = subprocess.run(base + ["--steps", "4", "--dump", "prim.bin"], cwd=whole, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run(base + ["--steps", "4", "--max-wall-time", "0"], cwd=part, capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0 and "Termination reason: WallClock" in r.stdout
+    name = (part / "restartFileName.txt").read_text().split()[0]
+    assert name.startswith("restartVarsT") and (part / name).exists()
+    assert float((part / "restartFileTime.txt").read_text()) > 0.
+    r = subprocess.run(base + ["--steps", "3", "--dump", "prim.bin"], cwd=part, capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0 and "Restarting from file" in r.stdout
+    g = 3
+    a = np.fromfile(whole / "prim.bin").reshape(8, 64 + 2 * g, 64 + 2 * g)[:, g:-g, g:-g]
+    b = np.fromfile(part / "prim.bin").reshape(8, 64 + 2 * g, 64 + 2 * g)[:, g:-g, g:-g]
+    scale = np.abs(a).reshape(8, -1).max(axis=1)[:, None, None] + 1e-300
+    diff = (np.abs(a - b) / scale).max()
+    print(f"restart vs uninterrupted: max scaled difference {diff:.2e}", flush=True)
+    assert diff < 1e-9
