@@ -522,6 +522,23 @@ def full_step_bench(args):
         wave[mode] = {"ms_per_step": ms, "zone_updates_per_s": 64 * 64 / (ms * 1e-3)}
         ts.close()
     res["linear_wave_64x64"] = wave
+
+    # grim's 1D EMHD shock tube (src/problem/shock_tests/, N1 = 512, MINMOD +
+    # LLF, conduction + viscosity, grim's three Newton iterations)
+    from grim_amd import problems
+
+    cfg = problems.shock_grid(512)
+    p = problems.shock_params()
+    shock = {"config": "1D Minkowski EMHD shock_tube_2, N1=512 (src/problem/shock_tests/params.cpp), FD-FP64 "
+                       "Newton, 20 steps after 2 warm-up from dt0 = 1e-5"}
+    for mode in ("eager", "graph"):
+        ts = TimeStepper(p, cfg, stream=torch.cuda.Stream())
+        ts.primOld = problems.shock_tube_prim(cfg, p, "shock_tube_2")
+        ts.set_time(0., 1e-5)
+        ms, _ = time_steps(ts, 20, 2, graph=(mode == "graph"))
+        shock[mode] = {"ms_per_step": ms, "zone_updates_per_s": 512 / (ms * 1e-3)}
+        ts.close()
+    res["shock_tube_1d_512"] = shock
     return res
 
 

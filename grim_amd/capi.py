@@ -23,6 +23,7 @@ GRIM_ERR_OOM = -4
 CLOSURE_LINEAR_MODES = 0
 CLOSURE_PER_ZONE = 1
 CLOSURE_TORUS = 2
+CLOSURE_SHOCK_TESTS = 3
 JAC_FD = 0
 JAC_ANALYTIC = 1
 LU_FP64 = 0

@@ -188,6 +188,11 @@ __device__ __forceinline__ void closure(const double* P, const Kinematics& k,
     c.tau = prm.tau0;
     c.chi = alphaC * cs * cs * c.tau;
     c.nu = alphaV * cs * cs * c.tau;
+  } else if (prm.closure == GRIM_CLOSURE_SHOCK_TESTS) {
+    // src/problem/shock_tests/problem.cpp:13-18
+    c.tau = 0.1;
+    c.chi = prm.condAlpha * cs * cs * c.tau;
+    c.nu = prm.viscAlpha * cs * cs * c.tau;
   } else {
     // pow(r, 1.5) and pow(c_s, 3) of problem.cpp:7-55 as r sqrt(r) and
     // c_s c_s c_s: equal to the last bit, reproducible on host and device,

@@ -1,0 +1,79 @@
+"""Initial states of grim's 1D EMHD shock tests (src/problem/shock_tests/),
+host side: the grid, the parameters and the left/right primitive states,
+set into a TimeStepper with ``primOld = shock_tube_prim(...)``.
+
+The tests run on grim's time step unchanged; the problem supplies only the
+closure (GRIM_CLOSURE_SHOCK_TESTS: tau = 0.1, chi = ConductionAlpha c_s^2 tau,
+nu = ViscosityAlpha c_s^2 tau, problem.cpp:13-18) and the initial state.
+stationary_shock_BVP_input reads solution files that the reference does not
+hold, so it is not offered.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import capi
+
+# (rho, pressure, u1, u2, u3, B1, B2, B3) left and right, finalTime
+# (shock_tests/problem.cpp:31-146).  The stationary shock's pressures are
+# multiples of (adiabaticIndex - 1): see shock_tube_prim.
+SHOCK_TESTS = {
+    "fast_shock": ((1., 1., 25., 0., 0., 20., 25.02, 0.), (25.48, 367.5, 1.091, 0.3923, 0., 20., 49., 0.), 2.5),
+    "slow_shock": ((1., 10., 1.53, 0., 0., 10., 18.28, 0.), (3.323, 55.36, 0.9571, -0.6822, 0., 10., 14.49, 0.),
+                   2.),
+    "switch_on_slow": ((1.78e-3, .1, -.765, -1.386, 0., 1., 1.022, 0.), (0.01, 1., 0., 0., 0., 1., 0., 0.), 2.),
+    "switch_off_fast": ((0.1, 1., -2., 0., 0., 2., 0., 0.), (0.562, 10., -0.212, -0.590, 0., 2., 4.71, 0.), 1.),
+    "alfven_wave": ((1., 1., 0., 0., 0., 3., 3., 0.), (1., 1., 3.7, 5.76, 0., 3., -6.857, 0.), 2.),
+    "shock_tube_1": ((1., 1000., 0., 0., 0., 1., 0., 0.), (.1, 1., 0., 0., 0., 1., 0., 0.), 1.),
+    "shock_tube_2": ((1., 30., 0., 0., 0., 0., 20., 0.), (.1, 1., 0., 0., 0., 0., 0., 0.), 1.),
+    "collision": ((1., 1., 5., 0., 0., 10., 10., 0.), (1., 1., -5., 0., 0., 10., -10., 0.), 1.22),
+    "stationary_shock": ((1., 1., 1., 0., 0., 0.00001, 0., 0.), (3.08312999, 4.94577705, 0.32434571, 0., 0.,
+                                                               0.00001, 0., 0.), None),
+}
+
+
+def shock_grid(N1: int = 512) -> capi.GridConfig:
+    """src/problem/shock_tests/params.cpp: 1D Minkowski X1 in [-0.5, 1.5],
+    outflow in X1, MINMOD (theta 2) + LLF, Courant 0.25."""
+    from .timestepper import grid_config
+
+    return grid_config(N1=N1, N2=1, N3=1, dim=1, X1Start=-.5, X1End=1.5, boundaryLeft=capi.BC_OUTFLOW,
+                       boundaryRight=capi.BC_OUTFLOW, reconstruction=capi.RECON_MINMOD,
+                       riemannSolver=capi.RIEMANN_LLF, slopeLimTheta=2., CourantFactor=0.25, maxDtIncrement=1.3)
+
+
+def shock_params(test: str = "shock_tube_2") -> capi.Params:
+    """grim's shock-test parameters (params.cpp:8-94): EMHD with high-order
+    terms, ConductionAlpha 5, ViscosityAlpha 3, Gamma 4/3, floors 1e-20,
+    3 Newton iterations of at most 3 line-search steps."""
+    p = capi.default_params(True)
+    p.conduction = p.viscosity = 1
+    p.highOrderTermsConduction = p.highOrderTermsViscosity = 1
+    p.ConductionAlpha, p.ViscosityAlpha = 5., 3.
+    p.adiabaticIndex = 4. / 3
+    p.rhoFloorInFluidElement = p.uFloorInFluidElement = 1e-20
+    p.bSqrFloorInFluidElement = p.temperatureFloorInFluidElement = 1e-20
+    p.closure = capi.CLOSURE_SHOCK_TESTS
+    p.maxNonLinearIter, p.maxLineSearchIters = 3, 3
+    p.nonlinearsolve_atol, p.JacobianAssembleEpsilon, p.linesearchfloor = 1e-20, 4e-8, 1e-24
+    return p
+
+
+def shock_tube_prim(grid: capi.GridConfig, p: capi.Params, test: str) -> np.ndarray:
+    """timeStepper::initialConditions (problem.cpp:20-163): the left state on
+    array indices [0, N1/2) of every X1 line (ghost zones included, the
+    inclusive af::seq(0, N1/2) overwritten at N1/2 by the right half), the
+    right state on [N1/2, N1T); u = pressure / (Gamma - 1); q~ = dP~ = 0."""
+    left, right, _ = SHOCK_TESTS[test]
+    gam = p.adiabaticIndex
+    prim = np.zeros((p.dof, *grid.shape))
+    b1 = p.dof - 3
+    for sl, st in ((slice(0, grid.N1 // 2), left), (slice(grid.N1 // 2, None), right)):
+        rho, pres, u1, u2, u3, B1, B2, B3 = st
+        if test == "stationary_shock":
+            pres = pres * (gam - 1.)  # pressureLeft = 1*(Gamma-1), pressureRight = 4.94...*(Gamma-1)
+        prim[0][..., sl] = rho
+        prim[1][..., sl] = pres / (gam - 1.)
+        prim[2][..., sl], prim[3][..., sl], prim[4][..., sl] = u1, u2, u3
+        prim[b1][..., sl], prim[b1 + 1][..., sl], prim[b1 + 2][..., sl] = B1, B2, B3
+    return prim.reshape(p.dof, -1)

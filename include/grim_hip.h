@@ -36,7 +36,9 @@ typedef enum grim_status {
 typedef enum grim_closure {
   GRIM_CLOSURE_LINEAR_MODES = 0, /* src/problem/linear_modes/problem.cpp:3-8 */
   GRIM_CLOSURE_PER_ZONE = 1,     /* same form, per-zone conduction/viscosity alpha */
-  GRIM_CLOSURE_TORUS = 2         /* src/problem/torus/problem.cpp:7-55 */
+  GRIM_CLOSURE_TORUS = 2,        /* src/problem/torus/problem.cpp:7-55 */
+  GRIM_CLOSURE_SHOCK_TESTS = 3   /* src/problem/shock_tests/problem.cpp:13-18: tau = 0.1,
+                                    chi (nu) = ConductionAlpha (ViscosityAlpha) c_s^2 tau */
 } grim_closure;
 
 /* The part of grim's `namespace params` (src/params.hpp:96-210) that the

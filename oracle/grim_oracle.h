@@ -22,6 +22,7 @@
 #define GRIM_ORACLE_CLOSURE_LINEAR_MODES 0
 #define GRIM_ORACLE_CLOSURE_PER_ZONE 1
 #define GRIM_ORACLE_CLOSURE_TORUS 2
+#define GRIM_ORACLE_CLOSURE_SHOCK_TESTS 3
 
 #define GRIM_ORACLE_NEWTON_PER_ZONE 0
 #define GRIM_ORACLE_NEWTON_GLOBAL 1

@@ -100,6 +100,11 @@ struct FluidElement {
       tau = T(p.tau0);
       chi_emhd = zc.conductionAlpha * soundSpeed * soundSpeed * tau;
       nu_emhd = zc.viscosityAlpha * soundSpeed * soundSpeed * tau;
+    } else if (p.closure == GRIM_ORACLE_CLOSURE_SHOCK_TESTS) {
+      // src/problem/shock_tests/problem.cpp:13-18
+      tau = T(0.1);
+      chi_emhd = p.ConductionAlpha * soundSpeed * soundSpeed * tau;
+      nu_emhd = p.ViscosityAlpha * soundSpeed * soundSpeed * tau;
     } else {
       // torus closure, src/problem/torus/problem.cpp:7-55.  pow(r, 1.5) and
       // pow(c_s, 3) are evaluated as r sqrt(r) and c_s c_s c_s (equal up to

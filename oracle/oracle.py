@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(HERE, "libgrim_oracle.so")
 CLOSURE_LINEAR_MODES = 0
 CLOSURE_PER_ZONE = 1
 CLOSURE_TORUS = 2
+CLOSURE_SHOCK_TESTS = 3
 NEWTON_PER_ZONE = 0
 NEWTON_GLOBAL = 1
 MAX_VARS = 10

@@ -256,6 +256,28 @@ def test_fast_mode_steps_bitwise(torch, semantics):
     run_pair(ts, g, po, O.Geometry(g), 10, semantics=semantics)
 
 
+@pytest.mark.parametrize("test", ["shock_tube_2", "collision", "fast_shock", "alfven_wave"])
+def test_shock_tube_steps_bitwise(torch, test):
+    """grim's 1D EMHD shock tests (src/problem/shock_tests/: MINMOD + LLF,
+    outflow, conduction and viscosity with the shock closure, grim's three
+    Newton iterations): 1D grids, outflow ghosts and the closure bit for bit,
+    unconverged zones included."""
+    from grim_amd import problems as PR
+    from grim_amd.timestepper import TimeStepper
+
+    cfg = PR.shock_grid(128)
+    p = PR.shock_params()
+    po = O.default_params(True)
+    for name, _ in O.Params._fields_:
+        setattr(po, name, getattr(p, name))
+    ts = TimeStepper(p, cfg)
+    ts.primOld = PR.shock_tube_prim(cfg, p, test)
+    g = O.grid(**{f: getattr(cfg, f) for f, _ in cfg._fields_})
+    st = run_pair(ts, g, po, O.Geometry(g), 8, dt0=1e-5)
+    ts.close()
+    print(f"{test}: unconverged after the last step {list(st.unconverged)}")
+
+
 def test_emhd_mode_steps_bitwise(torch):
     """The reference's EMHD linear mode (MINMOD + HLL, grim's default solver settings)."""
     ts, g, p, po = make(dict(N1=32, N2=32), emhd=True)
