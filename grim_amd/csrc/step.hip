@@ -351,11 +351,12 @@ __global__ __launch_bounds__(kBlock) void emfKernel(Dims D, Box box, const doubl
 
 // Flux of variable v through the lower face of zone z along d, after
 // timeStepper::fluxCT (constrainedtransport.cpp:3-47) for dim >= 2.
-template <class L>
+template <class L, int DIM = 0>
 __device__ __forceinline__ double ctFlux(const Dims& D, const double* __restrict__ F,
                                          const double* __restrict__ emf, int d, int v, int64_t z) {
+  const int dim = DIM ? DIM : D.dim;  // DIM 0: the grid's dimension read at run time
   const int64_t n = D.n;
-  if (D.dim >= 2 && v >= L::B1) {
+  if (dim >= 2 && v >= L::B1) {
     const double* e1 = emf;
     const double* e2 = emf + n;
     const double* e3 = emf + 2 * n;
@@ -364,11 +365,11 @@ __device__ __forceinline__ double ctFlux(const Dims& D, const double* __restrict
     if (d == 0) {
       if (b == 0) return 0.;
       if (b == 1) return 0.5 * (e3[z] + e3[z + s2]);
-      if (D.dim == 3) return -0.5 * (e2[z] + e2[z + s3]);
+      if (dim == 3) return -0.5 * (e2[z] + e2[z + s3]);
     } else if (d == 1) {
       if (b == 0) return -0.5 * (e3[z] + e3[z + s1]);
       if (b == 1) return 0.;
-      if (D.dim == 3) return 0.5 * (e1[z] + e1[z + s3]);
+      if (dim == 3) return 0.5 * (e1[z] + e1[z + s3]);
     } else {
       if (b == 0) return 0.5 * (e2[z] + e2[z + s1]);
       if (b == 1) return -0.5 * (e1[z] + e1[z + s2]);
@@ -380,31 +381,32 @@ __device__ __forceinline__ double ctFlux(const Dims& D, const double* __restrict
 
 // Divergence of the fluxes of zone z, variable v (fvmfluxes.cpp:36-46,
 // 91-103, 164-180): dim < 3 is af::convolve with {1, -1, 0}/dX,
-// dim 3 is (shift(F, -1) - F)/dX.
-template <class L>
+// dim 3 is (shift(F, -1) - F)/dX.  With the dimension and the variable known
+// at compile time every branch folds away and the flux filter becomes
+// selects, so the loads of all faces issue together.
+template <class L, int DIM>
 __device__ __forceinline__ double divergence(const Dims& D, const double* dX,
                                              const double* __restrict__ F,
                                              const double* __restrict__ emf, int v, int64_t z,
                                              int i, int j, int filter) {
   double acc = 0.;
-  for (int d = 0; d < D.dim; d++) {
-    double f0 = ctFlux<L>(D, F, emf, d, v, z);
-    double f1 = ctFlux<L>(D, F, emf, d, v, z + D.s[d]);
-    if (filter) {
-      // The torus's flux filter (problem.cpp:1243-1285), after flux-CT: no
-      // mass inflow through the radial boundary faces, nothing through the
-      // polar faces.
-      if (d == 0 && v == RHO) {
-        if (i == D.g[0]) f0 = stdmin(f0, 0.);
-        if (i + 1 == D.N[0] + D.g[0]) f1 = stdmax(f1, 0.);
-      }
-      if (d == 1) {
-        if (j == D.g[1]) f0 = 0.;
-        if (j + 1 == D.N[1] + D.g[1]) f1 = 0.;
-      }
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    double f0 = ctFlux<L, DIM>(D, F, emf, d, v, z);
+    double f1 = ctFlux<L, DIM>(D, F, emf, d, v, z + D.s[d]);
+    // The torus's flux filter (problem.cpp:1243-1285), after flux-CT: no
+    // mass inflow through the radial boundary faces, nothing through the
+    // polar faces.
+    if (d == 0 && v == RHO) {
+      f0 = (filter && i == D.g[0]) ? stdmin(f0, 0.) : f0;
+      f1 = (filter && i + 1 == D.N[0] + D.g[0]) ? stdmax(f1, 0.) : f1;
+    }
+    if (d == 1) {
+      f0 = (filter && j == D.g[1]) ? 0. : f0;
+      f1 = (filter && j + 1 == D.N[1] + D.g[1]) ? 0. : f1;
     }
     double term;
-    if (D.dim < 3) {
+    if constexpr (DIM < 3) {
       const double w = 1. / dX[d];
       term = f1 * w + f0 * (-w);
     } else {
@@ -415,10 +417,12 @@ __device__ __forceinline__ double divergence(const Dims& D, const double* dX,
   return acc;
 }
 
-// Bulk zones: divF (optional output), and the stage fold
+// Bulk zones of a DIM-dimensional grid.  FOLD: the stage fold
 //   rhs = consOld - dt'((divF + S_expl) + 1/2 S_implOld)      (fluid rows)
 //   B   = (consOld_B - dt' divF_B) / g                        (timestep.cpp:80-89)
-template <class L>
+// and, when mask is given, the residual scale 1/max|rhs|; otherwise divF
+// into divOut.
+template <class L, int DIM, bool FOLD>
 __global__ __launch_bounds__(kBlock) void foldKernel(StepArgs a, Box box, const double* __restrict__ F,
                                                      const double* __restrict__ emf,
                                                      const double* __restrict__ geomC,
@@ -436,25 +440,48 @@ __global__ __launch_bounds__(kBlock) void foldKernel(StepArgs a, Box box, const 
   box.at(t, i, j, k);
   const Dims& D = a.D;
   const int64_t n = D.n, z = flatIndex(D, i, j, k);
-  const double dtPrime = rhs ? *dtPrimeDev : 0.;
+  const double dtPrime = FOLD ? *dtPrimeDev : 0.;
   double rmax = 0.;
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) {
-    const double div = divergence<L>(D, a.dX, F, emf, v, z, i, j, a.fluxFilter);
-    if (divOut) divOut[v * n + z] = div;
-    if (!rhs) continue;
-    const double U = consOld[v * n + z];
-    if (v < L::NF) {
-      const double si = L::EMHD ? sImp[v * n + z] : 0.;
-      const double r = U - dtPrime * ((div + sExp[v * n + z]) + 0.5 * si);
-      rhs[v * n + z] = r;
-      rmax = stdmax(rmax, fabs(r));
+    const double div = divergence<L, DIM>(D, a.dX, F, emf, v, z, i, j, a.fluxFilter);
+    if constexpr (!FOLD) {
+      divOut[v * n + z] = div;
     } else {
-      const double consB = U - dtPrime * div;
-      primNew[v * n + z] = consB / geomC[GEOM_G * n + z];
+      const double U = consOld[v * n + z];
+      if (v < L::NF) {
+        const double si = L::EMHD ? sImp[v * n + z] : 0.;
+        const double r = U - dtPrime * ((div + sExp[v * n + z]) + 0.5 * si);
+        rhs[v * n + z] = r;
+        rmax = stdmax(rmax, fabs(r));
+      } else {
+        const double consB = U - dtPrime * div;
+        primNew[v * n + z] = consB / geomC[GEOM_G * n + z];
+      }
     }
   }
-  if (rhs && mask) mask[z] = 1. / rmax;
+  if (FOLD && mask) mask[z] = 1. / rmax;
+}
+
+// foldKernel for the grid's dimension.
+template <class L, bool FOLD>
+void launchFold(const StepArgs& a, const Box& b, hipStream_t s, const double* F, const double* emf,
+                const double* geomC, const double* consOld, const double* sExp, const double* sImp,
+                const double* dtPrimeDev, double* divOut, double* rhs, double* primNew, double* mask) {
+  const unsigned nb = blocksFor(b.count());
+  switch (a.D.dim) {
+    case 1:
+      foldKernel<L, 1, FOLD><<<nb, kBlock, 0, s>>>(a, b, F, emf, geomC, consOld, sExp, sImp, dtPrimeDev, divOut,
+                                                   rhs, primNew, mask);
+      break;
+    case 2:
+      foldKernel<L, 2, FOLD><<<nb, kBlock, 0, s>>>(a, b, F, emf, geomC, consOld, sExp, sImp, dtPrimeDev, divOut,
+                                                   rhs, primNew, mask);
+      break;
+    default:
+      foldKernel<L, 3, FOLD><<<nb, kBlock, 0, s>>>(a, b, F, emf, geomC, consOld, sExp, sImp, dtPrimeDev, divOut,
+                                                   rhs, primNew, mask);
+  }
 }
 
 // Post-CT flux arrays for the faces of the bulk zones (API output only).
@@ -1572,9 +1599,8 @@ int stage(grim_ctx* x, int st, grim_step_stats* stats) {
   if (int rc = fluxesAndEmf<L>(x, primStage, false)) return rc;
   GRIM_HIP_TRY(hipMemcpyAsync(out, primStage, bytes, hipMemcpyDeviceToDevice, x->s));
   const Box b = bulkBox(D);
-  foldKernel<L><<<blocksFor(b.count()), kBlock, 0, x->s>>>(x->a, b, x->fluxes, x->emf, x->geomC, x->consOld,
-                                                           x->sExp, x->sImp, dtPrime, nullptr, x->rhs,
-                                                           out, x->c.residualScaling ? x->mask : nullptr);
+  launchFold<L, true>(x->a, b, x->s, x->fluxes, x->emf, x->geomC, x->consOld, x->sExp, x->sImp, dtPrime, nullptr,
+                      x->rhs, out, x->c.residualScaling ? x->mask : nullptr);
   if (int rc = checkLaunch("fold")) return rc;
 
   grim_zone_batch zb = {};
@@ -1908,9 +1934,8 @@ int grim_div_fluxes(grim_ctx* x, const double* prim, double* div_f, double* flux
   return withLayout(x, [&]<class L>() -> int {
     if (int rc = fluxesAndEmf<L>(x, prim, fluxes != nullptr)) return rc;
     const Box b = bulkBox(x->D);
-    foldKernel<L><<<blocksFor(b.count()), kBlock, 0, x->s>>>(x->a, b, x->fluxes, x->emf, x->geomC, nullptr,
-                                                             nullptr, nullptr, nullptr, div_f, nullptr, nullptr,
-                                                             nullptr);
+    launchFold<L, false>(x->a, b, x->s, x->fluxes, x->emf, x->geomC, nullptr, nullptr, nullptr, nullptr, div_f,
+                         nullptr, nullptr, nullptr);
     if (fluxes) {
       GRIM_HIP_TRY(hipMemcpyAsync(fluxes, x->fluxes, sizeof(double) * (size_t)L::DOF * x->D.dim * x->D.n,
                                   hipMemcpyDeviceToDevice, x->s));
