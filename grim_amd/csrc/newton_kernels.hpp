@@ -102,12 +102,54 @@ __device__ __forceinline__ void fdJacobianFast(const double* P, const double* R,
                                                double dt, const ZoneScale& zs, const EmhdWeights& ew,
                                                double (&A)[L::NF][L::NF]);
 
+// Small batches: grim's FD solve with one zone per kWide lanes of a
+// wavefront.  Lane r + 1 of a zone's group evaluates the residual with P_r
+// perturbed (grim's rule), so the NF columns of the FD Jacobian come from
+// one residual evaluation's latency instead of NF; every other step runs on
+// all lanes of the group with identical operands.  The result is the
+// one-lane solve's bit for bit.
+constexpr int kWide = 8;
+constexpr int64_t kWideMaxZones = 16384;  // below this a one-lane grid leaves most CUs idle
+
+template <class L>
+__device__ __forceinline__ void fdJacobianWide(const double* P, const double* R, const ZoneGeom& G,
+                                               const ZoneConst<L>& c, const DevParams& prm,
+                                               double dt, double (&A)[L::NF][L::NF]) {
+  static_assert(L::NF < kWide, "one lane per FD column");
+  const int lane = threadIdx.x & 63;
+  const int sub = lane % kWide;
+  const int first = lane - sub;
+  // lanes 1..NF take columns 0..NF-1; the others repeat column 0 (unused)
+  const int row = (sub >= 1 && sub <= L::NF) ? sub - 1 : 0;
+  const double eps = prm.jacEps;
+  double Pp[L::DOF];
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) Pp[v] = P[v];
+  double dP = 0.;
+#pragma unroll
+  for (int v = 0; v < L::NF; v++) {
+    const bool small = fabs(P[v]) < .5 * eps;
+    const double pv = small ? P[v] + eps : P[v] + eps * P[v];
+    Pp[v] = (v == row) ? pv : P[v];
+    dP = (v == row) ? pv - P[v] : dP;
+  }
+  double Rp[L::NF];
+  residual<L>(Pp, G, c, prm, dt, Rp);
+  double colv[L::NF];
+#pragma unroll
+  for (int k = 0; k < L::NF; k++) colv[k] = (Rp[k] - R[k]) / dP;
+#pragma unroll
+  for (int r = 0; r < L::NF; r++)
+#pragma unroll
+    for (int k = 0; k < L::NF; k++) A[k][r] = __shfl(colv[k], first + 1 + r, 64);
+}
+
 // One Newton update after the convergence test (solve.cpp:80-198): Jacobian,
 // LU solve of J d = -R, quadratic backtracking line search, P += lambda d.
 // Returns true when the last evaluated trial was accepted; Rt then holds the
 // residual at the new P (bit-identical to a fresh evaluation, since the new P
 // is computed by the same expression as the trial).
-template <class L, int MODE>
+template <class L, int MODE, bool WIDE = false>
 __device__ __forceinline__ bool newtonUpdate(double* P, const double* R, double l2,
                                              const ZoneGeom& G, const ZoneConst<L>& c,
                                              const DevParams& prm, double dt, const ZoneScale& zs,
@@ -115,7 +157,10 @@ __device__ __forceinline__ bool newtonUpdate(double* P, const double* R, double 
   double d[L::NF];
   if constexpr (MODE == MODE_FD) {
     double A[L::NF][L::NF];
-    fdJacobian<L>(P, R, G, c, prm, dt, A);
+    if constexpr (WIDE)
+      fdJacobianWide<L>(P, R, G, c, prm, dt, A);
+    else
+      fdJacobian<L>(P, R, G, c, prm, dt, A);
     evals += L::NF;
 #pragma unroll
     for (int v = 0; v < L::NF; v++) d[v] = -R[v];
@@ -195,15 +240,18 @@ __device__ __forceinline__ bool newtonUpdate(double* P, const double* R, double 
 // added to the earlier phase's when `accumulate`.  Returns true when the
 // zone ran out of budget unconverged (a later phase may continue it: every
 // iteration is a function of P alone, so stopping and resuming changes no
-// result).
-template <class L, int MODE>
+// result).  WIDE: the zone's kWide lanes run it together; only the first
+// writes.
+template <class L, int MODE, bool WIDE = false>
 __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_batch& b,
                                           const grim_newton_stats& st, int64_t z, int budget,
                                           bool accumulate, bool lastPhase) {
   const int64_t n = b.n;
+  const bool lead = !WIDE || (threadIdx.x % kWide) == 0;
   if (b.residual_mask && b.residual_mask[z] == 0.) {
     // outside the residual mask (ghost zones of a grid) R = 0: the zone
     // takes its floors and stops at the first test, as in the full path
+    if (!lead) return false;
     b.prim[RHO * n + z] = smax(b.prim[RHO * n + z], prm.rhoFloor);
     b.prim[UU * n + z] = smax(b.prim[UU * n + z], prm.uFloor);
     if (st.iterations) st.iterations[z] = accumulate ? st.iterations[z] : 0;
@@ -251,7 +299,7 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
       break;
     }
     double Rt[L::NF];
-    haveR = newtonUpdate<L, MODE>(P, R, l2, G, c, prm, dt, zs, ew, Rt, evals);
+    haveR = newtonUpdate<L, MODE, WIDE>(P, R, l2, G, c, prm, dt, zs, ew, Rt, evals);
     iters++;
     if (haveR) {
 #pragma unroll
@@ -272,6 +320,7 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
       converged = true;
     }
   }
+  if (!lead) return !converged && !lastPhase;
 #pragma unroll
   for (int v = 0; v < L::NF; v++) b.prim[v * n + z] = P[v];
   if (st.iterations) st.iterations[z] = (accumulate ? st.iterations[z] : 0) + iters;
@@ -292,6 +341,16 @@ __global__ __launch_bounds__(kDefaultBlock) __attribute__((amdgpu_waves_per_eu(2
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= b.n) return;
   solveZone<L, MODE>(prm, b, st, z, prm.maxIter, false, true);
+}
+
+// The one-pass FD solve of a small batch, kWide lanes per zone (every lane
+// of a group shares z, so groups leave together).
+template <class L>
+__global__ __launch_bounds__(kDefaultBlock) void newtonPerZoneWide(DevParams prm, grim_zone_batch b,
+                                                                  grim_newton_stats st) {
+  const int64_t z = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWide;
+  if (z >= b.n) return;
+  solveZone<L, MODE_FD, true>(prm, b, st, z, prm.maxIter, false, true);
 }
 
 // Compacting solve, phase 1: at most `budget1` updates per zone.  Each
@@ -482,6 +541,13 @@ void launchSolve(const DevParams& prm, const grim_zone_batch& b, const grim_newt
                  const LaunchCfg& cfg, hipStream_t s) {
   const unsigned grid = (unsigned)((b.n + cfg.block - 1) / cfg.block);
   if (!cfg.compaction || cfg.budget1 >= prm.maxIter) {
+    if constexpr (MODE == MODE_FD) {
+      if (b.n <= kWideMaxZones) {
+        const unsigned wideGrid = (unsigned)((b.n * kWide + kDefaultBlock - 1) / kDefaultBlock);
+        newtonPerZoneWide<L><<<wideGrid, kDefaultBlock, 0, s>>>(prm, b, st);
+        return;
+      }
+    }
     newtonPerZone<L, MODE><<<grid, cfg.block, 0, s>>>(prm, b, st);
     return;
   }

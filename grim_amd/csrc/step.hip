@@ -183,7 +183,15 @@ __global__ __launch_bounds__(kBlock) void communicateKernel(Dims D, BcTypes bc, 
   }
   if (!ghost) return;
   const int64_t w = flatIndex(D, I[0], I[1], I[2]);
-  for (int v = 0; v < nvars; v++) prim[v * D.n + z] = outside ? 0. : prim[v * D.n + w];
+  // every load first, then every store (w is a bulk zone: nothing here
+  // writes it), so the loads of all variables are in flight together
+  double val[GRIM_MAX_VARS];
+#pragma unroll
+  for (int v = 0; v < GRIM_MAX_VARS; v++)
+    if (v < nvars) val[v] = outside ? 0. : prim[v * D.n + w];
+#pragma unroll
+  for (int v = 0; v < GRIM_MAX_VARS; v++)
+    if (v < nvars) prim[v * D.n + z] = val[v];
 }
 
 // One ghost slab of boundaries::applyBoundaryConditions (boundary.cpp:32-239):
@@ -201,7 +209,14 @@ __global__ __launch_bounds__(kBlock) void boundaryKernel(Dims D, Box box, int d,
   else
     I[d] = side == 0 ? g : N + g - 1;
   const int64_t w = flatIndex(D, I[0], I[1], I[2]);
-  for (int v = 0; v < nvars; v++) prim[v * D.n + z] = prim[v * D.n + w];
+  // w lies inside along d, z in the ghost slab: loads first, then stores
+  double val[GRIM_MAX_VARS];
+#pragma unroll
+  for (int v = 0; v < GRIM_MAX_VARS; v++)
+    if (v < nvars) val[v] = prim[v * D.n + w];
+#pragma unroll
+  for (int v = 0; v < GRIM_MAX_VARS; v++)
+    if (v < nvars) prim[v * D.n + z] = val[v];
 }
 
 // ---------------------------------------------------------------------------
@@ -712,8 +727,15 @@ __global__ __launch_bounds__(kBlock) void dtKernel(StepArgs a, const double* __r
   if (threadIdx.x == 0) blockMax[blockIdx.x] = red[0];
 }
 
+// The step's clock lives in device memory so that a whole step can be
+// enqueued (or captured in a HIP graph) without a host round trip:
+// clock[0] = time, clock[1] = dt, clock[2] = dt/2 (the half stage's dt').
+enum { CLOCK_TIME = 0, CLOCK_DT = 1, CLOCK_HALF_DT = 2, CLOCK_WORDS = 4 };
+
 __global__ __launch_bounds__(kBlock) void maxReduceKernel(int64_t count, const double* __restrict__ in,
-                                                          double* __restrict__ out) {
+                                                          double* __restrict__ out,
+                                                          double* __restrict__ clock = nullptr,
+                                                          double courant = 0., double maxIncrement = 0.) {
   __shared__ double red[kBlock];
   double m = -INFINITY;
   for (int64_t t = threadIdx.x; t < count; t += kBlock) m = stdmax(m, in[t]);
@@ -723,23 +745,17 @@ __global__ __launch_bounds__(kBlock) void maxReduceKernel(int64_t count, const d
     if ((int)threadIdx.x < w) red[threadIdx.x] = stdmax(red[threadIdx.x], red[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = red[0];
-}
-
-// The step's clock lives in device memory so that a whole step can be
-// enqueued (or captured in a HIP graph) without a host round trip:
-// clock[0] = time, clock[1] = dt, clock[2] = dt/2 (the half stage's dt').
-enum { CLOCK_TIME = 0, CLOCK_DT = 1, CLOCK_HALF_DT = 2, CLOCK_WORDS = 4 };
-
-// The end of timeStepper::computeDt (timestep.cpp:393-399) from the reduced
-// maximum of the inverse time step.
-__global__ void dtFinishKernel(const double* __restrict__ maxInvDt, double courant, double maxIncrement,
-                               double* __restrict__ clock) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double newDt = courant / maxInvDt[0];
-  if (newDt > maxIncrement * clock[CLOCK_DT]) newDt = maxIncrement * clock[CLOCK_DT];
-  clock[CLOCK_DT] = newDt;
-  clock[CLOCK_HALF_DT] = newDt / 2.;
+  if (threadIdx.x == 0) {
+    out[0] = red[0];
+    // with a clock: the end of timeStepper::computeDt (timestep.cpp:393-399)
+    // from the maximum just reduced
+    if (clock) {
+      double newDt = courant / red[0];
+      if (newDt > maxIncrement * clock[CLOCK_DT]) newDt = maxIncrement * clock[CLOCK_DT];
+      clock[CLOCK_DT] = newDt;
+      clock[CLOCK_HALF_DT] = newDt / 2.;
+    }
+  }
 }
 
 __global__ void setClockKernel(double* __restrict__ clock, double time, double dt) {
@@ -749,16 +765,13 @@ __global__ void setClockKernel(double* __restrict__ clock, double time, double d
   clock[CLOCK_HALF_DT] = dt / 2.;
 }
 
-__global__ void advanceTimeKernel(double* __restrict__ clock) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  clock[CLOCK_TIME] += clock[CLOCK_DT];
-}
-
 // dst1 = dst2 = src over `count` doubles: the end of the step writes the new
 // state into primOld and into the state computeDt reads next, reading it once.
 __global__ __launch_bounds__(kBlock) void copy2Kernel(int64_t count, const double* __restrict__ src,
-                                                      double* __restrict__ dst1, double* __restrict__ dst2) {
+                                                      double* __restrict__ dst1, double* __restrict__ dst2,
+                                                      double* __restrict__ clock) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) clock[CLOCK_TIME] += clock[CLOCK_DT];  // the step's end: time += dt
   const int64_t pairs = count / 2;
   if (t < pairs) {
     const double2 v = reinterpret_cast<const double2*>(src)[t];
@@ -1497,12 +1510,14 @@ int explicitSources(grim_ctx* x, const double* prim, const double* primOld, doub
   return checkLaunch("sources");
 }
 
-// max over zones of the inverse time step, into x->red[nb] (device).
+// max over zones of the inverse time step, into x->red[nb] (device); with
+// clock, also the new dt into the device clock.
 template <class L>
-const double* reduceInvDt(grim_ctx* x, const double* prim) {
+const double* reduceInvDt(grim_ctx* x, const double* prim, double* clock = nullptr) {
   const unsigned nb = blocksFor(x->D.n);
   dtKernel<L><<<nb, kBlock, 0, x->s>>>(x->a, prim, x->geomC, x->red);
-  maxReduceKernel<<<1, kBlock, 0, x->s>>>(nb, x->red, x->red + nb);
+  maxReduceKernel<<<1, kBlock, 0, x->s>>>(nb, x->red, x->red + nb, clock, x->c.CourantFactor,
+                                          x->c.maxDtIncrement);
   return x->red + nb;
 }
 
@@ -1523,8 +1538,7 @@ int computeDt(grim_ctx* x, const double* prim, double dt, double* out) {
 // The same on the device clock: clock[dt] becomes the new dt, no host read.
 template <class L>
 int computeDtOnDevice(grim_ctx* x, const double* prim) {
-  const double* maxInv = reduceInvDt<L>(x, prim);
-  dtFinishKernel<<<1, 64, 0, x->s>>>(maxInv, x->c.CourantFactor, x->c.maxDtIncrement, x->clock);
+  reduceInvDt<L>(x, prim, x->clock);
   return checkLaunch("computeDt");
 }
 
@@ -1872,8 +1886,8 @@ int grim_time_step(grim_ctx* x, grim_step_stats* stats) {
     // primOld = the full stage's result, and elemOld of the next computeDt
     // (the problem's fullStepDiagnostics set it from this step's primOld).
     const int64_t count = (int64_t)L::DOF * x->D.n;
-    copy2Kernel<<<blocksFor(count / 2 + 1), kBlock, 0, x->s>>>(count, x->primNew, x->primOld, x->primDt);
-    advanceTimeKernel<<<1, 64, 0, x->s>>>(x->clock);
+    copy2Kernel<<<blocksFor(count / 2 + 1), kBlock, 0, x->s>>>(count, x->primNew, x->primOld, x->primDt,
+                                                               x->clock);
     if (int rc = checkLaunch("grim_time_step")) return rc;
     if (stats) {
       GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf + 4, x->clock, 2 * sizeof(double), hipMemcpyDeviceToHost, x->s));
