@@ -539,6 +539,20 @@ def full_step_bench(args):
         shock[mode] = {"ms_per_step": ms, "zone_updates_per_s": 512 / (ms * 1e-3)}
         ts.close()
     res["shock_tube_1d_512"] = shock
+
+    # grim's Orszag-Tang vortex (src/problem/orzag_tang/params.cpp: 256^2,
+    # MINMOD + LLF, ideal MHD, 5 Newton iterations)
+    cfg = problems.orszag_tang_grid(256)
+    p = problems.orszag_tang_params()
+    ot = {"config": "2D Minkowski Orszag-Tang vortex 256x256 (src/problem/orzag_tang/params.cpp), FD-FP64 Newton, "
+                    "20 steps after 2 warm-up from dt0 = 0.002"}
+    ts = TimeStepper(p, cfg, stream=torch.cuda.Stream())
+    ts.primOld = problems.orszag_tang_prim(cfg, p)
+    ts.set_time(0., 0.002)
+    ms, _ = time_steps(ts, 20, 2)
+    ot["eager"] = {"ms_per_step": ms, "zone_updates_per_s": 256 * 256 / (ms * 1e-3)}
+    ts.close()
+    res["orszag_tang_256"] = ot
     return res
 
 

@@ -1,12 +1,13 @@
-"""Initial states of grim's 1D EMHD shock tests (src/problem/shock_tests/),
-host side: the grid, the parameters and the left/right primitive states,
-set into a TimeStepper with ``primOld = shock_tube_prim(...)``.
+"""Initial states of further grim problems on the same time step, host side:
+the 1D EMHD shock tests (src/problem/shock_tests/) and the Orszag-Tang
+vortex (src/problem/orzag_tang/).  Each gives the grid, the parameters and
+the primitive state, set into a TimeStepper with ``primOld = ...``.
 
-The tests run on grim's time step unchanged; the problem supplies only the
-closure (GRIM_CLOSURE_SHOCK_TESTS: tau = 0.1, chi = ConductionAlpha c_s^2 tau,
-nu = ViscosityAlpha c_s^2 tau, problem.cpp:13-18) and the initial state.
-stationary_shock_BVP_input reads solution files that the reference does not
-hold, so it is not offered.
+Both run on grim's time step unchanged.  The shock tests supply a closure
+(GRIM_CLOSURE_SHOCK_TESTS: tau = 0.1, chi = ConductionAlpha c_s^2 tau,
+nu = ViscosityAlpha c_s^2 tau, shock_tests/problem.cpp:13-18) and the
+initial state; stationary_shock_BVP_input reads solution files that the
+reference does not hold, so it is not offered.  The vortex is ideal MHD.
 """
 from __future__ import annotations
 
@@ -76,4 +77,52 @@ def shock_tube_prim(grid: capi.GridConfig, p: capi.Params, test: str) -> np.ndar
         prim[1][..., sl] = pres / (gam - 1.)
         prim[2][..., sl], prim[3][..., sl], prim[4][..., sl] = u1, u2, u3
         prim[b1][..., sl], prim[b1 + 1][..., sl], prim[b1 + 2][..., sl] = B1, B2, B3
+    return prim.reshape(p.dof, -1)
+
+
+# ---------------------------------------------------------------------------
+# Orszag-Tang vortex (src/problem/orzag_tang/)
+# ---------------------------------------------------------------------------
+def orszag_tang_grid(N: int = 256) -> capi.GridConfig:
+    """orzag_tang/params.cpp: 2D Minkowski [0, 1]^2, periodic, MINMOD (theta
+    2) + LLF, Courant 0.9."""
+    from .timestepper import grid_config
+
+    return grid_config(N1=N, N2=N, reconstruction=capi.RECON_MINMOD, riemannSolver=capi.RIEMANN_LLF,
+                       slopeLimTheta=2., CourantFactor=0.9, maxDtIncrement=1.3)
+
+
+def orszag_tang_params() -> capi.Params:
+    """orzag_tang/params.cpp: ideal MHD, Gamma 5/3, floors 1e-20, 5 Newton
+    iterations of at most 3 line-search steps to sum R^2 <= 1e-12."""
+    p = capi.default_params(False)
+    p.adiabaticIndex = 5. / 3
+    p.rhoFloorInFluidElement = p.uFloorInFluidElement = 1e-20
+    p.bSqrFloorInFluidElement = p.temperatureFloorInFluidElement = 1e-20
+    p.maxNonLinearIter, p.maxLineSearchIters = 5, 3
+    p.nonlinearsolve_atol, p.JacobianAssembleEpsilon, p.linesearchfloor = 1e-12, 4e-8, 1e-24
+    return p
+
+
+def orszag_tang_prim(grid: capi.GridConfig, p: capi.Params) -> np.ndarray:
+    """timeStepper::initialConditions (orzag_tang/problem.cpp:3-34) on every
+    zone, ghosts included: rho = 25/(36 pi), u = 5/(12 pi (Gamma-1)),
+    v = 0.5 (-sin 2 pi X2, sin 2 pi X1), u~ = gamma v,
+    B = (-sin 2 pi X2, sin 4 pi X1) / sqrt(4 pi), at the zone centres
+    X = Start + (i + 1/2) dX of the bulk index i (grid.cpp:200-203)."""
+    n3, n2, n1 = grid.shape
+    g = grid.numGhost
+    X1 = grid.X1Start + (np.arange(n1) - g + 0.5) * ((grid.X1End - grid.X1Start) / grid.N1)
+    X2 = grid.X2Start + (np.arange(n2) - (g if grid.dim > 1 else 0) + 0.5) * ((grid.X2End - grid.X2Start) / grid.N2)
+    X1, X2 = np.broadcast_to(X1[None, None, :], grid.shape), np.broadcast_to(X2[None, :, None], grid.shape)
+    prim = np.zeros((p.dof, *grid.shape))
+    prim[0] = 25. / (36. * np.pi)
+    prim[1] = 5. / (12. * np.pi * (p.adiabaticIndex - 1.))
+    v1 = -0.5 * np.sin(2 * np.pi * X2)
+    v2 = 0.5 * np.sin(2 * np.pi * X1)
+    gamma = 1. / np.sqrt(1. - v1 * v1 - v2 * v2 - 0.)
+    prim[2], prim[3] = gamma * v1, gamma * v2
+    b1 = p.dof - 3
+    prim[b1] = -1. / np.sqrt(4 * np.pi) * np.sin(2 * np.pi * X2)
+    prim[b1 + 1] = 1. / np.sqrt(4 * np.pi) * np.sin(4 * np.pi * X1)
     return prim.reshape(p.dof, -1)

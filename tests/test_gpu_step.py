@@ -278,6 +278,24 @@ def test_shock_tube_steps_bitwise(torch, test):
     print(f"{test}: unconverged after the last step {list(st.unconverged)}")
 
 
+def test_orszag_tang_steps_bitwise(torch):
+    """The Orszag-Tang vortex with grim's parameters (src/problem/orzag_tang/:
+    MINMOD + LLF, Gamma 5/3, 5 Newton iterations to sum R^2 <= 1e-12), 64^2."""
+    from grim_amd import problems as PR
+    from grim_amd.timestepper import TimeStepper
+
+    cfg = PR.orszag_tang_grid(64)
+    p = PR.orszag_tang_params()
+    po = O.default_params(False)
+    for name, _ in O.Params._fields_:
+        setattr(po, name, getattr(p, name))
+    ts = TimeStepper(p, cfg)
+    ts.primOld = PR.orszag_tang_prim(cfg, p)
+    g = O.grid(**{f: getattr(cfg, f) for f, _ in cfg._fields_})
+    run_pair(ts, g, po, O.Geometry(g), 6, dt0=0.002)
+    ts.close()
+
+
 def test_emhd_mode_steps_bitwise(torch):
     """The reference's EMHD linear mode (MINMOD + HLL, grim's default solver settings)."""
     ts, g, p, po = make(dict(N1=32, N2=32), emhd=True)
