@@ -51,6 +51,15 @@ __device__ __forceinline__ int64_t flatIndex(const Dims& D, int i, int j, int k)
   return i + (int64_t)D.NT[0] * (j + (int64_t)D.NT[1] * k);
 }
 
+// (i, j, k) of zone z, in 32-bit arithmetic (grids hold fewer than 2^31 zones)
+__device__ __forceinline__ void zoneIndex(const Dims& D, int64_t z, int& i, int& j, int& k) {
+  const unsigned u = (unsigned)z, n0 = (unsigned)D.NT[0], n1 = (unsigned)D.NT[1];
+  i = (int)(u % n0);
+  const unsigned r = u / n0;
+  j = (int)(r % n1);
+  k = (int)(r / n1);
+}
+
 inline unsigned blocksFor(int64_t count) { return (unsigned)((count + kBlock - 1) / kBlock); }
 
 // ---------------------------------------------------------------------------
@@ -74,8 +83,8 @@ __global__ __launch_bounds__(kBlock) void geometryKernel(Dims D, CoordMap cm, Co
                                                          double* __restrict__ out) {
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= D.n) return;
-  const int i = (int)(z % D.NT[0]), j = (int)((z / D.NT[0]) % D.NT[1]),
-            k = (int)(z / ((int64_t)D.NT[0] * D.NT[1]));
+  int i, j, k;
+  zoneIndex(D, z, i, j, k);
   double X[3], x[3], G[4][4], C[4][4], det;
   XCoordsAt(D, cg, i, j, k, loc, X);
   cm.toX(X, x);
@@ -99,8 +108,8 @@ __global__ __launch_bounds__(kBlock) void connectionKernel(Dims D, CoordMap cm, 
                                                            double* __restrict__ gamma) {
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= D.n) return;
-  const int i = (int)(z % D.NT[0]), j = (int)((z / D.NT[0]) % D.NT[1]),
-            k = (int)(z / ((int64_t)D.NT[0] * D.NT[1]));
+  int i, j, k;
+  zoneIndex(D, z, i, j, k);
   const double EPS = 1.e-5;
   double X[3], G[4][4], C[4][4], det;
   XCoordsAt(D, cg, i, j, k, 0, X);
@@ -142,8 +151,8 @@ __global__ __launch_bounds__(kBlock) void connectionKernel(Dims D, CoordMap cm, 
 __global__ __launch_bounds__(kBlock) void maskKernel(Dims D, double* __restrict__ mask) {
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= D.n) return;
-  const int i = (int)(z % D.NT[0]), j = (int)((z / D.NT[0]) % D.NT[1]),
-            k = (int)(z / ((int64_t)D.NT[0] * D.NT[1]));
+  int i, j, k;
+  zoneIndex(D, z, i, j, k);
   const bool bulk = i >= D.g[0] && i < D.g[0] + D.N[0] && j >= D.g[1] && j < D.g[1] + D.N[1] &&
                     k >= D.g[2] && k < D.g[2] + D.N[2];
   mask[z] = bulk ? 1. : 0.;
@@ -165,8 +174,8 @@ __global__ __launch_bounds__(kBlock) void communicateKernel(Dims D, BcTypes bc, 
                                                             double* __restrict__ prim) {
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= D.n) return;
-  int I[3] = {(int)(z % D.NT[0]), (int)((z / D.NT[0]) % D.NT[1]),
-              (int)(z / ((int64_t)D.NT[0] * D.NT[1]))};
+  int I[3];
+  zoneIndex(D, z, I[0], I[1], I[2]);
   bool ghost = false, outside = false;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -852,8 +861,8 @@ __global__ __launch_bounds__(kBlock) void linearModeKernel(StepArgs a, int mode,
   const Dims& D = a.D;
   const int64_t n = D.n;
   if (z >= n) return;
-  const int i = (int)(z % D.NT[0]), j = (int)((z / D.NT[0]) % D.NT[1]),
-            k = (int)(z / ((int64_t)D.NT[0] * D.NT[1]));
+  int i, j, k;
+  zoneIndex(D, z, i, j, k);
   double X[3];
   XCoordsAt(D, cg, i, j, k, 0, X);  // Minkowski: xCoords = XCoords
   double P[L::DOF];
@@ -1685,6 +1694,7 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
   GRIM_REQUIRE(out, "grim_create: null output");
   *out = nullptr;
   GRIM_REQUIRE(validConfig(p, c), "grim_create: bad params or grid config");
+  GRIM_REQUIRE(grim_grid_zones(c) < (int64_t(1) << 31), "grim_create: grids hold fewer than 2^31 zones");
   if (int rc = checkDevice()) return rc;
   grim_ctx* x = new (std::nothrow) grim_ctx();
   GRIM_REQUIRE(x, "grim_create: host allocation failed");

@@ -54,10 +54,14 @@ struct Box {
   __host__ __device__ int64_t count() const {
     return (int64_t)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
   }
+  // 32-bit index arithmetic: grim_create admits grids of fewer than 2^31
+  // zones, so every box and index fits (64-bit division is a long software
+  // sequence on the GPU, 32-bit a few instructions)
   __device__ __forceinline__ void at(int64_t t, int& i, int& j, int& k) const {
-    const int64_t c0 = hi[0] - lo[0], c1 = hi[1] - lo[1];
-    i = lo[0] + (int)(t % c0);
-    const int64_t r = t / c0;
+    const unsigned c0 = (unsigned)(hi[0] - lo[0]), c1 = (unsigned)(hi[1] - lo[1]);
+    const unsigned u = (unsigned)t;
+    i = lo[0] + (int)(u % c0);
+    const unsigned r = u / c0;
     j = lo[1] + (int)(r % c1);
     k = lo[2] + (int)(r / c1);
   }
