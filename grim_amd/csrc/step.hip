@@ -1371,6 +1371,7 @@ struct grim_ctx {
   int torus;
   grim_torus_params tp;
   double* floors;  // 2 planes: rho and u floor values
+  double* dump;    // dump variables staged for a host copy (first such call)
 };
 
 namespace grim {
@@ -1577,6 +1578,85 @@ int observers(grim_ctx* x, const double* prim) {
   observersKernel<L><<<kObsBlocks, kBlock, 0, x->s>>>(x->a, b, volElem, prim, x->geomC, x->obs);
   obsFinishKernel<<<1, kBlock, 0, x->s>>>(x->obs, x->obs + (size_t)kObsBlocks * OBS_WORDS);
   return checkLaunch("observers");
+}
+
+// fullStepDiagnostics' WriteData variables (torus/problem.cpp:808-875) of
+// the bulk zones, X1 fastest, as planes out[q * bulk + t]: rho, u, v^i =
+// u^i/u^t and B^i = b^i u^t - b^t u^i in the x coordinates, [q], [deltaP],
+// b^2, the Lorentz factor u^t alpha_x (params.cpp:136-147), then the three
+// xCoords of the zone centre.  dxdX is computeTransformationMatrices
+// (geometry.cpp:441-508): central differences of the coordinate map with
+// GAMMA_EPS = 1e-5, dxdX[0][0] = 1 and the other time entries 0.
+template <class L>
+__global__ __launch_bounds__(kBlock) void dumpVarsKernel(StepArgs a, Box box, CoordMap cm, CoordGrid cg,
+                                                         const double* __restrict__ prim,
+                                                         const double* __restrict__ geomC,
+                                                         double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nb = box.count();
+  if (t >= nb) return;
+  const int64_t n = a.D.n;
+  int i, j, k;
+  box.at(t, i, j, k);
+  const int64_t z = flatIndex(a.D, i, j, k);
+  double X[3], x[3];
+  XCoordsAt(a.D, cg, i, j, k, 0, X);
+  cm.toX(X, x);
+  const double eps = 1.e-5;
+  double J[3][3];
+  for (int b = 0; b < 3; b++) {
+    double Xp[3] = {X[0], X[1], X[2]}, Xm[3] = {X[0], X[1], X[2]}, xp[3], xm[3];
+    Xp[b] += eps;
+    Xm[b] -= eps;
+    cm.toX(Xp, xp);
+    cm.toX(Xm, xm);
+#pragma unroll
+    for (int r = 0; r < 3; r++) J[r][b] = (xp[r] - xm[r]) / (2. * eps);
+  }
+  double P[L::DOF];
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
+  ZoneGeom G;
+  loadGeom(geomC, n, z, G);
+  Kinematics e;
+  kinematics<L>(P, G, a.prm, e);
+  Closure c;
+  closure<L>(P, e, a.prm, 1., 1., geomC[(int64_t)LOC_R * n + z], c);
+  double ux[4], bx[4];
+  ux[0] = e.uCon[0];
+  bx[0] = e.bCon[0];
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    ux[r + 1] = 0.;
+    bx[r + 1] = 0.;
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      ux[r + 1] += J[r][b] * e.uCon[b + 1];
+      bx[r + 1] += J[r][b] * e.bCon[b + 1];
+    }
+  }
+  const double alphax = 1. / sqrt(-G.gcon0[0]);
+  double* o = out + t;
+  int q = 0;
+  o[(q++) * nb] = e.rho;
+  o[(q++) * nb] = e.u;
+#pragma unroll
+  for (int r = 1; r <= 3; r++) o[(q++) * nb] = ux[r] / ux[0];
+  if constexpr (L::COND_ON) o[(q++) * nb] = c.q;
+  if constexpr (L::VISC_ON) o[(q++) * nb] = c.deltaP;
+#pragma unroll
+  for (int r = 1; r <= 3; r++) o[(q++) * nb] = bx[r] * ux[0] - bx[0] * ux[r];
+  o[(q++) * nb] = e.bSqr;
+  o[(q++) * nb] = ux[0] * alphax;
+#pragma unroll
+  for (int d = 0; d < 3; d++) o[(q++) * nb] = x[d];
+}
+
+template <class L>
+int dumpVars(grim_ctx* x, const double* prim, double* out) {
+  const Box b = bulkBox(x->D);
+  dumpVarsKernel<L><<<blocksFor(b.count()), kBlock, 0, x->s>>>(x->a, b, x->cm, x->cg, prim, x->geomC, out);
+  return checkLaunch("dump variables");
 }
 
 // setProblemSpecificBCs of the torus: inflowCheck on both radial ghost
@@ -1823,6 +1903,7 @@ int grim_destroy(grim_ctx* x) {
     if (q) (void)hipFree(q);
   if (x->hostBuf) (void)hipHostFree(x->hostBuf);
   if (x->floors) (void)hipFree(x->floors);
+  if (x->dump) (void)hipFree(x->dump);
   delete x;
   return GRIM_OK;
 }
@@ -2049,6 +2130,19 @@ int grim_torus_observers(grim_ctx* x, const double* prim, grim_torus_observables
   out->mdot_out_relativistic = r[OBS_MDOT_REL];
   out->rho_max = r[OBS_RHO_MAX];
   out->beta_min = r[OBS_BETA_MIN];
+  return GRIM_OK;
+}
+
+int grim_dump_vars(grim_ctx* x, const double* prim, double* out, int on_device) {
+  GRIM_REQUIRE(x && out, "grim_dump_vars: null argument");
+  if (!prim) prim = x->primOld;
+  const size_t bytes = sizeof(double) * (size_t)(x->dof + 5) * (size_t)bulkBox(x->D).count();
+  if (!on_device && !x->dump) GRIM_HIP_TRY(hipMalloc(&x->dump, bytes));
+  double* dst = on_device ? out : x->dump;
+  if (int rc = withLayout(x, [&]<class L>() -> int { return dumpVars<L>(x, prim, dst); })) return rc;
+  if (on_device) return GRIM_OK;
+  GRIM_HIP_TRY(hipMemcpyAsync(out, x->dump, bytes, hipMemcpyDeviceToHost, x->s));
+  GRIM_HIP_TRY(hipStreamSynchronize(x->s));
   return GRIM_OK;
 }
 

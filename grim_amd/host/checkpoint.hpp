@@ -3,6 +3,8 @@
 //
 //  dumpPrimitives   grid::dump of primOld as "primitives" (the torus's
 //                   WriteData dumps, torus/problem.cpp:778-840)
+//  dumpVTS          the dump variables of primOld as dump%06d.vts
+//                   (torus/problem.cpp:808-876)
 //  loadPrimitives   grid::load into primOld (src/grid/grid.cpp:482-499)
 //  writeRestart     CheckWallClockTermination's restart dump and the two
 //                   text files naming it (torus/problem.cpp:880-906)
@@ -51,6 +53,25 @@ inline void loadPrimitives(timeStepper& ts, const std::string& file, const std::
 inline std::string primVarsName(double time, double writeDataEveryDt) {
   const std::string idx = std::to_string((long long)std::floor(time / writeDataEveryDt));
   return "primVarsT" + std::string(idx.size() < 6 ? 6 - idx.size() : 0, '0') + idx + ".h5";
+}
+
+// dump%06d.vts, WriteIdx = floor(time / WriteDataEveryDt)
+inline std::string dumpVTSName(double time, double writeDataEveryDt) {
+  const std::string idx = std::to_string((long long)std::floor(time / writeDataEveryDt));
+  return "dump" + std::string(idx.size() < 6 ? 6 - idx.size() : 0, '0') + idx + ".vts";
+}
+
+// The variable names of torus/problem.cpp:808-825 in the plane order of
+// grim_dump_vars (params.cpp:136-147), then the VTS file.
+inline void dumpVTS(const timeStepper& ts, const std::string& file) {
+  std::vector<const char*> names = {"rho", "u", "u1", "u2", "u3"};
+  if (ts.physics().conduction) names.push_back("q");
+  if (ts.physics().viscosity) names.push_back("dP");
+  for (const char* n : {"B1", "B2", "B3", "bSqr", "gamma"}) names.push_back(n);
+  if ((int)names.size() != ts.numVars + 2) throw std::runtime_error("dumpVTS: variable count");
+  const std::vector<double> vars = ts.dumpVars();
+  checkIo(grim_io_dump_vts(&ts.config(), (int)names.size(), names.data(), vars.data(), file.c_str()),
+          "grim_io_dump_vts");
 }
 
 // restartVarsT%08d.h5 with index floor(100 time), then the file name and

@@ -12,6 +12,7 @@
 #include <dlfcn.h>
 
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <string>
 #include <vector>
@@ -203,6 +204,56 @@ int grim_io_load(const grim_grid_config* c, int dof, double* prim, const char* d
         const int64_t z = i + (int64_t)S.NT[0] * (j + (int64_t)S.NT[1] * k);
         for (int v = 0; v < dof; v++) prim[(int64_t)v * S.n + z] = buf[o++];
       }
+  return 0;
+}
+
+int grim_io_dump_vts(const grim_grid_config* c, int nvars, const char* const* names, const double* vars,
+                     const char* file) {
+  if (!validGrid(c, nvars + 3) || !names || !vars || !file) return fail("grim_io_dump_vts: bad argument");
+  for (int v = 0; v < nvars; v++)
+    if (!names[v] || !names[v][0] || std::string(names[v]).find_first_of("\"<>&") != std::string::npos)
+      return fail("grim_io_dump_vts: variable " + std::to_string(v) + " has no usable name");
+  const int n1 = c->N1, n2 = c->N2, n3 = c->N3;
+  const uint64_t nb = (uint64_t)n1 * n2 * n3;
+  const uint64_t bytesVar = nb * sizeof(double), bytesPts = 3 * bytesVar;
+  std::string x = "<?xml version=\"1.0\"?>\n"
+                  "<VTKFile type=\"StructuredGrid\" version=\"1.0\" byte_order=\"LittleEndian\" "
+                  "header_type=\"UInt64\">\n";
+  const std::string extent =
+      "0 " + std::to_string(n1 - 1) + " 0 " + std::to_string(n2 - 1) + " 0 " + std::to_string(n3 - 1);
+  x += "  <StructuredGrid WholeExtent=\"" + extent + "\">\n    <Piece Extent=\"" + extent + "\">\n";
+  x += "      <Points>\n        <DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"appended\" "
+       "offset=\"0\"/>\n      </Points>\n      <PointData>\n";
+  uint64_t off = sizeof(uint64_t) + bytesPts;
+  for (int v = 0; v < nvars; v++) {
+    x += "        <DataArray type=\"Float64\" Name=\"" + std::string(names[v]) +
+         "\" NumberOfComponents=\"1\" format=\"appended\" offset=\"" + std::to_string(off) + "\"/>\n";
+    off += sizeof(uint64_t) + bytesVar;
+  }
+  x += "      </PointData>\n    </Piece>\n  </StructuredGrid>\n  <AppendedData encoding=\"raw\">\n   _";
+  FILE* f = std::fopen(file, "wb");
+  if (!f) return fail(std::string("grim_io_dump_vts: cannot create ") + file);
+  bool ok = std::fwrite(x.data(), 1, x.size(), f) == x.size();
+  // points (x1, x2, x3) interleaved; the x planes follow the nvars planes
+  const double* xc = vars + (size_t)nvars * nb;
+  ok = ok && std::fwrite(&bytesPts, sizeof bytesPts, 1, f) == 1;
+  std::vector<double> pts;
+  const uint64_t chunk = 1 << 16;
+  for (uint64_t t0 = 0; ok && t0 < nb; t0 += chunk) {
+    const uint64_t m = nb - t0 < chunk ? nb - t0 : chunk;
+    pts.resize(3 * m);
+    for (uint64_t t = 0; t < m; t++)
+      for (int d = 0; d < 3; d++) pts[3 * t + d] = xc[d * nb + t0 + t];
+    ok = std::fwrite(pts.data(), sizeof(double), 3 * m, f) == 3 * m;
+  }
+  for (int v = 0; ok && v < nvars; v++) {
+    ok = std::fwrite(&bytesVar, sizeof bytesVar, 1, f) == 1 &&
+         std::fwrite(vars + (size_t)v * nb, sizeof(double), nb, f) == nb;
+  }
+  static const char kEnd[] = "\n  </AppendedData>\n</VTKFile>\n";
+  ok = ok && std::fwrite(kEnd, 1, sizeof kEnd - 1, f) == sizeof kEnd - 1;
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(std::string("grim_io_dump_vts: writing ") + file);
   return 0;
 }
 

@@ -17,8 +17,9 @@
 // after the last step.  --observe-every DT prints the torus observers
 // whenever time crosses a multiple of DT (fullStepDiagnostics,
 // torus/problem.cpp:752-776).  --write-every DT writes primOld to
-// primVarsT%06d.h5 whenever time crosses a multiple of DT (WriteData,
-// :777-840, the primitives only).  A run restarts as grim's constructor does
+// primVarsT%06d.h5 and the dump variables to dump%06d.vts whenever time
+// crosses a multiple of DT (WriteData, :777-876; the metric files of the
+// first write are not produced).  A run restarts as grim's constructor does
 // (timestepper.cpp:387-441) when restartFileName.txt and restartFileTime.txt
 // exist in the working directory, or from restartFile.h5 with --restart;
 // --max-wall-time stops after the first step that ends past the limit and
@@ -202,10 +203,12 @@ int run(const Options& o) {
       grim::timeStepper::ComputeEnergyIntegrals(obs);
       grim::timeStepper::ComputeBoundaryFluxes(obs);
     }
-    // WriteData (:753, :777-840): the primitives only
+    // WriteData (:753, :777-876): the primitives and the dump variables
     if (o.writeEvery > 0. &&
-        std::floor(ts.time / o.writeEvery) != std::floor((ts.time - st.dt) / o.writeEvery))
+        std::floor(ts.time / o.writeEvery) != std::floor((ts.time - st.dt) / o.writeEvery)) {
       grim::dumpPrimitives(ts, grim::primVarsName(ts.time, o.writeEvery));
+      grim::dumpVTS(ts, grim::dumpVTSName(ts.time, o.writeEvery));
+    }
     done++;
     // CheckWallClockTermination (torus/problem.cpp:880-906)
     if (o.maxWallTime >= 0. &&

@@ -56,6 +56,7 @@ class timeStepper {
   grim_ctx* ctx = nullptr;
   grim_step_stats stats{};
   grim_grid_config cfg{};
+  grim_params phys{};
   bool isTorus = false;
 
  public:
@@ -81,6 +82,7 @@ class timeStepper {
         boundaryBottom(boundaryBottom), boundaryFront(boundaryFront), boundaryBack(boundaryBack) {
     if (numVars != 8 + params.physics.conduction + params.physics.viscosity)
       throw std::runtime_error("timeStepper: numVars does not match the conduction/viscosity switches");
+    phys = params.physics;
     cfg.N1 = N1;
     cfg.N2 = N2;
     cfg.N3 = N3;
@@ -120,6 +122,7 @@ class timeStepper {
   // Zones of one variable plane, ghosts included (N1T * N2T * N3T).
   int64_t zones() const { return grim_grid_zones(&cfg); }
   const grim_grid_config& config() const { return cfg; }
+  const grim_params& physics() const { return phys; }
   // the torus problem's hooks are active (initialConditionsTorus)
   bool torus() const { return isTorus; }
 
@@ -145,6 +148,15 @@ class timeStepper {
   std::vector<double> getPrim() const {
     std::vector<double> out((size_t)numVars * (size_t)zones());
     check(grim_get_prim(ctx, out.data(), 0), "grim_get_prim");
+    return out;
+  }
+  // fullStepDiagnostics' dump variables of primOld (grim_dump_vars): numVars
+  // + 5 planes of the N1*N2*N3 bulk zones on the host, the last three the
+  // x coordinates of the zone centres.
+  std::vector<double> dumpVars() const {
+    const size_t nb = (size_t)cfg.N1 * cfg.N2 * cfg.N3;
+    std::vector<double> out((size_t)(numVars + 5) * nb);
+    check(grim_dump_vars(ctx, nullptr, out.data(), 0), "grim_dump_vars");
     return out;
   }
   // Device address of primOld (numVars planes), for on-device analysis.

@@ -145,6 +145,18 @@ class TimeStepper:
                                                    C.byref(out)), "grim_torus_observers")
         return out.as_dict()
 
+    def dump_vars(self, prim=None):
+        """fullStepDiagnostics' dump variables (torus/problem.cpp:808-875) of
+        prim (None: primOld) as a device tensor (dof + 5, N1*N2*N3), X1
+        fastest: rho, u, u1..3, [q], [dP], B1..3, bSqr, gamma, x1..3."""
+        import torch
+
+        nb = self.grid.N1 * self.grid.N2 * self.grid.N3
+        out = torch.empty((self.params.dof + 5, nb), dtype=torch.float64, device=self.device)
+        capi.check(capi.lib().grim_dump_vars(self._ctx, None if prim is None else capi._ptr(prim), capi._ptr(out), 1),
+                   "grim_dump_vars")
+        return out
+
     # -- the step -----------------------------------------------------------
     def timeStep(self, stats: bool = True) -> capi.StepStats | None:
         """One step.  stats=False enqueues it without a host synchronisation

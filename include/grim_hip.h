@@ -384,6 +384,20 @@ typedef struct grim_torus_observables {
  * the context's stream. */
 int grim_torus_observers(grim_ctx* ctx, const double* prim, grim_torus_observables* out);
 
+/* Replaces the dumpVars fill of fullStepDiagnostics (torus/problem.cpp:
+ * 808-875, layout params.cpp:136-147).  out: device buffer of (dof + 5)
+ * planes of grim_grid_bulk zones each (N1*N2*N3, X1 fastest, no ghosts):
+ * rho, u, v^1..3 = u^i/u^t and B^1..3 = b^i u^t - b^t u^i in the physical
+ * x coordinates, q (conduction on), deltaP (viscosity on) placed between
+ * v^3 and B^1, b^2, the Lorentz factor; then x^1..3 of the zone centres (the
+ * points of the VTS grid).  dx/dX by central differences with step 1e-5
+ * (geometry.cpp:441-508).  prim: device planes, or NULL for primOld.
+ * on_device != 0: out is device memory and the call is asynchronous on the
+ * context's stream; 0: out is host memory, staged through a device buffer
+ * that the context allocates on the first such call, and the call
+ * synchronises the stream. */
+int grim_dump_vars(grim_ctx* ctx, const double* prim, double* out, int on_device);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,16 @@ int grim_io_dump(const grim_grid_config* c, int dof, const double* prim, const c
 int grim_io_load(const grim_grid_config* c, int dof, double* prim, const char* dataset,
                  const char* file);
 
+/* grid::dumpVTS (grid.cpp:383-480) of the dump variables: a VTK XML
+ * StructuredGrid over the N1 x N2 x N3 bulk zone centres, written by this
+ * library (no PETSc, no HDF5): vars holds nvars planes of N1*N2*N3 doubles
+ * (X1 fastest) followed by the three planes x1, x2, x3 of the points — the
+ * layout grim_dump_vars produces, with nvars = dof + 2.  Each plane becomes
+ * a Float64 point-data array named names[v]; raw appended data with UInt64
+ * size headers, little-endian. */
+int grim_io_dump_vts(const grim_grid_config* c, int nvars, const char* const* names, const double* vars,
+                     const char* file);
+
 const char* grim_io_last_error(void);
 
 #ifdef __cplusplus

@@ -293,6 +293,13 @@ void grim_oracle_torus_observers(const grim_oracle_grid* c, const grim_oracle_pa
                                  const grim_oracle_geometry* gm, const double* prim,
                                  double* out);
 
+/* fullStepDiagnostics' WriteData variables (src/problem/torus/problem.cpp:
+ * 808-875) of the element set at prim: (dof + 5) planes of N1*N2*N3 bulk
+ * zones, X1 fastest: rho, u, v^1..3, [q], [deltaP], B^1..3, b^2, gamma
+ * (params.cpp:136-147), then the x coordinates of the zone centres. */
+void grim_oracle_dump_vars(const grim_oracle_grid* c, const grim_oracle_params* p,
+                           const grim_oracle_geometry* gm, const double* prim, double* out);
+
 typedef struct grim_oracle_step_stats {
   double dt;
   int64_t unconverged[2];   /* bulk zones above atol after each stage */

@@ -1517,6 +1517,72 @@ void grim_oracle_torus_bcs(const grim_oracle_grid* c, const grim_oracle_params* 
   torusBCs(*c, *p, *t, D, *gm, prim);
 }
 
+// fullStepDiagnostics' WriteData variables (torus/problem.cpp:808-875) of the
+// element set at prim, bulk zones in natural order (X1 fastest), planes
+//   rho, u (floored), v^i = u^i/u^t and B^i = b^i u^t - b^t u^i in the
+//   x coordinates, [q], [deltaP], b^2, the Lorentz factor u^t alpha_x
+// (torus params.cpp:136-147), then 3 planes of xCoords (the VTS points).
+// dxdX by geometry::computeTransformationMatrices (geometry.cpp:441-508):
+// central differences of XCoordsToxCoords with GAMMA_EPS, dxdX[0][0] = 1 and
+// the other time entries 0; conXTox (geometry.cpp:426-439) sums over all four.
+void grim_oracle_dump_vars(const grim_oracle_grid* c, const grim_oracle_params* p,
+                           const grim_oracle_geometry* gm, const double* prim, double* out) {
+  const Dims D(*c);
+  const Vars vars(*p);
+  const Coord C{*c};
+  const int64_t n = D.n, nb = (int64_t)D.N[0] * D.N[1] * D.N[2];
+  const double eps = 1.e-5;  // GAMMA_EPS, geometry.cpp:11
+  int64_t t = 0;
+  for (int k = D.g[2]; k < D.g[2] + D.N[2]; k++)
+    for (int j = D.g[1]; j < D.g[1] + D.N[1]; j++)
+      for (int i = D.g[0]; i < D.g[0] + D.N[0]; i++, t++) {
+        const int64_t z = D.idx(i, j, k);
+        double X[3], x[3];
+        XCoordsAt(*c, D, i, j, k, LOC_CENTER, X);
+        C.XCoordsToxCoords(X, x);
+        double dxdX[4][4] = {{1., 0., 0., 0.}, {0., 0., 0., 0.}, {0., 0., 0., 0.}, {0., 0., 0., 0.}};
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) {
+            double Xp[3] = {X[0], X[1], X[2]}, Xm[3] = {X[0], X[1], X[2]}, xp[3], xm[3];
+            Xp[b] += eps;
+            Xm[b] -= eps;
+            C.XCoordsToxCoords(Xp, xp);
+            C.XCoordsToxCoords(Xm, xm);
+            dxdX[a + 1][b + 1] = (xp[a] - xm[a]) / (2. * eps);
+          }
+        double P[GRIM_ORACLE_MAX_VARS];
+        loadPrim(prim, vars.dof, n, z, P);
+        ZoneGeom<double> G;
+        loadLocation(gm->center, n, z, &G);
+        FluidElement<double> e;
+        e.set(P, G, *p, ZoneClosure{1., 1.});
+        double ux[4], bx[4];
+        for (int mu = 0; mu < 4; mu++) {
+          ux[mu] = 0.;
+          bx[mu] = 0.;
+          for (int nu = 0; nu < 4; nu++) {
+            ux[mu] += dxdX[mu][nu] * e.uCon[nu];
+            bx[mu] += dxdX[mu][nu] * e.bCon[nu];
+          }
+        }
+        double gCon00x = 0.;
+        for (int mu = 0; mu < 4; mu++)
+          for (int nu = 0; nu < 4; nu++) gCon00x += dxdX[0][mu] * dxdX[0][nu] * G.gCon[mu][nu];
+        const double alphax = 1. / std::sqrt(-gCon00x);
+        std::vector<double> v;
+        v.push_back(e.rho);
+        v.push_back(e.u);
+        for (int a = 1; a <= 3; a++) v.push_back(ux[a] / ux[0]);
+        if (p->conduction) v.push_back(e.q);
+        if (p->viscosity) v.push_back(e.deltaP);
+        for (int a = 1; a <= 3; a++) v.push_back(bx[a] * ux[0] - bx[0] * ux[a]);
+        v.push_back(e.bSqr);
+        v.push_back(ux[0] * alphax);
+        for (int d = 0; d < 3; d++) v.push_back(x[d]);
+        for (size_t q = 0; q < v.size(); q++) out[(int64_t)q * nb + t] = v[q];
+      }
+}
+
 // fullStepDiagnostics' observers (torus/problem.cpp:757-776) as written in
 // src/problem/torus/ObserversTorus.hpp, on one rank: the element is set at
 // prim (centre geometry); af::sum(af::flat(X(domain))) becomes a serial sum

@@ -192,6 +192,7 @@ def lib():
         L.grim_oracle_apply_floor.argtypes = [Gp, P, Tp, Gm, vp]
         L.grim_oracle_torus_bcs.argtypes = [Gp, P, Tp, Gm, vp]
         L.grim_oracle_torus_observers.argtypes = [Gp, P, Gm, vp, vp]
+        L.grim_oracle_dump_vars.argtypes = [Gp, P, Gm, vp, vp]
         _lib = L
     return _lib
 
@@ -535,3 +536,22 @@ def torus_observers(g: Grid, p: Params, gm: Geometry, prim) -> dict:
     out = np.zeros(9)
     lib().grim_oracle_torus_observers(C.byref(g), C.byref(p), C.byref(gm._c()), _ptr(_f64(prim)), _ptr(out))
     return dict(zip(OBSERVABLES, out.tolist()))
+
+
+def dump_var_names(p: Params) -> list:
+    """dumpVars names in plane order (torus/problem.cpp:808-825,
+    params.cpp:136-147), then the VTS points x1, x2, x3."""
+    names = ["rho", "u", "u1", "u2", "u3"]
+    if p.conduction:
+        names.append("q")
+    if p.viscosity:
+        names.append("dP")
+    return names + ["B1", "B2", "B3", "bSqr", "gamma", "x1", "x2", "x3"]
+
+
+def dump_vars(g: Grid, p: Params, gm: Geometry, prim) -> np.ndarray:
+    """fullStepDiagnostics' dump variables of the element set at prim, as
+    (planes, N1*N2*N3) with X1 fastest (names: dump_var_names)."""
+    out = np.zeros((len(dump_var_names(p)), g.N1 * g.N2 * g.N3))
+    lib().grim_oracle_dump_vars(C.byref(g), C.byref(p), C.byref(gm._c()), _ptr(_f64(prim)), _ptr(out))
+    return out

@@ -62,7 +62,7 @@ def test_io_library_exports_and_binding():
 
     io = checkpoint.lib()
     names = declared_functions("grim_io.h")
-    assert names == ["grim_io_dump", "grim_io_last_error", "grim_io_load"]
+    assert names == ["grim_io_dump", "grim_io_dump_vts", "grim_io_last_error", "grim_io_load"]
     assert all(hasattr(io, n) and getattr(io, n).argtypes is not None for n in names)
 
 

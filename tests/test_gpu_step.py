@@ -485,6 +485,72 @@ def test_torus_observers_match_oracle(torch, emhd):
     assert ts.observers(dev(torch, prim)) == got
 
 
+def dump_vars_compare(got, ref, names, loose_points=False):
+    # v^i and B^i mix the X components through dx/dX, whose central
+    # differences carry an absolute error (x rounding / 1e-5): each is
+    # measured against the largest value of its three planes
+    group = {n: [names.index(f"{n[0]}{i}") for i in (1, 2, 3)] for n in ("u1", "u2", "u3", "B1", "B2", "B3")}
+    for v, n in enumerate(names):
+        scale = np.abs(ref[group[n]]).max() if n in group else np.abs(ref[v]).max()
+        err = np.abs(got[v] - ref[v])
+        print(f"dump {n}: max diff / plane max {err.max() / (scale + 1e-300):.2e}", flush=True)
+        if n in ("rho", "u", "x3"):
+            np.testing.assert_array_equal(got[v], ref[v], err_msg=n)
+        elif loose_points and n not in ("bSqr", "gamma", "q", "dP"):
+            # acos / asin near 1 in the cylindrified map turn an ulp of libm
+            # into ~1e-8 of theta and its central differences into ~1e-3 in a
+            # few zones (as for the metric): bound the 99th percentile
+            assert np.percentile(err, 99) <= 1e-9 * scale and err.max() <= 1e-2 * scale, n
+        elif n in ("q", "dP", "x1", "x2"):
+            np.testing.assert_allclose(got[v], ref[v], rtol=1e-13, atol=0, err_msg=n)
+        elif n in ("bSqr", "gamma"):
+            np.testing.assert_allclose(got[v], ref[v], rtol=1e-12, atol=0, err_msg=n)
+        else:
+            np.testing.assert_allclose(got[v], ref[v], rtol=0, atol=1e-9 * scale, err_msg=n)
+
+
+@pytest.mark.parametrize("emhd", [False, True])
+def test_dump_vars_match_oracle(torch, emhd):
+    """fullStepDiagnostics' dump variables (torus/problem.cpp:808-875) on a
+    Kerr-Schild grid (oracle given the device geometry): rho, u and x3
+    identical; q, dP and the points within 1e-13 relative (device and host
+    exp / sin round differently); b^2 and gamma within 1e-12; v^i and B^i
+    carry dx/dX by central differences with step 1e-5 (one rounding of x
+    becomes about 1e-11 of dx/dX): 1e-9 of the largest of the three
+    components.  prim = NULL reads primOld."""
+    ts, g, p, po = make(MKS3D, emhd=emhd)
+    gm = oracle_geometry(ts, g)
+    prim = O.apply_boundaries(g, mks_state(g, po))
+    ts.primOld = prim
+    names = O.dump_var_names(po)
+    got = ts.dump_vars(dev(torch, prim)).cpu().numpy()
+    np.testing.assert_array_equal(ts.dump_vars().cpu().numpy(), got)
+    ref = O.dump_vars(g, po, gm, prim)
+    assert got.shape == ref.shape == (len(names), g.N1 * g.N2 * g.N3)
+    dump_vars_compare(got, ref, names)
+
+
+def test_dump_vars_torus_and_vts(torch, tmp_path):
+    """The dump of a stepped torus state (cylindrified coordinates) against
+    the oracle, then written with grim_io_dump_vts and read back bitwise."""
+    from grim_amd import checkpoint
+
+    ts, g, p, po, gm, t, floors = torus_case(True, maxNonLinearIter=6)
+    ts.set_time(0., 0.005)
+    ts.timeStep()
+    prim = ts.primOld
+    names = O.dump_var_names(po)
+    got = ts.dump_vars().cpu().numpy()
+    dump_vars_compare(got, O.dump_vars(g, po, gm, prim), names, loose_points=True)
+    f = tmp_path / checkpoint.dump_vts_name(0.005, 0.005)
+    checkpoint.dump_vts(ts.grid, names[:-3], got, str(f))
+    extent, pts, arrays = checkpoint.read_vts(str(f))
+    assert extent == [0, g.N1 - 1, 0, g.N2 - 1, 0, g.N3 - 1]
+    np.testing.assert_array_equal(pts, got[-3:].T)
+    for v, n in enumerate(names[:-3]):
+        np.testing.assert_array_equal(arrays[n], got[v])
+
+
 @pytest.mark.parametrize("case", ["fast_mode_fd", "torus_ideal_mixed"])
 def test_graph_captured_time_step_matches_eager(torch, case):
     """grim_time_step without stats enqueues a whole step with dt and time

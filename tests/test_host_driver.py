@@ -125,3 +125,39 @@ def test_driver_restart_continues_the_run(tmp_path):
     diff = (np.abs(a - b) / scale).max()
     print(f"restart vs uninterrupted: max scaled difference {diff:.2e}", flush=True)
     assert diff < 1e-9
+
+
+@pytest.mark.gpu
+def test_driver_write_every_writes_primitives_and_vts(tmp_path):
+    """WriteData (torus/problem.cpp:777-876): every time the run crosses a
+    multiple of --write-every it writes primVarsT%06d.h5 and dump%06d.vts;
+    after the last step they hold the final state (rho and u of the VTS equal
+    the primitives, the field within the 1e-10 that dx/dX by central
+    differences leaves, the points the zone centres)."""
+    from grim_amd import checkpoint
+    from grim_amd.timestepper import grid_config
+
+    args = [DRIVER, "--problem", "fastmode", "--solver", "fd", "--dt0", "0.05", "--steps", "2",
+            "--write-every", "1e-9", "--dump", "prim.bin"]
+    r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0, r.stderr
+    vts = sorted(tmp_path.glob("dump*.vts"))
+    h5 = sorted(tmp_path.glob("primVarsT*.h5"))
+    assert len(vts) == len(h5) == 2
+    g = 3
+    prim = np.fromfile(tmp_path / "prim.bin").reshape(8, 64 + 2 * g, 64 + 2 * g)[:, g:-g, g:-g].reshape(8, -1)
+    extent, pts, arrays = checkpoint.read_vts(str(vts[-1]))
+    assert extent == [0, 63, 0, 63, 0, 0]
+    assert list(arrays) == ["rho", "u", "u1", "u2", "u3", "B1", "B2", "B3", "bSqr", "gamma"]
+    np.testing.assert_array_equal(arrays["rho"], prim[0])
+    np.testing.assert_array_equal(arrays["u"], prim[1])
+    for i in range(3):
+        np.testing.assert_allclose(arrays[f"B{i + 1}"], prim[5 + i], rtol=1e-10, atol=1e-14)
+    X = (np.arange(64) + 0.5) / 64
+    np.testing.assert_array_equal(pts[:, 0], np.tile(X, 64))
+    np.testing.assert_array_equal(pts[:, 1], np.repeat(X, 64))
+    cfg = grid_config(N1=64, N2=64)
+    full = np.zeros((8, cfg.N1 + 2 * g, cfg.N2 + 2 * g))
+    loaded = checkpoint.load(cfg, full.reshape(8, -1).copy(), str(h5[-1]))
+    np.testing.assert_array_equal(loaded.reshape(8, 70, 70)[:, g:-g, g:-g].reshape(8, -1), prim)

@@ -35,6 +35,10 @@ except the boundary identities and the EMHD linear-mode error table):
 * torus observers (ObserversTorus.hpp): closed forms on uniform Minkowski
   states, an independent vectorised numpy evaluation of every integrand on a
   Kerr-Schild torus state, and the set-up's normalisation read back.
+* dump variables (torus/problem.cpp:808-875): Minkowski closed forms (v, B,
+  gamma, b^2, q and dP with and without the high-order factors, floors,
+  points); on MKS the analytic dx/dX applied to v^i and B^i and the
+  coordinate-free Lorentz factor.
 * parity unpinned (no independent check beyond the restatement): the torus
   closure limiter's exact form (problem.cpp:7-55), the inflowCheck velocity
   rescaling, the flux filter near the poles, and the splitmix64 noise that
@@ -657,6 +661,139 @@ def test_observers_torus_normalisation():
     got = O.torus_observers(g, p, gm, O.torus_ic(g, p, t, gm))
     assert got["rho_max"] == pytest.approx(1., rel=1e-12)
     assert got["beta_min"] == pytest.approx(100., rel=1e-9)
+
+
+# ---------------------------------------------------------------------------
+# Dump variables of fullStepDiagnostics (torus/problem.cpp:808-875)
+# ---------------------------------------------------------------------------
+def bulk_X(g):
+    """Zone-centre X of the bulk, X1 fastest (grid.cpp:200-203)."""
+    X = [getattr(g, f"X{d}Start") + (np.arange(N) + 0.5) * ((getattr(g, f"X{d}End") - getattr(g, f"X{d}Start")) / N)
+         for d, N in ((1, g.N1), (2, g.N2), (3, g.N3))]
+    X3, X2, X1 = np.meshgrid(X[2], X[1], X[0], indexing="ij")
+    return X1.ravel(), X2.ravel(), X3.ravel()
+
+
+def bulk_of(g, a):
+    """The bulk zones of ghosted planes, X1 fastest."""
+    a = np.asarray(a).reshape(-1, *g.shape)
+    s = tuple(slice(3, -3) if n > 1 else slice(None) for n in (g.N3, g.N2, g.N1))
+    return a[(slice(None), *s)].reshape(a.shape[0], -1)
+
+
+@pytest.mark.parametrize("high_order", [False, True])
+def test_dump_vars_closed_form_minkowski(high_order):
+    """Minkowski: dx/dX = 1, so v^i = u~^i / gamma, B^i is the primitive
+    field, gamma = sqrt(1 + u~.u~), b^2 = (B^2 + (B.u~)^2) / gamma^2 + floor,
+    q and dP are the primitives (times T sqrt(rho alpha c_s^2), T sqrt(T rho
+    alpha c_s^2) with high-order terms, c_s^2 = Gamma p / (rho + Gamma u)),
+    the floors hold, the points are the zone centres, and the planes are in
+    the order of params.cpp:136-147."""
+    g = O.grid(N1=7, N2=5, N3=1, dim=2, X1Start=-1., X1End=2.5, X2Start=0.5, X2End=1.)
+    p = O.default_params(True)
+    p.highOrderTermsConduction = p.highOrderTermsViscosity = int(high_order)
+    p.ConductionAlpha, p.ViscosityAlpha = 0.7, 1.3
+    gm = O.Geometry(g)
+    rng = np.random.default_rng(3)
+    prim = np.zeros((p.dof, g.n))
+    prim[0] = rng.uniform(0.5, 2., g.n)
+    prim[1] = rng.uniform(0.1, 1., g.n)
+    prim[2:5] = rng.uniform(-0.5, 0.5, (3, g.n))
+    prim[5:7] = rng.uniform(-0.1, 0.1, (2, g.n))
+    prim[7:10] = rng.uniform(-1., 1., (3, g.n))
+    prim[0, (3 + 3) * g.shape[2] + (3 + 4)] = 1e-30  # bulk zone (4, 3) below the density floor
+    got = O.dump_vars(g, p, gm, prim)
+    names = O.dump_var_names(p)
+    assert names == ["rho", "u", "u1", "u2", "u3", "q", "dP", "B1", "B2", "B3", "bSqr", "gamma", "x1", "x2", "x3"]
+    d = dict(zip(names, got))
+    P = bulk_of(g, prim)
+    rho = np.maximum(P[0], p.rhoFloorInFluidElement)
+    u = np.maximum(P[1], p.uFloorInFluidElement)
+    assert d["rho"].min() == p.rhoFloorInFluidElement
+    np.testing.assert_array_equal(d["rho"], rho)
+    np.testing.assert_array_equal(d["u"], u)
+    ut, B = P[2:5], P[7:10]
+    gam = np.sqrt(1. + (ut * ut).sum(0))
+    np.testing.assert_allclose(d["gamma"], gam, rtol=1e-14)
+    # dx/dX of the identity map by central differences is 1 only to rounding
+    # over the step (|X| ulp / 1e-5, about 1e-11): grim's dump carries it too
+    for i in range(3):
+        np.testing.assert_allclose(d[f"u{i + 1}"], ut[i] / gam, rtol=1e-10, atol=1e-14)
+        np.testing.assert_allclose(d[f"B{i + 1}"], B[i], rtol=1e-10, atol=1e-14)
+    bsq = ((B * B).sum(0) + ((B * ut).sum(0)) ** 2) / gam ** 2 + p.bSqrFloorInFluidElement
+    np.testing.assert_allclose(d["bSqr"], bsq, rtol=1e-13)
+    pres = (p.adiabaticIndex - 1.) * u
+    T = np.maximum(pres / rho, p.temperatureFloorInFluidElement)
+    cs2 = p.adiabaticIndex * pres / (rho + p.adiabaticIndex * u)
+    fq = T * np.sqrt(rho * p.ConductionAlpha * cs2) if high_order else 1.
+    fp = np.sqrt(T * rho * p.ViscosityAlpha * cs2) if high_order else 1.
+    np.testing.assert_allclose(d["q"], P[5] * fq, rtol=1e-13)
+    np.testing.assert_allclose(d["dP"], P[6] * fp, rtol=1e-13)
+    X1, X2, X3 = bulk_X(g)
+    np.testing.assert_array_equal(d["x1"], X1)
+    np.testing.assert_array_equal(d["x2"], X2)
+    np.testing.assert_array_equal(d["x3"], X3)
+
+
+def test_dump_vars_mks_transformation():
+    """Modified Kerr-Schild (no cylindrification): r = e^X1, theta = pi X2 +
+    (1-h)/2 sin 2 pi X2, so dx/dX = diag(r, pi (1 + (1-h) cos 2 pi X2), 1);
+    v^i and B^i transform with that diagonal (u^t and b^t do not change),
+    gamma = sqrt(1 + g_ij u~^i u~^j) is coordinate-free.  u^mu from the
+    primitive: u^t = gamma / alpha, u^i = u~^i - gamma alpha g^{ti}."""
+    h = 0.3
+    g = O.grid(N1=10, N2=8, N3=3, dim=3, metric=O.METRIC_MKS, hSlope=h, X1Start=np.log(1.5),
+               X1End=np.log(40.), X2Start=0.05, X2End=0.95, X3Start=0., X3End=2 * np.pi)
+    p = O.default_params(False)
+    gm = O.Geometry(g)
+    rng = np.random.default_rng(5)
+    prim = np.zeros((p.dof, g.n))
+    prim[0], prim[1] = rng.uniform(0.5, 2., g.n), rng.uniform(0.1, 1., g.n)
+    prim[2:5] = rng.uniform(-0.3, 0.3, (3, g.n))
+    prim[5:8] = rng.uniform(-1., 1., (3, g.n))
+    d = dict(zip(O.dump_var_names(p), O.dump_vars(g, p, gm, prim)))
+    X1, X2, X3 = bulk_X(g)
+    r = np.exp(X1)
+    J = np.array([r, np.pi * (1. + (1. - h) * np.cos(2 * np.pi * X2)), np.ones_like(X3)])
+    np.testing.assert_allclose(d["x1"], r, rtol=1e-15)
+    np.testing.assert_allclose(d["x2"], np.pi * X2 + 0.5 * (1 - h) * np.sin(2 * np.pi * X2), rtol=1e-15)
+    np.testing.assert_array_equal(d["x3"], X3)
+    geo = O.mks_geometry(X1, X2, g.blackHoleSpin, h)
+    gCov, gCon, alpha = geo["gCov"].reshape(4, 4, -1), geo["gCon"].reshape(4, 4, -1), geo["alpha"]
+    P = bulk_of(g, prim)
+    ut, B = P[2:5], P[5:8]
+    gam = np.sqrt(1. + np.einsum("ijn,in,jn->n", gCov[1:, 1:], ut, ut))
+    np.testing.assert_allclose(d["gamma"], gam, rtol=1e-12)
+    u0 = gam / alpha
+    ui = ut - gam * alpha * gCon[0, 1:]
+    for i in range(3):
+        np.testing.assert_allclose(d[f"u{i + 1}"], J[i] * ui[i] / u0, rtol=1e-8, atol=1e-12)
+        np.testing.assert_allclose(d[f"B{i + 1}"], J[i] * B[i], rtol=1e-8, atol=1e-12)
+
+
+def test_vts_round_trip(tmp_path):
+    """grim_io_dump_vts writes a StructuredGrid whose points and point data
+    read back bit for bit, with the extent of the bulk and the given names."""
+    from grim_amd import capi, checkpoint
+    from grim_amd.timestepper import grid_config
+
+    cfg = grid_config(N1=6, N2=4, N3=3, dim=3)
+    names = ["rho", "u", "gamma"]
+    data = np.random.default_rng(0).standard_normal((len(names) + 3, 6 * 4 * 3))
+    f = tmp_path / "dump000000.vts"
+    checkpoint.dump_vts(cfg, names, data, str(f))
+    extent, pts, arrays = checkpoint.read_vts(str(f))
+    assert extent == [0, 5, 0, 3, 0, 2]
+    np.testing.assert_array_equal(pts, data[-3:].T)
+    assert list(arrays) == names
+    for v, n in enumerate(names):
+        np.testing.assert_array_equal(arrays[n], data[v])
+    assert checkpoint.dump_vts_name(0.25, 0.1) == "dump000002.vts"
+    with pytest.raises(ValueError):
+        checkpoint.dump_vts(cfg, names, data[:-1], str(f))
+    with pytest.raises(RuntimeError, match="usable name"):
+        checkpoint.dump_vts(cfg, ["rho", "a\"b", "c"], data, str(f))
+    assert capi.GridConfig  # the binding's struct is the one the writer takes
 
 
 # ---------------------------------------------------------------------------
