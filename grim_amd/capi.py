@@ -243,6 +243,7 @@ SIGNATURES = {
     "grim_torus_bcs": (_i, [_vp, _vp]),
     "grim_torus_observers": (_i, [_vp, _vp, _vp]),
     "grim_dump_vars": (_i, [_vp, _vp, _vp, _i]),
+    "grim_set_ghost_state": (_i, [_vp, _vp, _i]),
 }
 
 _lib = None

@@ -1372,6 +1372,7 @@ struct grim_ctx {
   grim_torus_params tp;
   double* floors;  // 2 planes: rho and u floor values
   double* dump;    // dump variables staged for a host copy (first such call)
+  double* ghostState;  // dof planes whose ghost zones the half stage imposes, or null
 };
 
 namespace grim {
@@ -1659,6 +1660,20 @@ int dumpVars(grim_ctx* x, const double* prim, double* out) {
   return checkLaunch("dump variables");
 }
 
+// setProblemSpecificBCs of bondi_inflow (bondi_inflow/problem.cpp:157-183):
+// every ghost zone of prim from the stored initial state.
+__global__ __launch_bounds__(kBlock) void ghostStateKernel(Dims D, int nvars, const double* __restrict__ ic,
+                                                           double* __restrict__ prim) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= D.n) return;
+  int i, j, k;
+  zoneIndex(D, z, i, j, k);
+  if (i >= D.g[0] && i < D.g[0] + D.N[0] && j >= D.g[1] && j < D.g[1] + D.N[1] && k >= D.g[2] &&
+      k < D.g[2] + D.N[2])
+    return;
+  for (int v = 0; v < nvars; v++) prim[v * D.n + z] = ic[v * D.n + z];
+}
+
 // setProblemSpecificBCs of the torus: inflowCheck on both radial ghost
 // slabs, then fixPoles (problem.cpp:1217-1241).
 template <class L>
@@ -1696,6 +1711,12 @@ int stage(grim_ctx* x, int st, grim_step_stats* stats) {
   if (int rc = applyBoundaries(x, primStage, L::DOF, false)) return rc;
   if (x->torus)
     if (int rc = torusBCs<L>(x, primStage)) return rc;
+  // grim calls the hook in both stages, but it writes primOld: only the half
+  // stage changes anything
+  if (x->ghostState && st == 0) {
+    ghostStateKernel<<<blocksFor(D.n), kBlock, 0, x->s>>>(D, L::DOF, x->ghostState, primStage);
+    if (int rc = checkLaunch("ghost state")) return rc;
+  }
   if (int rc = explicitSources<L>(x, primStage, x->primOld, x->sExp, L::EMHD ? x->sImp : nullptr,
                                   st == 0 ? x->consOld : nullptr))
     return rc;
@@ -1904,6 +1925,7 @@ int grim_destroy(grim_ctx* x) {
   if (x->hostBuf) (void)hipHostFree(x->hostBuf);
   if (x->floors) (void)hipFree(x->floors);
   if (x->dump) (void)hipFree(x->dump);
+  if (x->ghostState) (void)hipFree(x->ghostState);
   delete x;
   return GRIM_OK;
 }
@@ -2142,6 +2164,22 @@ int grim_dump_vars(grim_ctx* x, const double* prim, double* out, int on_device) 
   if (int rc = withLayout(x, [&]<class L>() -> int { return dumpVars<L>(x, prim, dst); })) return rc;
   if (on_device) return GRIM_OK;
   GRIM_HIP_TRY(hipMemcpyAsync(out, x->dump, bytes, hipMemcpyDeviceToHost, x->s));
+  GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+  return GRIM_OK;
+}
+
+int grim_set_ghost_state(grim_ctx* x, const double* prim_ic, int on_device) {
+  GRIM_REQUIRE(x, "grim_set_ghost_state: null context");
+  if (!prim_ic) {
+    if (x->ghostState) GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+    if (x->ghostState) (void)hipFree(x->ghostState);
+    x->ghostState = nullptr;
+    return GRIM_OK;
+  }
+  const size_t bytes = sizeof(double) * (size_t)x->dof * x->D.n;
+  if (!x->ghostState) GRIM_HIP_TRY(hipMalloc(&x->ghostState, bytes));
+  GRIM_HIP_TRY(hipMemcpyAsync(x->ghostState, prim_ic, bytes,
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, x->s));
   GRIM_HIP_TRY(hipStreamSynchronize(x->s));
   return GRIM_OK;
 }

@@ -1,6 +1,7 @@
 """Initial states of further grim problems on the same time step, host side:
-the 1D EMHD shock tests (src/problem/shock_tests/) and the Orszag-Tang
-vortex (src/problem/orzag_tang/).  Each gives the grid, the parameters and
+the 1D EMHD shock tests (src/problem/shock_tests/), the Orszag-Tang vortex
+(src/problem/orzag_tang/) and Bondi inflow (src/problem/bondi_inflow/, whose
+ghost-zone hook is TimeStepper.set_ghost_state).  Each gives the grid, the parameters and
 the primitive state, set into a TimeStepper with ``primOld = ...``.
 
 Both run on grim's time step unchanged.  The shock tests supply a closure
@@ -125,4 +126,100 @@ def orszag_tang_prim(grid: capi.GridConfig, p: capi.Params) -> np.ndarray:
     b1 = p.dof - 3
     prim[b1] = -1. / np.sqrt(4 * np.pi) * np.sin(2 * np.pi * X2)
     prim[b1 + 1] = 1. / np.sqrt(4 * np.pi) * np.sin(4 * np.pi * X1)
+    return prim.reshape(p.dof, -1)
+
+
+# ---------------------------------------------------------------------------
+# Bondi inflow onto a Schwarzschild hole (src/problem/bondi_inflow/)
+# ---------------------------------------------------------------------------
+# The reference directory holds the problem but no params.cpp: sonicRadius,
+# mDot and bMag (params.hpp:201-203) get the values below, the usual
+# hydrodynamic Bondi test with a weak radial monopole field.
+BONDI = dict(sonicRadius=8., mDot=1., bMag=1e-2)
+
+
+def bondi_grid(N1: int = 256, Rin: float = 3., Rout: float = 40.) -> capi.GridConfig:
+    """1D modified Kerr-Schild grid, spin 0, X1 = ln r on [ln Rin, ln Rout],
+    the equator (X2 = 1/2 with hSlope 1), WENO5 + LLF, Courant 0.5."""
+    from .timestepper import grid_config
+
+    return grid_config(N1=N1, N2=1, N3=1, dim=1, metric=capi.METRIC_MKS, blackHoleSpin=0., hSlope=1.,
+                       X1Start=np.log(Rin), X1End=np.log(Rout), X2Start=0., X2End=1., X3Start=0.,
+                       X3End=2 * np.pi, boundaryLeft=capi.BC_OUTFLOW, boundaryRight=capi.BC_OUTFLOW,
+                       reconstruction=capi.RECON_WENO5, riemannSolver=capi.RIEMANN_LLF, CourantFactor=0.5,
+                       maxDtIncrement=1.3)
+
+
+def bondi_params() -> capi.Params:
+    """Ideal MHD, Gamma 4/3; the problem's closure is the linear-modes form
+    (tau = 1, chi = nu = c_s^2 tau, bondi_inflow/problem.cpp:3-8)."""
+    p = capi.default_params(False)
+    p.adiabaticIndex = 4. / 3
+    p.closure = capi.CLOSURE_LINEAR_MODES
+    return p
+
+
+def _bondi_T(R, C1, C2, n):
+    """SolveT (bondi_inflow/problem.cpp:20-55): regula falsi on FuncT for the
+    temperature at radius R, same bracket and stopping rule."""
+    def f(T):
+        return (1. + (1. + n) * T) ** 2 * (1. - 2. / R + (C1 / R / R / T ** n) ** 2) - C2
+
+    Tmin = 0.6 * (np.sqrt(C2) - 1.) / (n + 1.)
+    Tmax = (C1 * np.sqrt(2. / R ** 3)) ** (1. / n)
+    T0, T1 = 0.6 * Tmin, Tmax
+    f0, f1 = f(T0), f(T1)
+    if f0 * f1 > 0.:
+        raise ValueError(f"Bondi: no temperature bracket at R = {R}")
+    Th = (f1 * T0 - f0 * T1) / (f1 - f0)
+    fh = f(Th)
+    eps = 1e-12 * (Tmin + Tmax)
+    while abs(Th - T0) > eps and abs(Th - T1) > eps and abs(fh) > 1e-14:
+        if fh * f0 < 0.:
+            T0, f0 = Th, fh
+        else:
+            T1, f1 = Th, fh
+        Th = (f1 * T0 - f0 * T1) / (f1 - f0)
+        fh = f(Th)
+    return Th
+
+
+def bondi_constants(p: capi.Params, sonicRadius=BONDI["sonicRadius"], mDot=BONDI["mDot"]):
+    """(n, C1, C2, Kp) of the Bondi solution through the sonic point
+    (bondi_inflow/problem.cpp:59-67)."""
+    n = 1. / (p.adiabaticIndex - 1.)
+    Rc = sonicRadius
+    uc = np.sqrt(0.5 / Rc)
+    vc2 = n * uc * uc / (1. - 3. * uc * uc)
+    Tc = vc2 / (1. - vc2) / (n + 1.)
+    C1 = Tc ** n * uc * Rc * Rc
+    C2 = (1. + (n + 1.) * Tc) ** 2 * (1. - 2. / Rc + uc * uc)
+    Kp = (mDot / 4. / np.pi / C1) ** (-1. / n)
+    return n, C1, C2, Kp
+
+
+def bondi_prim(grid: capi.GridConfig, p: capi.Params, **kw) -> np.ndarray:
+    """timeStepper::initialConditions (bondi_inflow/problem.cpp:57-145) on
+    every zone, ghosts included: per radius r = e^X1 the temperature from
+    SolveT, rho = (T/Kp)^n, u = n rho T, u^r = -C1 / (r^2 T^n), u^t from
+    u.u = -1 in Kerr-Schild, u~^1 = (u^r + u^t 2/(r+2)) / r, B^1 = bMag / r^3
+    (rho floor, u = 0, u~ = 0 inside r = 2)."""
+    bMag = kw.pop("bMag", BONDI["bMag"])
+    n, C1, C2, Kp = bondi_constants(p, **kw)
+    n3, n2, n1 = grid.shape
+    X1 = grid.X1Start + (np.arange(n1) - grid.numGhost + 0.5) * ((grid.X1End - grid.X1Start) / grid.N1)
+    prim = np.zeros((p.dof, n3, n2, n1))
+    b1 = p.dof - 3
+    for i, R in enumerate(np.exp(X1)):
+        if R > 2. + 1e-8:
+            T = _bondi_T(R, C1, C2, n)
+            rho = (T / Kp) ** n
+            ur = -C1 / R / R / T ** n
+            ut = (2. * ur / R + np.sqrt(ur * ur + 1. - 2. / R)) / (1. - 2. / R)
+        else:
+            T, rho, ur, ut = 1., p.rhoFloorInFluidElement, 0., 1.
+        prim[0][..., i] = rho
+        prim[1][..., i] = n * rho * T
+        prim[2][..., i] = (ur + ut * 2. / (R + 2.)) / R
+        prim[b1][..., i] = bMag / R ** 3
     return prim.reshape(p.dof, -1)

@@ -145,6 +145,23 @@ class TimeStepper:
                                                    C.byref(out)), "grim_torus_observers")
         return out.as_dict()
 
+    def set_ghost_state(self, prim_ic):
+        """setProblemSpecificBCs of bondi_inflow (problem.cpp:157-183): every
+        ghost zone of primOld is reset to prim_ic (dof x zones, host array or
+        device tensor) in each half stage; None switches it off."""
+        if prim_ic is None:
+            capi.check(capi.lib().grim_set_ghost_state(self._ctx, None, 0), "grim_set_ghost_state")
+            return
+        on_device = hasattr(prim_ic, "is_cuda") and prim_ic.is_cuda
+        if on_device:
+            ptr = capi._ptr(prim_ic)
+        else:
+            prim_ic = np.ascontiguousarray(prim_ic, dtype=np.float64)
+            ptr = prim_ic.ctypes.data
+        if (prim_ic.numel() if on_device else prim_ic.size) != self.params.dof * self.n:
+            raise ValueError("set_ghost_state: expected dof x zones values")
+        capi.check(capi.lib().grim_set_ghost_state(self._ctx, ptr, int(on_device)), "grim_set_ghost_state")
+
     def dump_vars(self, prim=None):
         """fullStepDiagnostics' dump variables (torus/problem.cpp:808-875) of
         prim (None: primOld) as a device tensor (dof + 5, N1*N2*N3), X1

@@ -384,6 +384,16 @@ typedef struct grim_torus_observables {
  * the context's stream. */
 int grim_torus_observers(grim_ctx* ctx, const double* prim, grim_torus_observables* out);
 
+/* Replaces setProblemSpecificBCs of bondi_inflow (src/problem/bondi_inflow/
+ * problem.cpp:157-183): from the next step on, every ghost zone of primOld
+ * is set from prim_ic (dof planes of grim_grid_zones doubles; host memory,
+ * or device memory when on_device != 0; copied, so the caller may free it)
+ * after the half stage's boundary conditions.  grim calls the hook in both
+ * stages, but it always writes primOld, so only the half stage changes
+ * anything.  prim_ic = NULL switches the hook off.  Synchronises the
+ * context's stream. */
+int grim_set_ghost_state(grim_ctx* ctx, const double* prim_ic, int on_device);
+
 /* Replaces the dumpVars fill of fullStepDiagnostics (torus/problem.cpp:
  * 808-875, layout params.cpp:136-147).  out: device buffer of (dof + 5)
  * planes of grim_grid_bulk zones each (N1*N2*N3, X1 fastest, no ghosts):

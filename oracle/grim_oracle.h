@@ -318,6 +318,12 @@ void grim_oracle_time_step(const grim_oracle_grid* c, const grim_oracle_params* 
                            double* primOld, double* primDt, double* dt, double* time,
                            int semantics, grim_oracle_step_stats* stats);
 
+/* setProblemSpecificBCs of bondi_inflow (src/problem/bondi_inflow/
+ * problem.cpp:157-183): every ghost zone of primOld set from prim_ic (dof
+ * planes, kept by the caller) after the half stage's boundary conditions;
+ * NULL switches the hook off.  Process-wide. */
+void grim_oracle_set_ghost_state(const double* prim_ic);
+
 /* The two stages as separate calls, for stage-level parity tests.
  * stage 0 (half): boundaries on primOld, consOld, sources, divF, induction,
  * Newton; writes primHalf (dof planes) and consOld (dof planes).

@@ -1119,6 +1119,22 @@ void scrubNaNs(const Dims& D, int nv, double* prim) {
 }
 
 // One stage of timeStep (timestep.cpp:14-107 half, 183-255 full).
+// setProblemSpecificBCs of bondi_inflow (bondi_inflow/problem.cpp:157-183):
+// every ghost zone of primOld taken from the initial state after the
+// boundary conditions (null: hook off).  grim calls the hook in both stages
+// and it always writes primOld, so only the half stage changes anything.
+const double* g_ghostState = nullptr;
+
+void ghostStateBCs(const Dims& D, int nv, const double* ic, double* prim) {
+  for (int k = 0; k < D.NT[2]; k++)
+    for (int j = 0; j < D.NT[1]; j++)
+      for (int i = 0; i < D.NT[0]; i++) {
+        if (D.bulk(i, j, k)) continue;
+        const int64_t z = D.idx(i, j, k);
+        for (int v = 0; v < nv; v++) prim[v * D.n + z] = ic[v * D.n + z];
+      }
+}
+
 void stage(const grim_oracle_grid& c, const grim_oracle_params& p, const Dims& D,
            const grim_oracle_geometry& gm, const grim_oracle_torus* torus, int st, double dt,
            double* primOld, double* primHalf, double* consOld, double* primNew, int semantics,
@@ -1134,6 +1150,7 @@ void stage(const grim_oracle_grid& c, const grim_oracle_params& p, const Dims& D
   double* primStage = (st == 0) ? primOld : primHalf;
   applyBCs(c, D, nv, primStage);
   if (torus) torusBCs(c, p, *torus, D, gm, primStage);
+  if (g_ghostState && st == 0) ghostStateBCs(D, nv, g_ghostState, primStage);
   const double dtPrime = (st == 0) ? dt / 2. : dt;
 
   std::vector<double> Sexp((size_t)nv * n, 0.), divF((size_t)nv * n, 0.);
@@ -1465,6 +1482,8 @@ double grim_oracle_compute_dt(const grim_oracle_grid* c, const grim_oracle_param
   const Dims D(*c);
   return computeDt(*c, *p, D, *gm, prim, dt);
 }
+
+void grim_oracle_set_ghost_state(const double* prim_ic) { g_ghostState = prim_ic; }
 
 void grim_oracle_stage(const grim_oracle_grid* c, const grim_oracle_params* p,
                        const grim_oracle_geometry* gm, const grim_oracle_torus* t, int st,

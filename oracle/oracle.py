@@ -193,6 +193,7 @@ def lib():
         L.grim_oracle_torus_bcs.argtypes = [Gp, P, Tp, Gm, vp]
         L.grim_oracle_torus_observers.argtypes = [Gp, P, Gm, vp, vp]
         L.grim_oracle_dump_vars.argtypes = [Gp, P, Gm, vp, vp]
+        L.grim_oracle_set_ghost_state.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -462,8 +463,10 @@ class Simulation:
     (and the torus problem's hooks when torus is given)."""
 
     def __init__(self, g: Grid, p: Params, prim, dt: float, gm: Geometry | None = None, time: float = 0.,
-                 torus: "Torus | None" = None):
+                 torus: "Torus | None" = None, ghost_state=None):
         self.g, self.p = g, p
+        # bondi_inflow's hook: ghost zones of primOld from this state
+        self.ghost_state = None if ghost_state is None else _f64(ghost_state).copy()
         self.gm = gm if gm is not None else Geometry(g)
         self.prim = _f64(prim).copy()
         self.prim_dt = self.prim.copy()
@@ -473,10 +476,14 @@ class Simulation:
 
     def step(self, semantics=NEWTON_PER_ZONE) -> StepStats:
         st = StepStats()
-        lib().grim_oracle_time_step(C.byref(self.g), C.byref(self.p), C.byref(self.gm._c()),
-                                    C.byref(self.torus) if self.torus else None, _ptr(self.prim),
-                                    _ptr(self.prim_dt), C.byref(self.dt), C.byref(self.time), semantics,
-                                    C.byref(st))
+        lib().grim_oracle_set_ghost_state(None if self.ghost_state is None else _ptr(self.ghost_state))
+        try:
+            lib().grim_oracle_time_step(C.byref(self.g), C.byref(self.p), C.byref(self.gm._c()),
+                                        C.byref(self.torus) if self.torus else None, _ptr(self.prim),
+                                        _ptr(self.prim_dt), C.byref(self.dt), C.byref(self.time), semantics,
+                                        C.byref(st))
+        finally:
+            lib().grim_oracle_set_ghost_state(None)
         return st
 
 

@@ -222,13 +222,13 @@ def test_linear_mode_initial_conditions(torch):
     np.testing.assert_allclose(ts2.primOld, emhd_mode_ic(g2, po2), rtol=1e-15, atol=1e-22)
 
 
-def run_pair(ts, g, po, gm, steps, semantics=O.NEWTON_PER_ZONE, check_each=True, dt0=None):
+def run_pair(ts, g, po, gm, steps, semantics=O.NEWTON_PER_ZONE, check_each=True, dt0=None, ghost_state=None):
     """Steps the GPU context and the oracle from the same state, comparing the
     bulk bitwise after every step.  Returns the last stats."""
     prim0 = ts.primOld
     dt0 = dt0 if dt0 is not None else 0.5 / g.N1
     ts.set_time(0., dt0)
-    sim = O.Simulation(g, po, prim0, dt=dt0, gm=gm)
+    sim = O.Simulation(g, po, prim0, dt=dt0, gm=gm, ghost_state=ghost_state)
     for s in range(steps):
         st = ts.timeStep()
         so = sim.step(semantics)
@@ -293,6 +293,28 @@ def test_orszag_tang_steps_bitwise(torch):
     ts.primOld = PR.orszag_tang_prim(cfg, p)
     g = O.grid(**{f: getattr(cfg, f) for f, _ in cfg._fields_})
     run_pair(ts, g, po, O.Geometry(g), 6, dt0=0.002)
+    ts.close()
+
+
+def test_bondi_steps_bitwise(torch):
+    """Bondi inflow (src/problem/bondi_inflow/) on a 1D Kerr-Schild grid with
+    the problem's hook holding every ghost zone at the initial state; the
+    oracle runs on the device geometry."""
+    from grim_amd import problems as PR
+    from grim_amd.timestepper import TimeStepper
+
+    cfg = PR.bondi_grid(256)
+    p = PR.bondi_params()
+    po = O.default_params(False)
+    for name, _ in O.Params._fields_:
+        setattr(po, name, getattr(p, name))
+    ts = TimeStepper(p, cfg)
+    prim = PR.bondi_prim(cfg, p)
+    ts.primOld = prim
+    ts.set_ghost_state(prim)
+    g = O.grid(**{f: getattr(cfg, f) for f, _ in cfg._fields_})
+    run_pair(ts, g, po, oracle_geometry(ts, g), 8, dt0=1e-3, ghost_state=prim)
+    ts.set_ghost_state(None)
     ts.close()
 
 
