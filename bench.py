@@ -553,6 +553,22 @@ def full_step_bench(args):
     ot["eager"] = {"ms_per_step": ms, "zone_updates_per_s": 256 * 256 / (ms * 1e-3)}
     ts.close()
     res["orszag_tang_256"] = ot
+
+    # Bondi inflow (src/problem/bondi_inflow/): 1D Kerr-Schild r in [3, 40],
+    # 256 zones, WENO5 + LLF, ghost zones held at the initial state
+    cfg = problems.bondi_grid(256)
+    p = problems.bondi_params()
+    bondi = {"config": "1D Kerr-Schild Bondi inflow, 256 zones, r in [3, 40], sonic radius 8, WENO5+LLF, FD-FP64 "
+                       "Newton, ghost zones held at the initial state, 20 steps after 2 warm-up from dt0 = 1e-3"}
+    ts = TimeStepper(p, cfg, stream=torch.cuda.Stream())
+    prim = problems.bondi_prim(cfg, p)
+    ts.primOld = prim
+    ts.set_ghost_state(prim)
+    ts.set_time(0., 1e-3)
+    ms, _ = time_steps(ts, 20, 2)
+    bondi["eager"] = {"ms_per_step": ms, "zone_updates_per_s": 256 / (ms * 1e-3)}
+    ts.close()
+    res["bondi_1d_256"] = bondi
     return res
 
 
