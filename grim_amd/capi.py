@@ -297,12 +297,24 @@ def _stream(stream):
     return getattr(stream, "cuda_stream", stream)
 
 
-def _ptr(t):
+def _ptr(t, dtype="float64", numel: int = 0):
+    """Device address of a contiguous GPU tensor for the library.
+
+    dtype names the element type the entry point reads ("float64", "int32";
+    None for raw workspace bytes) and numel the number of elements it reads
+    at most.  A tensor of another type or a shorter one raises ValueError
+    here, before anything is enqueued: the library only sees the address and
+    would read past the end of the allocation."""
     if t is None:
         return None
     import torch
 
-    assert isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous(), "device tensors only"
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous()):
+        raise ValueError("expected a contiguous device tensor")
+    if dtype is not None and t.dtype != getattr(torch, dtype):
+        raise ValueError(f"expected a torch.{dtype} tensor, got {t.dtype}")
+    if t.numel() < numel:
+        raise ValueError(f"tensor holds {t.numel()} elements, the call reads {numel}")
     return t.data_ptr()
 
 
@@ -315,8 +327,9 @@ def device_info() -> str:
 def make_batch(prim, geom, rhs, prim_old=None, prim_coef=None, closure=None, radius=None,
                residual_mask=None, dt_prime=1.0) -> ZoneBatch:
     n = prim.shape[-1]
-    return ZoneBatch(n, _ptr(prim), _ptr(geom), _ptr(rhs), _ptr(prim_old), _ptr(prim_coef),
-                     _ptr(closure), _ptr(radius), _ptr(residual_mask), float(dt_prime))
+    return ZoneBatch(n, _ptr(prim, numel=8 * n), _ptr(geom, numel=GEOM_PLANES * n), _ptr(rhs, numel=5 * n),
+                     _ptr(prim_old, numel=8 * n), _ptr(prim_coef, numel=8 * n), _ptr(closure, numel=2 * n),
+                     _ptr(radius, numel=n), _ptr(residual_mask, numel=n), float(dt_prime))
 
 
 def conserved(p: Params, prim, geom, closure=None, radius=None, out=None, stream=None):
@@ -325,25 +338,29 @@ def conserved(p: Params, prim, geom, closure=None, radius=None, out=None, stream
     n = prim.shape[-1]
     if out is None:
         out = torch.empty((p.dof, n), dtype=torch.float64, device=prim.device)
-    check(lib().grim_conserved(C.byref(p), n, _ptr(prim), _ptr(geom), _ptr(closure), _ptr(radius),
-                               _ptr(out), _stream(stream)), "grim_conserved")
+    check(lib().grim_conserved(C.byref(p), n, _ptr(prim, numel=p.dof * n), _ptr(geom, numel=GEOM_PLANES * n),
+                               _ptr(closure, numel=2 * n), _ptr(radius, numel=n), _ptr(out, numel=p.dof * n),
+                               _stream(stream)), "grim_conserved")
     return out
 
 
 def residual(p: Params, batch: ZoneBatch, out, stream=None):
-    check(lib().grim_residual(C.byref(p), C.byref(batch), _ptr(out), _stream(stream)), "grim_residual")
+    check(lib().grim_residual(C.byref(p), C.byref(batch), _ptr(out, numel=p.nf * batch.n), _stream(stream)),
+          "grim_residual")
     return out
 
 
 def jacobian(p: Params, batch: ZoneBatch, kind: int, out, stream=None):
-    check(lib().grim_jacobian(C.byref(p), C.byref(batch), kind, _ptr(out), _stream(stream)), "grim_jacobian")
+    check(lib().grim_jacobian(C.byref(p), C.byref(batch), kind, _ptr(out, numel=p.nf * p.nf * batch.n),
+                              _stream(stream)), "grim_jacobian")
     return out
 
 
 def batch_linear_solve(A, b, x, lu_precision=LU_FP64, stream=None):
     nvar = b.shape[0]
     n = b.shape[-1]
-    check(lib().grim_batch_linear_solve(n, nvar, _ptr(A), _ptr(b), _ptr(x), lu_precision, _stream(stream)),
+    check(lib().grim_batch_linear_solve(n, nvar, _ptr(A, numel=nvar * nvar * n), _ptr(b, numel=nvar * n),
+                                        _ptr(x, numel=nvar * n), lu_precision, _stream(stream)),
           "grim_batch_linear_solve")
     return x
 
@@ -351,7 +368,7 @@ def batch_linear_solve(A, b, x, lu_precision=LU_FP64, stream=None):
 def newton_opts(jacobian=JAC_FD, lu_precision=LU_FP64, semantics=NEWTON_PER_ZONE, compaction=0,
                 block_size=0, fused=0, phase1_iters=0, phase2_group=0, workspace=None) -> NewtonOpts:
     return NewtonOpts(jacobian, lu_precision, semantics, compaction, block_size, fused, phase1_iters,
-                      phase2_group, _ptr(workspace), 0 if workspace is None else workspace.numel() *
+                      phase2_group, _ptr(workspace, dtype=None), 0 if workspace is None else workspace.numel() *
                       workspace.element_size())
 
 
@@ -367,19 +384,25 @@ def autotune_newton(p: Params, batch: ZoneBatch, opts: NewtonOpts, stream=None) 
     return o
 
 
+def newton_stats(n: int, iterations=None, residual_evals=None, final_l2=None) -> NewtonStats:
+    """grim_newton_stats of optional per-zone device arrays (int32, int32, float64)."""
+    return NewtonStats(_ptr(iterations, "int32", n), _ptr(residual_evals, "int32", n), _ptr(final_l2, numel=n))
+
+
 def batched_newton(p: Params, batch: ZoneBatch, jacobian=JAC_FD, lu_precision=LU_FP64,
                    semantics=NEWTON_PER_ZONE, compaction=0, block_size=0, iterations=None,
                    residual_evals=None, final_l2=None, stream=None, fused=0, opts=None):
     if opts is None:
         opts = newton_opts(jacobian, lu_precision, semantics, compaction, block_size, fused)
-    st = NewtonStats(_ptr(iterations), _ptr(residual_evals), _ptr(final_l2))
+    st = newton_stats(batch.n, iterations, residual_evals, final_l2)
     check(lib().grim_batched_newton(C.byref(p), C.byref(batch), C.byref(opts), C.byref(st), _stream(stream)),
           "grim_batched_newton")
 
 
 def count_above(final_l2, tol, stream=None) -> int:
     c = C.c_int64(0)
-    check(lib().grim_count_above(final_l2.shape[-1], _ptr(final_l2), float(tol), C.byref(c), _stream(stream)),
+    check(lib().grim_count_above(final_l2.shape[-1], _ptr(final_l2, numel=final_l2.shape[-1]), float(tol),
+                                 C.byref(c), _stream(stream)),
           "grim_count_above")
     return int(c.value)
 
@@ -405,6 +428,7 @@ def generate_kerr_schild_batch(p: Params, n: int, seed: int, a: float = 0.9375, 
 
 
 def known_root_rhs(p: Params, batch: ZoneBatch, rhs, mask=None, stream=None):
-    check(lib().grim_known_root_rhs(C.byref(p), C.byref(batch), _ptr(rhs), _ptr(mask), _stream(stream)),
+    check(lib().grim_known_root_rhs(C.byref(p), C.byref(batch), _ptr(rhs, numel=p.nf * batch.n),
+                                    _ptr(mask, numel=batch.n), _stream(stream)),
           "grim_known_root_rhs")
     return rhs

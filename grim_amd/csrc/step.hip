@@ -1662,10 +1662,13 @@ int dumpVars(grim_ctx* x, const double* prim, double* out) {
 
 // setProblemSpecificBCs of bondi_inflow (bondi_inflow/problem.cpp:157-183):
 // every ghost zone of prim from the stored initial state.
+// The hook is switched by a device flag that follows the dof planes of ic, so
+// a captured step picks up later switches and never reads freed memory.
 __global__ __launch_bounds__(kBlock) void ghostStateKernel(Dims D, int nvars, const double* __restrict__ ic,
                                                            double* __restrict__ prim) {
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= D.n) return;
+  if (ic[(int64_t)nvars * D.n] == 0.) return;
   int i, j, k;
   zoneIndex(D, z, i, j, k);
   if (i >= D.g[0] && i < D.g[0] + D.N[0] && j >= D.g[1] && j < D.g[1] + D.N[1] && k >= D.g[2] &&
@@ -1767,6 +1770,9 @@ int validConfig(const grim_params* p, const grim_grid_config* c) {
   if (!validParams(p) || !c) return 0;
   if (c->dim < 1 || c->dim > 3 || c->numGhost != 3) return 0;
   if (c->N1 < 1 || c->N2 < 1 || c->N3 < 1) return 0;
+  // makeDims adds the ghost zones in int: keep N + 2 numGhost representable
+  constexpr int kMaxN = (1 << 30);
+  if (c->N1 > kMaxN || c->N2 > kMaxN || c->N3 > kMaxN) return 0;
   if (c->dim < 2 && c->N2 != 1) return 0;
   if (c->dim < 3 && c->N3 != 1) return 0;
   if (c->metric != GRIM_METRIC_MINKOWSKI && c->metric != GRIM_METRIC_MKS) return 0;
@@ -1786,7 +1792,9 @@ int validConfig(const grim_params* p, const grim_grid_config* c) {
 extern "C" {
 
 int64_t grim_grid_zones(const grim_grid_config* c) {
-  if (!c) return 0;
+  if (!c || c->N1 < 1 || c->N2 < 1 || c->N3 < 1 || c->N1 > (1 << 30) || c->N2 > (1 << 30) || c->N3 > (1 << 30) ||
+      c->numGhost < 0 || c->numGhost > 64)
+    return 0;
   return makeDims(*c).n;
 }
 
@@ -2170,16 +2178,18 @@ int grim_dump_vars(grim_ctx* x, const double* prim, double* out, int on_device) 
 
 int grim_set_ghost_state(grim_ctx* x, const double* prim_ic, int on_device) {
   GRIM_REQUIRE(x, "grim_set_ghost_state: null context");
-  if (!prim_ic) {
-    if (x->ghostState) GRIM_HIP_TRY(hipStreamSynchronize(x->s));
-    if (x->ghostState) (void)hipFree(x->ghostState);
-    x->ghostState = nullptr;
-    return GRIM_OK;
-  }
+  // dof planes of the state, then the on/off flag (one double); the buffer
+  // stays allocated until grim_destroy, so steps captured while the hook was
+  // on keep reading valid memory after it is switched off
   const size_t bytes = sizeof(double) * (size_t)x->dof * x->D.n;
-  if (!x->ghostState) GRIM_HIP_TRY(hipMalloc(&x->ghostState, bytes));
-  GRIM_HIP_TRY(hipMemcpyAsync(x->ghostState, prim_ic, bytes,
-                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, x->s));
+  if (!prim_ic && !x->ghostState) return GRIM_OK;
+  if (!x->ghostState) GRIM_HIP_TRY(hipMalloc(&x->ghostState, bytes + sizeof(double)));
+  x->hostBuf[0] = prim_ic ? 1. : 0.;
+  if (prim_ic)
+    GRIM_HIP_TRY(hipMemcpyAsync(x->ghostState, prim_ic, bytes,
+                                on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, x->s));
+  GRIM_HIP_TRY(hipMemcpyAsync(x->ghostState + (size_t)x->dof * x->D.n, x->hostBuf, sizeof(double),
+                              hipMemcpyHostToDevice, x->s));
   GRIM_HIP_TRY(hipStreamSynchronize(x->s));
   return GRIM_OK;
 }

@@ -243,7 +243,7 @@ int grim_io_dump_vts(const grim_grid_config* c, int nvars, const char* const* na
     const uint64_t m = nb - t0 < chunk ? nb - t0 : chunk;
     pts.resize(3 * m);
     for (uint64_t t = 0; t < m; t++)
-      for (int d = 0; d < 3; d++) pts[3 * t + d] = xc[d * nb + t0 + t];
+      for (int d = 0; d < 3; d++) pts[3 * t + d] = d < c->dim ? xc[d * nb + t0 + t] : 0.;
     ok = std::fwrite(pts.data(), sizeof(double), 3 * m, f) == 3 * m;
   }
   for (int v = 0; ok && v < nvars; v++) {

@@ -110,12 +110,21 @@ class TimeStepper:
         """Torus initial conditions into primOld; enables the torus's per-stage hooks."""
         capi.check(capi.lib().grim_init_torus(self._ctx, C.byref(torus)), "grim_init_torus")
 
+    def _grid_ptr(self, t, planes: int):
+        """Address of a float64 device tensor holding at least `planes` planes
+        of the grid's zones (raises ValueError before any call otherwise)."""
+        if t is None:
+            return None
+        if t.dim() < 1 or t.shape[0] < planes:
+            raise ValueError(f"expected {planes} planes of {self.n} zones, got shape {tuple(t.shape)}")
+        return capi._ptr(t, numel=planes * self.n)
+
     def applyFloor(self, prim):
-        capi.check(capi.lib().grim_apply_floor(self._ctx, capi._ptr(prim)), "grim_apply_floor")
+        capi.check(capi.lib().grim_apply_floor(self._ctx, self._grid_ptr(prim, self.dof)), "grim_apply_floor")
         return prim
 
     def setProblemSpecificBCs(self, prim):
-        capi.check(capi.lib().grim_torus_bcs(self._ctx, capi._ptr(prim)), "grim_torus_bcs")
+        capi.check(capi.lib().grim_torus_bcs(self._ctx, self._grid_ptr(prim, self.dof)), "grim_torus_bcs")
         return prim
 
     def dump(self, file: str, dataset: str = "primitives"):
@@ -141,8 +150,8 @@ class TimeStepper:
         ComputeBoundaryFluxes (ObserversTorus.hpp:6-251) of prim (a device
         tensor of dof planes; None: primOld), as a dict."""
         out = capi.TorusObservables()
-        capi.check(capi.lib().grim_torus_observers(self._ctx, None if prim is None else capi._ptr(prim),
-                                                   C.byref(out)), "grim_torus_observers")
+        capi.check(capi.lib().grim_torus_observers(self._ctx, self._grid_ptr(prim, self.dof), C.byref(out)),
+                   "grim_torus_observers")
         return out.as_dict()
 
     def set_ghost_state(self, prim_ic):
@@ -154,12 +163,14 @@ class TimeStepper:
             return
         on_device = hasattr(prim_ic, "is_cuda") and prim_ic.is_cuda
         if on_device:
-            ptr = capi._ptr(prim_ic)
+            if prim_ic.numel() != self.dof * self.n:
+                raise ValueError("set_ghost_state: expected dof x zones values")
+            ptr = capi._ptr(prim_ic, numel=self.dof * self.n)
         else:
             prim_ic = np.ascontiguousarray(prim_ic, dtype=np.float64)
+            if prim_ic.size != self.dof * self.n:
+                raise ValueError("set_ghost_state: expected dof x zones values")
             ptr = prim_ic.ctypes.data
-        if (prim_ic.numel() if on_device else prim_ic.size) != self.params.dof * self.n:
-            raise ValueError("set_ghost_state: expected dof x zones values")
         capi.check(capi.lib().grim_set_ghost_state(self._ctx, ptr, int(on_device)), "grim_set_ghost_state")
 
     def dump_vars(self, prim=None):
@@ -170,8 +181,8 @@ class TimeStepper:
 
         nb = self.grid.N1 * self.grid.N2 * self.grid.N3
         out = torch.empty((self.params.dof + 5, nb), dtype=torch.float64, device=self.device)
-        capi.check(capi.lib().grim_dump_vars(self._ctx, None if prim is None else capi._ptr(prim), capi._ptr(out), 1),
-                   "grim_dump_vars")
+        capi.check(capi.lib().grim_dump_vars(self._ctx, self._grid_ptr(prim, self.dof),
+                                             capi._ptr(out, numel=out.numel()), 1), "grim_dump_vars")
         return out
 
     # -- the step -----------------------------------------------------------
@@ -189,40 +200,47 @@ class TimeStepper:
         return torch.full((planes, self.n), float("nan"), dtype=torch.float64, device=self.device)
 
     def applyBoundaryConditions(self, prim):
-        capi.check(capi.lib().grim_apply_boundaries(self._ctx, capi._ptr(prim), prim.shape[0]),
+        capi.check(capi.lib().grim_apply_boundaries(self._ctx, self._grid_ptr(prim, prim.shape[0]), prim.shape[0]),
                    "grim_apply_boundaries")
         return prim
 
     def communicate(self, prim):
-        capi.check(capi.lib().grim_communicate(self._ctx, capi._ptr(prim), prim.shape[0]), "grim_communicate")
+        capi.check(capi.lib().grim_communicate(self._ctx, self._grid_ptr(prim, prim.shape[0]), prim.shape[0]),
+                   "grim_communicate")
         return prim
 
     def reconstruct(self, prim, dir: int):
         left, right = self._empty(prim.shape[0]), self._empty(prim.shape[0])
-        capi.check(capi.lib().grim_reconstruct(self._ctx, capi._ptr(prim), prim.shape[0], dir, capi._ptr(left),
-                                               capi._ptr(right)), "grim_reconstruct")
+        nv = prim.shape[0]
+        capi.check(capi.lib().grim_reconstruct(self._ctx, self._grid_ptr(prim, nv), nv, dir,
+                                               self._grid_ptr(left, nv), self._grid_ptr(right, nv)),
+                   "grim_reconstruct")
         return left, right
 
     def riemannFlux(self, prim, dir: int):
         flux = self._empty(self.dof)
-        capi.check(capi.lib().grim_riemann_flux(self._ctx, capi._ptr(prim), dir, capi._ptr(flux)),
+        capi.check(capi.lib().grim_riemann_flux(self._ctx, self._grid_ptr(prim, self.dof), dir,
+                                                self._grid_ptr(flux, self.dof)),
                    "grim_riemann_flux")
         return flux
 
     def computeDivOfFluxes(self, prim, want_fluxes: bool = False):
         div = self._empty(self.dof)
         fl = self._empty(self.grid.dim * self.dof) if want_fluxes else None
-        capi.check(capi.lib().grim_div_fluxes(self._ctx, capi._ptr(prim), capi._ptr(div), capi._ptr(fl)),
+        capi.check(capi.lib().grim_div_fluxes(self._ctx, self._grid_ptr(prim, self.dof), self._grid_ptr(div, self.dof),
+                                              self._grid_ptr(fl, self.grid.dim * self.dof)),
                    "grim_div_fluxes")
         return (div, fl) if want_fluxes else div
 
     def computeExplicitSources(self, prim):
         src = self._empty(self.params.nf)
-        capi.check(capi.lib().grim_explicit_sources(self._ctx, capi._ptr(prim), capi._ptr(src)),
+        capi.check(capi.lib().grim_explicit_sources(self._ctx, self._grid_ptr(prim, self.dof),
+                                                    self._grid_ptr(src, self.params.nf)),
                    "grim_explicit_sources")
         return src
 
     def computeDt(self, prim, dt: float) -> float:
         out = C.c_double()
-        capi.check(capi.lib().grim_compute_dt(self._ctx, capi._ptr(prim), dt, C.byref(out)), "grim_compute_dt")
+        capi.check(capi.lib().grim_compute_dt(self._ctx, self._grid_ptr(prim, self.dof), dt, C.byref(out)),
+                   "grim_compute_dt")
         return out.value

@@ -268,7 +268,8 @@ typedef struct grim_step_stats {
                                  test counts them as converged) */
 } grim_step_stats;
 
-/* Number of zones (N1T*N2T*N3T) of a grid. */
+/* Number of zones (N1T*N2T*N3T) of a grid; 0 when N1, N2 or N3 lies outside
+ * [1, 2^30]. */
 int64_t grim_grid_zones(const grim_grid_config* c);
 
 /* timeStepper::timeStepper (src/timestepper/timestepper.cpp:3-386 without
@@ -390,8 +391,12 @@ int grim_torus_observers(grim_ctx* ctx, const double* prim, grim_torus_observabl
  * or device memory when on_device != 0; copied, so the caller may free it)
  * after the half stage's boundary conditions.  grim calls the hook in both
  * stages, but it always writes primOld, so only the half stage changes
- * anything.  prim_ic = NULL switches the hook off.  Synchronises the
- * context's stream. */
+ * anything.  prim_ic = NULL switches the hook off.  The hook's state buffer
+ * stays allocated until grim_destroy and the on/off switch is read on the
+ * device, so a step captured into a HIP graph after the first non-NULL call
+ * follows later switches and never reads freed memory; a step captured
+ * before that call does not contain the hook.  Synchronises the context's
+ * stream. */
 int grim_set_ghost_state(grim_ctx* ctx, const double* prim_ic, int on_device);
 
 /* Replaces the dumpVars fill of fullStepDiagnostics (torus/problem.cpp:

@@ -638,3 +638,39 @@ def test_graph_captured_time_step_matches_eager(torch, case):
     del gr
     eager.close()
     graphed.close()
+
+
+def test_bad_tensors_raise_before_any_launch(torch):
+    """A tensor of the wrong element type or too few planes is refused by the
+    Python binding (ValueError) before the library sees its address: the
+    library reads dof x zones doubles from whatever it is given."""
+    from grim_amd import capi
+
+    ts, g, p, po = make(dict(N1=16, N2=16, reconstruction=1, riemannSolver=1))
+    n = ts.n
+    good = torch.zeros((p.dof, n), dtype=torch.float64, device="cuda")
+    bad_type = torch.zeros((p.dof, 2 * n), dtype=torch.float32, device="cuda")  # same bytes, float32
+    short = torch.zeros((p.dof - 1, n), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        ts.set_ghost_state(bad_type.reshape(-1)[:p.dof * n])
+    with pytest.raises(ValueError):
+        ts.dump_vars(short)
+    with pytest.raises(ValueError):
+        ts.dump_vars(bad_type)
+    with pytest.raises(ValueError):
+        ts.applyBoundaryConditions(short.reshape(p.dof - 1, -1)[:, : n // 2])
+    with pytest.raises(ValueError):
+        ts.computeDivOfFluxes(short)
+    with pytest.raises(ValueError):
+        capi.make_batch(good[:, :8].contiguous(), torch.zeros((3, 8), dtype=torch.float64, device="cuda"),
+                        good[:5, :8].contiguous())
+    with pytest.raises(ValueError):
+        capi.batched_newton(p, capi.make_batch(good[:, :8].contiguous(),
+                                               torch.zeros((16, 8), dtype=torch.float64, device="cuda"),
+                                               good[:5, :8].contiguous()),
+                            iterations=torch.zeros(8, dtype=torch.int64, device="cuda"))
+    # the good tensors still go through
+    ts.set_ghost_state(good)
+    ts.set_ghost_state(None)
+    assert ts.dump_vars(good).shape == (p.dof + 5, 16 * 16)
+    ts.close()

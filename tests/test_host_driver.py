@@ -157,6 +157,7 @@ def test_driver_write_every_writes_primitives_and_vts(tmp_path):
     X = (np.arange(64) + 0.5) / 64
     np.testing.assert_array_equal(pts[:, 0], np.tile(X, 64))
     np.testing.assert_array_equal(pts[:, 1], np.repeat(X, 64))
+    np.testing.assert_array_equal(pts[:, 2], 0.)  # a 2D DMDA has (x, y) coordinates only
     cfg = grid_config(N1=64, N2=64)
     full = np.zeros((8, cfg.N1 + 2 * g, cfg.N2 + 2 * g))
     loaded = checkpoint.load(cfg, full.reshape(8, -1).copy(), str(h5[-1]))
