@@ -239,11 +239,16 @@ void grim_oracle_riemann_flux(const grim_oracle_grid* c, const grim_oracle_param
                               int dir, double* flux);
 
 /* computeDivOfFluxes: fluxes along each active direction, flux-CT for
- * dim >= 2, divergence (dof planes, bulk zones only).  fluxes (dim*dof
- * planes) may be NULL. */
+ * dim >= 2, the torus's applyProblemSpecificFluxFilter when flux_filter != 0
+ * (torus/problem.cpp:1243-1285), divergence (dof planes, bulk zones only).
+ * fluxes (dim*dof planes, after CT and the filter) may be NULL. */
 void grim_oracle_div_fluxes(const grim_oracle_grid* c, const grim_oracle_params* p,
                             const grim_oracle_geometry* gm, const double* prim,
-                            double* divF, double* fluxes);
+                            double* divF, double* fluxes, int flux_filter);
+
+/* The torus noise stream: uniform in [0, 1) of zone z = 0..n-1 (splitmix64
+ * of seed + 0x9E3779B97F4A7C15 (32 z + 1), top 53 bits). */
+void grim_oracle_uniform(uint64_t seed, int64_t n, double* out);
 
 /* fluidElement::computeExplicitSources of the element set at prim (dof
  * planes, bulk zones only). */

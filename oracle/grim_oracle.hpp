@@ -17,11 +17,9 @@
 // Pinning status of each part is recorded in the headers of
 // tests/test_oracle.py (per-zone routines) and tests/test_step_oracle.py
 // (the step, problems and observers).  Parity unpinned (no check beyond the
-// restatement itself): the torus closure limiter (torus/problem.cpp:7-55),
-// the inflowCheck velocity rescaling, the torus flux filter near the poles,
-// the splitmix64 noise that replaces PETSc's RAND48 stream, and, at the
-// ArrayFire boundary, af::solve's pivoting and the summation order of
-// af::sum (the observers' sums; the LU here is LAPACK's partial pivoting).
+// restatement itself): only the ArrayFire boundary -- af::solve's pivoting
+// and the summation order of af::sum (the observers' sums; the LU here is
+// LAPACK's partial pivoting) -- which no reference test exercises.
 #pragma once
 #include <algorithm>
 #include <cmath>

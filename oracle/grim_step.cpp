@@ -1465,9 +1465,13 @@ void grim_oracle_riemann_flux(const grim_oracle_grid* c, const grim_oracle_param
 
 void grim_oracle_div_fluxes(const grim_oracle_grid* c, const grim_oracle_params* p,
                             const grim_oracle_geometry* gm, const double* prim, double* divF,
-                            double* fluxes) {
+                            double* fluxes, int flux_filter) {
   const Dims D(*c);
-  divFluxes(*c, *p, D, *gm, prim, divF, fluxes);
+  divFluxes(*c, *p, D, *gm, prim, divF, fluxes, flux_filter != 0);
+}
+
+void grim_oracle_uniform(uint64_t seed, int64_t n, double* out) {
+  for (int64_t z = 0; z < n; z++) out[z] = uniformAt(seed, (uint64_t)z);
 }
 
 void grim_oracle_explicit_sources(const grim_oracle_grid* c, const grim_oracle_params* p,

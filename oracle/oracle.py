@@ -181,7 +181,8 @@ def lib():
         L.grim_oracle_apply_boundaries.argtypes = [Gp, C.c_int, vp]
         L.grim_oracle_reconstruct.argtypes = [Gp, C.c_int, vp, C.c_int, vp, vp]
         L.grim_oracle_riemann_flux.argtypes = [Gp, P, Gm, vp, C.c_int, vp]
-        L.grim_oracle_div_fluxes.argtypes = [Gp, P, Gm, vp, vp, vp]
+        L.grim_oracle_div_fluxes.argtypes = [Gp, P, Gm, vp, vp, vp, C.c_int]
+        L.grim_oracle_uniform.argtypes = [C.c_uint64, i64, vp]
         L.grim_oracle_explicit_sources.argtypes = [Gp, P, Gm, vp, vp]
         L.grim_oracle_compute_dt.argtypes = [Gp, P, Gm, vp, d]
         L.grim_oracle_compute_dt.restype = d
@@ -428,12 +429,21 @@ def riemann_flux(g: Grid, p: Params, gm: Geometry, prim, d: int):
     return out
 
 
-def div_fluxes(g: Grid, p: Params, gm: Geometry, prim, want_fluxes=False):
+def div_fluxes(g: Grid, p: Params, gm: Geometry, prim, want_fluxes=False, flux_filter=False):
+    """computeDivOfFluxes; flux_filter applies the torus's flux filter."""
     prim = _f64(prim)
     div = np.full((p.dof, g.n), np.nan)
     fl = np.zeros((g.dim * p.dof, g.n)) if want_fluxes else None
-    lib().grim_oracle_div_fluxes(C.byref(g), C.byref(p), C.byref(gm._c()), _ptr(prim), _ptr(div), _ptr(fl))
+    lib().grim_oracle_div_fluxes(C.byref(g), C.byref(p), C.byref(gm._c()), _ptr(prim), _ptr(div), _ptr(fl),
+                                 1 if flux_filter else 0)
     return (div, fl) if want_fluxes else div
+
+
+def uniform(seed: int, n: int) -> np.ndarray:
+    """The torus noise stream: uniform in [0, 1) of zones 0..n-1."""
+    out = np.empty(n)
+    lib().grim_oracle_uniform(C.c_uint64(seed), n, _ptr(out))
+    return out
 
 
 def explicit_sources(g: Grid, p: Params, gm: Geometry, prim):
