@@ -57,9 +57,17 @@ def parse():
 
 
 # Operation tallies per zone (add/sub/mul/div/sqrt/max = 1 each; a fused
-# multiply-add counts as its two operations), counted from
+# multiply-add counts as its two operations).  "Algorithmic" operations are
+# those a compiled evaluation must repeat: the exact residual's 177 are the
+# operations of grim's residual (src/timestepper/residual.cpp:3-180 over
+# src/physics/physics.cpp:53-217) that the residual rows depend on and that
+# depend on the guess, counted by the oracle's tracing scalar (72 add/sub,
+# 93 mul, 9 div, 1 sqrt, 2 max for ideal MHD; a CPU test keeps this table
+# equal to that count, and the LU's 120 equal to the counting scalar's).
+# grim's set() also forms quantities the residual never reads (333 operations
+# in all), which are not counted.  The other rows are tallied by hand from
 # grim_amd/csrc/zone_physics.hpp, analytic.hpp and newton_kernels.hpp:
-#   exact residual (grim order, ideal MHD)          151 FP64
+#   exact residual (grim order, ideal MHD)          177 FP64
 #   fast residual (identity form, fused paths)      115 FP64
 #   per test / line-search evaluation bookkeeping    24 FP64 (trial, 0.5 sum R^2, Armijo)
 #   FD quotients per Jacobian, grim order            65 FP64 (5 x (3 + 2 x 5))
@@ -70,7 +78,7 @@ def parse():
 #   row/column equilibration around the FP32 LU      90 FP32
 #   update bookkeeping (f0, f'0, P += lambda d)      12 FP64
 OPS = {
-    "res_exact": 151, "res_fast": 115, "eval_bookkeeping": 24, "fd_quotients": 65,
+    "res_exact": 177, "res_fast": 115, "eval_bookkeeping": 24, "fd_quotients": 65,
     "fd_quotients_fused": 75, "lu": 120, "analytic_jac": 282, "equilibration": 90, "update": 12,
 }
 

@@ -12,6 +12,7 @@
 
 namespace oracle {
 thread_local OpCounts g_ops;
+thread_local Tape g_tape;
 
 namespace {
 
@@ -478,6 +479,22 @@ void grim_oracle_count_ops(const grim_oracle_params* p,
   luOps[2] = c.div;
   luOps[3] = c.sqrt;
   luOps[4] = c.other;
+}
+
+void grim_oracle_count_live_ops(const grim_oracle_params* p, const grim_oracle_batch* b,
+                                int64_t* residualOps) {
+  const Vars vars(*p);
+  ZoneState<Traced> s;
+  s.load(*p, *b, 0);
+  g_tape.reset();  // everything load() formed is a per-zone constant
+  for (int v = 0; v < vars.dof; v++) s.prim[v] = Traced::variable(oval(s.prim[v]));
+  Traced r[GRIM_ORACLE_MAX_VARS];
+  computeResidual(s.prim, s.ctx, r);
+  long long c[6];
+  liveCounts(r, vars.numFluidVars, c);
+  for (int k = 0; k < 6; k++) residualOps[k] = c[k];
+  g_tape.reset();
+  g_tape.nodes.shrink_to_fit();
 }
 
 }  // extern "C"

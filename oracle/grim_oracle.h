@@ -158,6 +158,14 @@ void grim_oracle_count_ops(const grim_oracle_params* p,
                            const grim_oracle_batch* b, int64_t* residualOps,
                            int64_t* luOps);
 
+/* Live operations of one residual evaluation, from the tracing scalar type:
+ * those the residual rows depend on and that depend on the guess (work a
+ * compiled evaluation repeats per call; dead code and per-zone constants
+ * excluded): out[0]=add/sub, out[1]=mul, out[2]=div, out[3]=sqrt,
+ * out[4]=other, out[5]=max/min selects, for zone 0 of the batch. */
+void grim_oracle_count_live_ops(const grim_oracle_params* p,
+                                const grim_oracle_batch* b, int64_t* residualOps);
+
 /* ------------------------------------------------------------------------
  * Full time step (grim_step.cpp).
  *

@@ -172,6 +172,7 @@ def lib():
         L.grim_oracle_lu_solve.argtypes = [C.c_int, vp, vp]
         L.grim_oracle_lu_solve.restype = C.c_int
         L.grim_oracle_count_ops.argtypes = [P, B, vp, vp]
+        L.grim_oracle_count_live_ops.argtypes = [P, B, vp]
         Gp = C.POINTER(Grid)
         Gm = C.POINTER(_Geometry)
         L.grim_oracle_grid_zones.argtypes = [Gp]
@@ -367,6 +368,16 @@ def count_ops(p: Params, b: Batch):
     cb = b._c()
     lib().grim_oracle_count_ops(C.byref(p), C.byref(cb), _ptr(r), _ptr(l))
     return r, l
+
+
+def count_live_ops(p: Params, b: Batch):
+    """Live operations of one residual evaluation of zone 0 (the residual
+    depends on them and they depend on the guess) as [add, mul, div, sqrt,
+    other, select]."""
+    r = np.zeros(6, dtype=np.int64)
+    cb = b._c()
+    lib().grim_oracle_count_live_ops(C.byref(p), C.byref(cb), _ptr(r))
+    return r
 
 
 # ---------------------------------------------------------------------------

@@ -438,3 +438,47 @@ def test_newton_iteration_cap_leaves_unconverged(oracle):
     bad, it, l2, _ = oracle.newton(p, G)
     assert bad > 0 and np.all(it == 1)
     assert bad == int((l2 > p.nonlinearsolve_atol).sum())
+
+
+# -------------------------------------------------------- operation counts
+def test_bench_operation_tallies_match_oracle_counts(oracle):
+    """The per-zone operation tallies bench.py prices its VALU fraction with:
+    the exact residual's entry equals the oracle's live count (operations the
+    residual rows depend on and that depend on the guess), and the LU's
+    entry equals the counting scalar's tally of one 5x5 solve.  The live
+    count is a subset of every operation grim's formulation performs."""
+    import bench
+
+    ref = batchgen.kerr_schild_batch(oracle, 16, emhd=False, seed=20261202)
+    p = oracle.default_params(False)
+    B = oracle.Batch(ref["guess"], ref["geom"], np.ones((p.nf, 16)))
+    total_res, lu = oracle.count_ops(p, B)
+    live = oracle.count_live_ops(p, B)
+    assert int(live.sum()) == bench.OPS["res_exact"]
+    assert int(lu[:4].sum()) == bench.OPS["lu"] and lu[3] == 0 and lu[4] == 0
+    # live operations never exceed all operations of the same kind
+    assert (live[:5] <= total_res).all() and live[:5].sum() < total_res.sum()
+    # one square root (the Lorentz factor) per ideal residual, no transcendental
+    assert live[3] == 1 and live[4] == 0
+
+
+def test_live_count_is_a_property_of_the_formulation(oracle):
+    """The same count at every zone of a batch (no data-dependent branches
+    outside the floors), and more work for EMHD, whose rows add the closure,
+    the heat-flux and viscous stresses and the time-derivative sources."""
+    counts = {}
+    for emhd in (False, True):
+        ref = batchgen.kerr_schild_batch(oracle, 8, emhd=emhd, seed=20261202)
+        p = oracle.default_params(emhd)
+        if emhd:
+            p.closure = 1
+        per_zone = []
+        for z in range(8):
+            sl = slice(z, z + 1)
+            B = oracle.Batch(ref["guess"][:, sl], ref["geom"][:, sl], np.ones((p.nf, 1)),
+                             None if not emhd else ref["prim_old"][:, sl], None,
+                             None if not emhd else ref["closure"][:, sl])
+            per_zone.append(tuple(oracle.count_live_ops(p, B)))
+        assert len(set(per_zone)) == 1, per_zone
+        counts[emhd] = np.array(per_zone[0])
+    assert counts[True].sum() > counts[False].sum()
