@@ -213,6 +213,8 @@ SIGNATURES = {
     "grim_residual": (_i, [_P, _B, _vp, _vp]),
     "grim_jacobian": (_i, [_P, _B, _i, _vp, _vp]),
     "grim_batch_linear_solve": (_i, [_i64, _i, _vp, _vp, _vp, _i, _vp]),
+    "grim_residual_tangent": (_i, [_P, _B, _vp, _vp, _i, _vp]),
+    "grim_residual_adjoint": (_i, [_P, _B, _vp, _vp, _i, _vp]),
     "grim_batched_newton": (_i, [_P, _B, C.POINTER(NewtonOpts), C.POINTER(NewtonStats), _vp]),
     "grim_count_above": (_i, [_i64, _vp, _d, C.POINTER(C.c_int64), _vp]),
     "grim_generate_kerr_schild_batch": (_i, [_P, _i64, C.c_uint64, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -240,6 +242,7 @@ SIGNATURES = {
     "grim_init_linear_mode": (_i, [_vp, _i, _d]),
     "grim_init_torus": (_i, [_vp, C.POINTER(TorusParams)]),
     "grim_apply_floor": (_i, [_vp, _vp]),
+    "grim_full_step_diagnostics": (_i, [_vp, _vp]),
     "grim_torus_bcs": (_i, [_vp, _vp]),
     "grim_torus_observers": (_i, [_vp, _vp, _vp]),
     "grim_dump_vars": (_i, [_vp, _vp, _vp, _i]),
@@ -353,6 +356,22 @@ def residual(p: Params, batch: ZoneBatch, out, stream=None):
 def jacobian(p: Params, batch: ZoneBatch, kind: int, out, stream=None):
     check(lib().grim_jacobian(C.byref(p), C.byref(batch), kind, _ptr(out, numel=p.nf * p.nf * batch.n),
                               _stream(stream)), "grim_jacobian")
+    return out
+
+
+def residual_tangent(p: Params, batch: ZoneBatch, v, out, precision=LU_FP64, stream=None):
+    """out = J v per zone (analytic tangent of the residual at batch.prim)."""
+    n = batch.n
+    check(lib().grim_residual_tangent(C.byref(p), C.byref(batch), _ptr(v, numel=p.nf * n), _ptr(out, numel=p.nf * n),
+                                      precision, _stream(stream)), "grim_residual_tangent")
+    return out
+
+
+def residual_adjoint(p: Params, batch: ZoneBatch, w, out, precision=LU_FP64, stream=None):
+    """out = J^T w per zone (hand-written reverse sweep of the residual at batch.prim)."""
+    n = batch.n
+    check(lib().grim_residual_adjoint(C.byref(p), C.byref(batch), _ptr(w, numel=p.nf * n), _ptr(out, numel=p.nf * n),
+                                      precision, _stream(stream)), "grim_residual_adjoint")
     return out
 
 

@@ -299,6 +299,223 @@ __device__ __forceinline__ void analyticJacobian(const double* Pd, const ZoneGeo
   }
 }
 
+// Adjoint of the same residual (reverse mode, by hand): g = J^T w for one
+// zone without forming J.  With W_nu = (w_U, w_U1, w_U2, w_U3) the weighted
+// sum of the residual rows is
+//   w.R = gs [ (w_rho + w_U) rho u^0 + W^nu T^0_nu ] + w_Q R_Q + w_DP R_DP,
+//   W^nu T^0_nu = wt u^0 (W.u) + (p + b^2/2) W_0 - b^0 (W.b)
+//               + q ib (u^0 (W.b) + b^0 (W.u)) - dP (b^0 (W.b)/b^2 - (W_0 + u^0 (W.u))/3)
+// (wt = rho + u + p + b^2, W.u = W^nu u_nu, W.b = W^nu b_nu), and
+//   R_Q  = sQ  [g u^0 qTilde/dt + g qTilde/(2 tau_c) + wQT T + wQ^mu u_mu] + const,
+//   R_DP = sDP [g u^0 dPTilde/dt + g dPTilde/(2 tau_c) + wDP^mu u_mu] + const.
+// The sweep below runs the primal of analyticJacobian forward, then carries
+// the adjoints (the _b names) of every intermediate back to the primitives
+// in reverse order of their computation; floors follow the tangent's
+// one-sided rule.  <J v, w> = <v, J^T w> to rounding, for any v and w.
+template <class T, class L>
+__device__ __forceinline__ void analyticAdjoint(const double* Pd, const ZoneGeom& Gd,
+                                                const DevParams& prm, double scale,
+                                                const ZoneConst<L>& c, const EmhdWeights& ew,
+                                                double dt, const T (&wr)[L::NF], T (&out)[L::NF]) {
+  T gc[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) gc[m][k] = (T)Gd.cov(m, k);
+  const T alpha = (T)Gd.alpha, g = (T)Gd.g;
+  const T sRho = (Pd[RHO] >= prm.rhoFloor) ? T(1) : T(0);
+  const T sU = (Pd[UU] >= prm.uFloor) ? T(1) : T(0);
+  const T rho = (T)smax(Pd[RHO], prm.rhoFloor);
+  const T u = (T)smax(Pd[UU], prm.uFloor);
+  const T gam = (T)prm.gam;
+  const T p = (gam - T(1)) * u;
+  const T ut[3] = {(T)Pd[U1], (T)Pd[U2], (T)Pd[U3]};
+  const T B[3] = {(T)Pd[L::B1], (T)Pd[L::B2], (T)Pd[L::B3]};
+
+  // ---- primal (as analyticJacobian)
+  T utLow[3], BLow[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    utLow[i] = gc[i + 1][1] * ut[0] + gc[i + 1][2] * ut[1] + gc[i + 1][3] * ut[2];
+    BLow[i] = gc[i + 1][1] * B[0] + gc[i + 1][2] * B[1] + gc[i + 1][3] * B[2];
+  }
+  const T lor = sqrt(T(1) + ut[0] * utLow[0] + ut[1] * utLow[1] + ut[2] * utLow[2]);
+  const T invLor = T(1) / lor;
+  const T invAlpha = T(1) / alpha;
+  const T u0 = lor * invAlpha;
+  const T invU0 = alpha * invLor;
+  T uCov[4];
+  uCov[0] = -alpha * lor + gc[0][1] * ut[0] + gc[0][2] * ut[1] + gc[0][3] * ut[2];
+#pragma unroll
+  for (int i = 0; i < 3; i++) uCov[i + 1] = utLow[i];
+  const T b0 = B[0] * utLow[0] + B[1] * utLow[1] + B[2] * utLow[2];
+  T bCov[4];
+  bCov[0] = (gc[0][1] * B[0] + gc[0][2] * B[1] + gc[0][3] * B[2] + b0 * uCov[0]) * invU0;
+#pragma unroll
+  for (int i = 0; i < 3; i++) bCov[i + 1] = (BLow[i] + b0 * utLow[i]) * invU0;
+  const T BB = B[0] * BLow[0] + B[1] * BLow[1] + B[2] * BLow[2];
+  const T bb = (BB + b0 * b0) * invU0 * invU0;
+  const T bsq = bb + (T)prm.bSqrFloor;
+  const T wt = rho + u + p + bsq;
+  const T gs = g * (T)scale;
+
+  T qt = T(0), dpt = T(0), Kq = T(1), Kd = T(1), q = T(0), dP = T(0);
+  T dlnKq_rho = T(0), dlnKq_u = T(0), dlnKd_rho = T(0), dlnKd_u = T(0);
+  T ib = T(0), temp = T(0), sT = T(0);
+  if constexpr (L::EMHD) {
+    const T rawT = p / rho;
+    sT = (rawT > (T)prm.tFloor) ? T(1) : T(0);
+    temp = (rawT > (T)prm.tFloor) ? rawT : (T)prm.tFloor;
+    const T cs2 = gam * p / (rho + gam * u);
+    const T invRho = T(1) / rho;
+    const T invRG = T(1) / (rho + gam * u);
+    const T invU = T(1) / u;
+    ib = T(1) / sqrt(bsq);
+    if constexpr (L::COND_ON) {
+      const T aC = (T)((prm.closure == GRIM_CLOSURE_PER_ZONE) ? c.alphaC : prm.condAlpha);
+      qt = (T)Pd[L::Q];
+      if (prm.highOrderCond) {
+        Kq = temp * sqrt(rho * aC * cs2);
+        dlnKq_rho = -sT * invRho + T(0.5) * (invRho - invRG);
+        dlnKq_u = sT * invU + T(0.5) * rho * invU * invRG;
+      }
+      q = qt * Kq;
+    }
+    if constexpr (L::VISC_ON) {
+      const T aV = (T)((prm.closure == GRIM_CLOSURE_PER_ZONE) ? c.alphaV : prm.viscAlpha);
+      dpt = (T)Pd[L::DP];
+      if (prm.highOrderVisc) {
+        Kd = sqrt(temp * rho * aV * cs2);
+        dlnKd_rho = T(0.5) * (-sT * invRho + invRho - invRG);
+        dlnKd_u = T(0.5) * (sT * invU + rho * invU * invRG);
+      }
+      dP = dpt * Kd;
+    }
+  }
+  const T W[4] = {wr[UU], wr[U1], wr[U2], wr[U3]};
+  const T Wu = W[0] * uCov[0] + W[1] * uCov[1] + W[2] * uCov[2] + W[3] * uCov[3];
+  const T Wb = W[0] * bCov[0] + W[1] * bCov[1] + W[2] * bCov[2] + W[3] * bCov[3];
+
+  // ---- reverse sweep
+  T rho_b = T(0), u_b = T(0), u0_b = T(0), Wu_b = T(0), Wb_b = T(0), bsq_b = T(0), b0_b = T(0);
+  T qt_b = T(0), dpt_b = T(0);
+  T uCov_b[4] = {T(0), T(0), T(0), T(0)};
+  // ideal stress rows, weighted by gs
+  const T wRhoU = wr[RHO] + wr[UU];
+  rho_b += gs * wRhoU * u0;
+  u0_b += gs * (wRhoU * rho + wt * Wu);
+  const T wt_b = gs * u0 * Wu;
+  Wu_b += gs * wt * u0;
+  T p_b = gs * W[0];
+  bsq_b += gs * T(0.5) * W[0];
+  b0_b -= gs * Wb;
+  Wb_b -= gs * b0;
+  if constexpr (L::EMHD) {
+    // E1 = q ib (u0 Wb + b0 Wu)
+    const T e1 = u0 * Wb + b0 * Wu;
+    const T q_b = gs * ib * e1;
+    const T ib_b = gs * q * e1;
+    u0_b += gs * q * ib * Wb;
+    Wb_b += gs * q * ib * u0;
+    b0_b += gs * q * ib * Wu;
+    Wu_b += gs * q * ib * b0;
+    // E2 = -dP (b0 Wb / bsq - (W0 + u0 Wu)/3)
+    const T invBsq = T(1) / bsq;
+    const T dP_b = -gs * (b0 * Wb * invBsq - T(1. / 3.) * (W[0] + u0 * Wu));
+    b0_b -= gs * dP * Wb * invBsq;
+    Wb_b -= gs * dP * b0 * invBsq;
+    bsq_b += gs * dP * b0 * Wb * invBsq * invBsq;
+    u0_b += gs * dP * Wu * T(1. / 3.);
+    Wu_b += gs * dP * u0 * T(1. / 3.);
+    // EMHD rows
+    const T invDt = T(1) / (T)dt;
+    const T diag = g * u0 * invDt + T(0.5) * (T)ew.gOverTau;
+    T temp_b = T(0);
+    if constexpr (L::COND_ON) {
+      const T cQ = wr[L::Q] * (T)ew.sQ;
+      qt_b += cQ * diag;
+      u0_b += cQ * g * qt * invDt;
+      temp_b += cQ * (T)ew.wQT;
+#pragma unroll
+      for (int mu = 0; mu < 4; mu++) uCov_b[mu] += cQ * (T)ew.wQ[mu];
+    }
+    if constexpr (L::VISC_ON) {
+      const T cD = wr[L::DP] * (T)ew.sDP;
+      dpt_b += cD * diag;
+      u0_b += cD * g * dpt * invDt;
+#pragma unroll
+      for (int mu = 0; mu < 4; mu++) uCov_b[mu] += cD * (T)ew.wDP[mu];
+    }
+    // q = qTilde K_q(rho, u), dP = dPTilde K_d(rho, u), T = max(p/rho, floor)
+    if constexpr (L::COND_ON) {
+      qt_b += q_b * Kq;
+      rho_b += q_b * q * dlnKq_rho;
+      u_b += q_b * q * dlnKq_u;
+    }
+    if constexpr (L::VISC_ON) {
+      dpt_b += dP_b * Kd;
+      rho_b += dP_b * dP * dlnKd_rho;
+      u_b += dP_b * dP * dlnKd_u;
+    }
+    rho_b -= temp_b * sT * temp / rho;
+    u_b += temp_b * sT * temp / u;
+    // ib = bsq^(-1/2)
+    bsq_b += ib_b * T(-0.5) * ib * ib * ib;
+  }
+  // wt = rho + u + p + bsq, p = (Gamma - 1) u
+  rho_b += wt_b;
+  u_b += wt_b;
+  p_b += wt_b;
+  bsq_b += wt_b;
+  u_b += p_b * (gam - T(1));
+  // W.u and W.b
+  T bCov_b[4];
+#pragma unroll
+  for (int nu = 0; nu < 4; nu++) {
+    uCov_b[nu] += Wu_b * W[nu];
+    bCov_b[nu] = Wb_b * W[nu];
+  }
+  // bsq = (BB + b0^2) / u0^2 + floor
+  b0_b += bsq_b * T(2) * b0 * invU0 * invU0;
+  u0_b -= bsq_b * T(2) * bb * invU0;
+  // b_nu = (B_nu + b0 u_nu) / u0
+  T utLow_b[3] = {T(0), T(0), T(0)};
+  b0_b += bCov_b[0] * uCov[0] * invU0;
+  uCov_b[0] += bCov_b[0] * b0 * invU0;
+  u0_b -= bCov_b[0] * bCov[0] * invU0;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    b0_b += bCov_b[i + 1] * utLow[i] * invU0;
+    utLow_b[i] += bCov_b[i + 1] * b0 * invU0;
+    u0_b -= bCov_b[i + 1] * bCov[i + 1] * invU0;
+  }
+  // b0 = B^i utLow_i
+#pragma unroll
+  for (int i = 0; i < 3; i++) utLow_b[i] += b0_b * B[i];
+  // u_0 = -alpha lor + g_0j utilde^j,  u_i = utLow_i,  u^0 = lor / alpha
+  T lor_b = u0_b * invAlpha - alpha * uCov_b[0];
+  T ut_b[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    utLow_b[j] += uCov_b[j + 1];
+    ut_b[j] = uCov_b[0] * gc[0][j + 1];
+  }
+  // lor = sqrt(1 + utilde^i gamma_ij utilde^j): dlor/dutilde^j = utLow_j / lor
+#pragma unroll
+  for (int j = 0; j < 3; j++) ut_b[j] += lor_b * utLow[j] * invLor;
+  // utLow_i = gamma_ij utilde^j
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+    ut_b[j] += utLow_b[0] * gc[1][j + 1] + utLow_b[1] * gc[2][j + 1] + utLow_b[2] * gc[3][j + 1];
+
+  out[RHO] = sRho * rho_b;
+  out[UU] = sU * u_b;
+#pragma unroll
+  for (int j = 0; j < 3; j++) out[U1 + j] = ut_b[j];
+  if constexpr (L::COND_ON) out[L::Q] = qt_b;
+  if constexpr (L::VISC_ON) out[L::DP] = dpt_b;
+}
+
 // Ideal-MHD entry kept for the kernels that carry no EMHD constants.
 template <class T, class L>
 __device__ __forceinline__ void analyticJacobianIdeal(const double* Pd, const ZoneGeom& Gd,

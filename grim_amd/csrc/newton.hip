@@ -189,6 +189,46 @@ __global__ __launch_bounds__(kDefaultBlock) void analyticJacobianKernel(DevParam
     for (int j = 0; j < L::NF; j++) J[(i * L::NF + j) * n + z] = A[i][j];
 }
 
+// Matrix-free products of the analytic derivative of the residual, in the
+// arithmetic of T: the tangent J v (the column-wise tangent contracted with
+// v in registers) and the adjoint J^T w (the reverse sweep, J never formed).
+template <class L, class T, bool ADJOINT>
+__global__ __launch_bounds__(kDefaultBlock) void derivativeProductKernel(DevParams prm, grim_zone_batch b,
+                                                                        const double* __restrict__ x,
+                                                                        double* __restrict__ out) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= b.n) return;
+  const int64_t n = b.n;
+  const double dt = batchDt(b);
+  ZoneGeom G;
+  loadGeom(b.geom, n, z, G);
+  ZoneConst<L> c;
+  loadZoneConst<L>(b, prm, G, z, c);
+  EmhdWeights ew;
+  if constexpr (L::EMHD) emhdWeights<L>(G, c, prm, dt, ew);
+  double P[L::DOF];
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) P[v] = b.prim[v * n + z];
+  T xs[L::NF], y[L::NF];
+#pragma unroll
+  for (int v = 0; v < L::NF; v++) xs[v] = (T)x[v * n + z];
+  if constexpr (ADJOINT) {
+    analyticAdjoint<T, L>(P, G, prm, c.mask / dt, c, ew, dt, xs, y);
+  } else {
+    T A[L::NF][L::NF];
+    analyticJacobian<T, L>(P, G, prm, c.mask / dt, c, ew, dt, A);
+#pragma unroll
+    for (int i = 0; i < L::NF; i++) {
+      T s = T(0);
+#pragma unroll
+      for (int j = 0; j < L::NF; j++) s += A[i][j] * xs[j];
+      y[i] = s;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < L::NF; v++) out[v * n + z] = (double)y[v];
+}
+
 template <class Real, int N>
 __global__ __launch_bounds__(kDefaultBlock) void batchLinearSolveKernel(int64_t n, const double* A,
                                                                        const double* bvec,
@@ -292,6 +332,41 @@ int grim_jacobian(const grim_params* p, const grim_zone_batch* b, int jacobian, 
     }
     return checkLaunch("grim_jacobian");
   });
+}
+
+namespace {
+int derivativeProduct(const char* what, bool adjoint, const grim_params* p, const grim_zone_batch* b,
+                      const double* x, double* out, int precision, grim_stream_t stream) {
+  GRIM_REQUIRE(validBatch(p, b) && (b->n == 0 || (x && out)), std::string(what) + ": bad arguments");
+  GRIM_REQUIRE(precision == GRIM_LU_FP64 || precision == GRIM_LU_FP32, std::string(what) + ": bad precision");
+  if (b->n == 0) return GRIM_OK;
+  if (int rc = checkDevice()) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const DevParams prm = makeDevParams(*p);
+  const grim_zone_batch bb = *b;
+  const unsigned grid = gridFor(bb.n, kDefaultBlock);
+  return dispatchLayout(*p, [&]<class L>() -> int {
+    if (adjoint && precision == GRIM_LU_FP64)
+      derivativeProductKernel<L, double, true><<<grid, kDefaultBlock, 0, s>>>(prm, bb, x, out);
+    else if (adjoint)
+      derivativeProductKernel<L, float, true><<<grid, kDefaultBlock, 0, s>>>(prm, bb, x, out);
+    else if (precision == GRIM_LU_FP64)
+      derivativeProductKernel<L, double, false><<<grid, kDefaultBlock, 0, s>>>(prm, bb, x, out);
+    else
+      derivativeProductKernel<L, float, false><<<grid, kDefaultBlock, 0, s>>>(prm, bb, x, out);
+    return checkLaunch(what);
+  });
+}
+}  // namespace
+
+int grim_residual_tangent(const grim_params* p, const grim_zone_batch* b, const double* v, double* Jv,
+                          int precision, grim_stream_t stream) {
+  return derivativeProduct("grim_residual_tangent", false, p, b, v, Jv, precision, stream);
+}
+
+int grim_residual_adjoint(const grim_params* p, const grim_zone_batch* b, const double* w, double* JTw,
+                          int precision, grim_stream_t stream) {
+  return derivativeProduct("grim_residual_adjoint", true, p, b, w, JTw, precision, stream);
 }
 
 int grim_batch_linear_solve(int64_t n, int nvar, const double* A, const double* bvec, double* x,

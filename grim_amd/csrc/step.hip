@@ -2138,6 +2138,14 @@ int grim_apply_floor(grim_ctx* x, double* prim) {
   return withLayout(x, [&]<class L>() -> int { return applyFloor<L>(x, prim); });
 }
 
+int grim_full_step_diagnostics(grim_ctx* x, double* prim) {
+  GRIM_REQUIRE(x && prim && x->torus, "grim_full_step_diagnostics: no torus (grim_init_torus) or null prim");
+  return withLayout(x, [&]<class L>() -> int {
+    if (int rc = scrubNaNs(x, prim, L::DOF)) return rc;
+    return applyFloor<L>(x, prim);
+  });
+}
+
 int grim_torus_bcs(grim_ctx* x, double* prim) {
   GRIM_REQUIRE(x && prim && x->torus, "grim_torus_bcs: no torus (grim_init_torus) or null prim");
   return withLayout(x, [&]<class L>() -> int { return torusBCs<L>(x, prim); });

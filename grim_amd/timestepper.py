@@ -123,6 +123,13 @@ class TimeStepper:
         capi.check(capi.lib().grim_apply_floor(self._ctx, self._grid_ptr(prim, self.dof)), "grim_apply_floor")
         return prim
 
+    def fullStepDiagnostics(self, prim):
+        """The NaN scrub and applyFloor of the torus's fullStepDiagnostics
+        (torus/problem.cpp:719-746) on device primitives, in place."""
+        capi.check(capi.lib().grim_full_step_diagnostics(self._ctx, self._grid_ptr(prim, self.dof)),
+                   "grim_full_step_diagnostics")
+        return prim
+
     def setProblemSpecificBCs(self, prim):
         capi.check(capi.lib().grim_torus_bcs(self._ctx, self._grid_ptr(prim, self.dof)), "grim_torus_bcs")
         return prim

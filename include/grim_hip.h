@@ -163,6 +163,20 @@ int grim_residual(const grim_params* p, const grim_zone_batch* b, double* R,
 int grim_jacobian(const grim_params* p, const grim_zone_batch* b, int jacobian,
                   double* J, grim_stream_t stream);
 
+/* Matrix-free products with the analytic derivative J = dR/dP of the
+ * per-zone residual at batch->prim (numFluidVars planes in and out), in
+ * FP64 (precision = GRIM_LU_FP64) or FP32 (GRIM_LU_FP32) arithmetic.  They
+ * replace grim's finite-difference Jacobian (src/timestepper/solve.cpp:82-109,
+ * one residual evaluation of src/timestepper/residual.cpp:3-180 per
+ * primitive):
+ *  grim_residual_tangent: Jv_z = J_z v_z (the hand-written tangent);
+ *  grim_residual_adjoint: JTw_z = J_z^T w_z (the hand-written reverse sweep,
+ *    J never formed).  <J v, w> = <v, J^T w> to rounding. */
+int grim_residual_tangent(const grim_params* p, const grim_zone_batch* b, const double* v, double* Jv,
+                          int precision, grim_stream_t stream);
+int grim_residual_adjoint(const grim_params* p, const grim_zone_batch* b, const double* w, double* JTw,
+                          int precision, grim_stream_t stream);
+
 /* timeStepper::batchLinearSolve (src/timestepper/solve.cpp:203-230): solves
  * A_z x_z = b_z for every zone, A planes i*nvar+j, partial pivoting, in FP64
  * or FP32 arithmetic. */
@@ -356,6 +370,13 @@ typedef struct grim_torus_params {
 int grim_init_torus(grim_ctx* ctx, const grim_torus_params* t);
 /* applyFloor (problem.cpp:542-712) on caller-provided primitive planes. */
 int grim_apply_floor(grim_ctx* ctx, double* prim);
+/* The state hooks of the torus's fullStepDiagnostics (problem.cpp:719-746)
+ * on caller-provided primitive planes, as the full stage of grim_time_step
+ * runs them: when a bulk zone holds a NaN in any variable, every zone
+ * holding one (ghosts included) is set to zero in all variables; then
+ * applyFloor.  The observers and dumps of the same hook are
+ * grim_torus_observers and grim_dump_vars. */
+int grim_full_step_diagnostics(grim_ctx* ctx, double* prim);
 /* setProblemSpecificBCs of the torus on caller-provided primitive planes. */
 int grim_torus_bcs(grim_ctx* ctx, double* prim);
 

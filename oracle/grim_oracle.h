@@ -294,6 +294,11 @@ void grim_oracle_apply_floor(const grim_oracle_grid* c, const grim_oracle_params
                              const grim_oracle_torus* t, const grim_oracle_geometry* gm,
                              double* prim);
 /* setProblemSpecificBCs: inflowCheck then fixPoles (problem.cpp:941-1240). */
+/* fullStepDiagnostics' state hooks (problem.cpp:719-746): the NaN scrub,
+ * then applyFloor. */
+void grim_oracle_full_step_diagnostics(const grim_oracle_grid* c, const grim_oracle_params* p,
+                                       const grim_oracle_torus* t, const grim_oracle_geometry* gm,
+                                       double* prim);
 void grim_oracle_torus_bcs(const grim_oracle_grid* c, const grim_oracle_params* p,
                            const grim_oracle_torus* t, const grim_oracle_geometry* gm,
                            double* prim);

@@ -1533,6 +1533,15 @@ void grim_oracle_apply_floor(const grim_oracle_grid* c, const grim_oracle_params
   applyFloorAll(*c, *p, *t, D, *gm, prim);
 }
 
+void grim_oracle_full_step_diagnostics(const grim_oracle_grid* c, const grim_oracle_params* p,
+                                       const grim_oracle_torus* t, const grim_oracle_geometry* gm,
+                                       double* prim) {
+  const Dims D(*c);
+  const Vars vars(*p);
+  scrubNaNs(D, vars.dof, prim);
+  applyFloorAll(*c, *p, *t, D, *gm, prim);
+}
+
 void grim_oracle_torus_bcs(const grim_oracle_grid* c, const grim_oracle_params* p,
                            const grim_oracle_torus* t, const grim_oracle_geometry* gm,
                            double* prim) {

@@ -193,6 +193,7 @@ def lib():
         L.grim_oracle_torus_ic.argtypes = [Gp, P, Tp, Gm, vp]
         L.grim_oracle_apply_floor.argtypes = [Gp, P, Tp, Gm, vp]
         L.grim_oracle_torus_bcs.argtypes = [Gp, P, Tp, Gm, vp]
+        L.grim_oracle_full_step_diagnostics.argtypes = [Gp, P, Tp, Gm, vp]
         L.grim_oracle_torus_observers.argtypes = [Gp, P, Gm, vp, vp]
         L.grim_oracle_dump_vars.argtypes = [Gp, P, Gm, vp, vp]
         L.grim_oracle_set_ghost_state.argtypes = [vp]
@@ -545,6 +546,13 @@ def torus_ic(g: Grid, p: Params, t: Torus, gm: Geometry):
 def apply_floor(g: Grid, p: Params, t: Torus, gm: Geometry, prim):
     prim = _f64(prim).copy()
     lib().grim_oracle_apply_floor(C.byref(g), C.byref(p), C.byref(t), C.byref(gm._c()), _ptr(prim))
+    return prim
+
+
+def full_step_diagnostics(g: Grid, p: Params, t: Torus, gm: Geometry, prim):
+    """fullStepDiagnostics' NaN scrub and applyFloor (torus/problem.cpp:719-746)."""
+    prim = _f64(prim).copy()
+    lib().grim_oracle_full_step_diagnostics(C.byref(g), C.byref(p), C.byref(t), C.byref(gm._c()), _ptr(prim))
     return prim
 
 

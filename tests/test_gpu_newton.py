@@ -385,3 +385,60 @@ def test_graph_captured_step_matches_eager(capi, torch_cuda):
         step()
         torch.cuda.synchronize()
         assert torch.equal(prim, eager[0])
+
+
+# ------------------------------------------------ tangent and adjoint products
+@pytest.mark.parametrize("emhd", [False, True])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_adjoint_dot_product_and_dual_numbers(capi, torch_cuda, oracle, emhd, precision):
+    """The reverse sweep J^T w against the column-wise tangent J v (dot-product
+    test <J v, w> = <v, J^T w> per zone) and against the oracle's
+    forward-mode dual-number Jacobian, transposed."""
+    torch = torch_cuda
+    n = 16384
+    p, b, rhs, mask = make_problem(capi, torch, n, emhd)
+    prec = capi.LU_FP64 if precision == "fp64" else capi.LU_FP32
+    batch = capi.make_batch(b["guess"], b["geom"], rhs, b["prim_old"], None, b["closure"], residual_mask=mask)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(SEED)
+    # v in the primitives' own scale, w in the residual rows' scale
+    v = (torch.rand((p.nf, n), generator=gen, device="cuda", dtype=torch.float64) - 0.5) * b["guess"][:p.nf].abs()
+    w = torch.rand((p.nf, n), generator=gen, device="cuda", dtype=torch.float64) - 0.5
+    Jv = torch.empty_like(v)
+    JTw = torch.empty_like(w)
+    capi.residual_tangent(p, batch, v, Jv, prec)
+    capi.residual_adjoint(p, batch, w, JTw, prec)
+    torch.cuda.synchronize()
+    lhs = (Jv * w).sum(dim=0)
+    rhs_ = (v * JTw).sum(dim=0)
+    # scale of the sums: the largest single product term of either side
+    J = torch.empty((p.nf * p.nf, n), dtype=torch.float64, device="cuda")
+    capi.jacobian(p, batch, capi.JAC_ANALYTIC, J)
+    torch.cuda.synchronize()
+    Jm = host(J).reshape(p.nf, p.nf, n)
+    terms = np.abs(Jm) * np.abs(host(w))[:, None, :] * np.abs(host(v))[None, :, :]
+    scale = terms.sum(axis=(0, 1)) + 1e-300
+    dot_err = np.abs(host(lhs) - host(rhs_)) / scale
+    # J^T w against the dual-number Jacobian of the oracle's residual
+    q = oracle_params(oracle, p)
+    ref = oracle.dual_jacobian(q, oracle.Batch(host(b["guess"]), host(b["geom"]), host(rhs), host(b["prim_old"]),
+                                               None, host(b["closure"]), None, 1.0,
+                                               host(mask))).reshape(p.nf, p.nf, n)
+    ref_JTw = np.einsum("ijz,iz->jz", ref, host(w))
+    colscale = (np.abs(ref) * np.abs(host(w))[:, None, :]).sum(axis=0) + 1e-300
+    adj_err = (np.abs(host(JTw) - ref_JTw) / colscale).max(axis=0)
+    print("%s emhd=%s: dot-product relative error median %.2e p99 %.2e max %.2e; J^T w vs dual median %.2e "
+          "p99 %.2e max %.2e" % (precision, emhd, np.median(dot_err), np.percentile(dot_err, 99), dot_err.max(),
+                                 np.median(adj_err), np.percentile(adj_err, 99), adj_err.max()))
+    assert np.isfinite(host(JTw)).all()
+    if precision == "fp64":
+        assert dot_err.max() < 1e-12
+        if not emhd:
+            assert adj_err.max() < 1e-9
+        else:
+            # near-floor zones lose digits to cancellation in either evaluation
+            assert np.median(adj_err) < 1e-12 and np.percentile(adj_err, 99) < 1e-7
+    else:
+        # FP32 sweep: rounding of single precision over a few hundred operations
+        assert np.percentile(dot_err, 99) < 1e-4
+        assert np.median(adj_err) < 1e-5

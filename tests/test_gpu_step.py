@@ -427,6 +427,50 @@ def test_torus_hooks_bitwise(torch):
     assert_bitwise(got, O.torus_bcs(g, po, t, gm, prim), "torus boundaries")
 
 
+@pytest.mark.parametrize("where", ["ghosts_only", "bulk_and_ghosts"])
+def test_nan_scrub_matches_oracle(torch, where):
+    """fullStepDiagnostics' NaN scrub (torus/problem.cpp:722-741) and floors
+    on a torus state with NaNs planted in single variables of some zones.
+    grim counts NaNs over the bulk only, but then zeroes every zone holding
+    one: NaNs in ghost zones alone stay; with one in the bulk, every NaN
+    zone, ghosts included, becomes the floored zero state."""
+    ts, g, p, po, gm, t, floors = torus_case(False)
+    rng = np.random.default_rng(11)
+    prim = ts.primOld
+    shape = g.shape
+    is_bulk = np.zeros(shape, dtype=bool)
+    is_bulk[g.bulk()] = True
+    is_bulk = is_bulk.ravel()
+    ghosts = np.flatnonzero(~is_bulk)
+    inside = np.flatnonzero(is_bulk)
+    planted = list(rng.choice(ghosts, 4, replace=False))
+    if where == "bulk_and_ghosts":
+        planted += list(rng.choice(inside, 6, replace=False))
+    for z in planted:
+        prim[rng.integers(0, po.dof), z] = np.nan
+    print(f"{where}: NaN in {len(planted)} zones", flush=True)
+    got = ts.fullStepDiagnostics(dev(torch, prim)).cpu().numpy()
+    ref = O.full_step_diagnostics(g, po, t, gm, prim)
+    floored = O.apply_floor(g, po, t, gm, prim)
+    nan_zone = np.isnan(prim).any(axis=0)
+    if where == "ghosts_only":
+        # no bulk NaN: the scrub does nothing and the floors act alone; the
+        # planted zones keep a NaN (how the floors' max/min treat it is not
+        # compared: ArrayFire's NaN convention is unpinned)
+        assert_bitwise(got[:, ~nan_zone], ref[:, ~nan_zone], "NaN scrub + floors")
+        assert_bitwise(got[:, ~nan_zone], floored[:, ~nan_zone], "floors only")
+        assert np.isnan(got[:, planted]).any(axis=0).all() and np.isnan(ref[:, planted]).any(axis=0).all()
+    else:
+        assert_bitwise(got, ref, "NaN scrub + floors")
+        zero_floored = O.apply_floor(g, po, t, gm, np.where(nan_zone, 0., prim))
+        assert not np.isnan(got).any()
+        assert_bitwise(got[:, nan_zone], zero_floored[:, nan_zone], "scrubbed zones")
+        assert_bitwise(got[:, ~nan_zone], floored[:, ~nan_zone], "untouched zones")
+        # the atmosphere a scrubbed zone takes: the floors, at rest, no field
+        assert (got[2:5, nan_zone] == 0).all() and (got[po.nf:, nan_zone] == 0).all()
+        assert (got[0, nan_zone] > 0).all() and (got[1, nan_zone] > 0).all()
+
+
 def test_torus_ideal_steps_bitwise(torch):
     from grim_amd import capi
 

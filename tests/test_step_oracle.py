@@ -548,6 +548,35 @@ def test_torus_floor_invariants():
     np.testing.assert_allclose(F2, F, rtol=1e-9, atol=1e-300)
 
 
+@pytest.mark.parametrize("where", ["ghosts_only", "bulk_and_ghosts"])
+def test_nan_scrub_rule(where):
+    """fullStepDiagnostics (torus/problem.cpp:722-741): the NaN count runs over
+    the bulk (domainX1..3) but the reset uses where() over the whole array;
+    zeroed zones then pass through applyFloor like every other zone."""
+    g, p, gm, t = torus_setup(False)
+    P = O.torus_ic(g, p, t, gm)
+    rng = np.random.default_rng(13)
+    is_bulk = np.zeros(g.shape, dtype=bool)
+    is_bulk[g.bulk()] = True
+    is_bulk = is_bulk.ravel()
+    zones = list(rng.choice(np.flatnonzero(~is_bulk), 3, replace=False))
+    if where == "bulk_and_ghosts":
+        zones += list(rng.choice(np.flatnonzero(is_bulk), 4, replace=False))
+    for z in zones:
+        P[rng.integers(0, p.dof), z] = np.nan
+    got = O.full_step_diagnostics(g, p, t, gm, P)
+    nan_zone = np.isnan(P).any(axis=0)
+    if where == "ghosts_only":
+        assert np.isnan(got[:, zones]).any(axis=0).all()
+        ok = ~nan_zone
+        np.testing.assert_array_equal(got[:, ok], O.apply_floor(g, p, t, gm, P)[:, ok])
+    else:
+        expect = O.apply_floor(g, p, t, gm, np.where(nan_zone, 0., P))
+        np.testing.assert_array_equal(got, expect)
+        assert not np.isnan(got).any()
+        assert (got[2:, nan_zone] == 0).all() and (got[:2, nan_zone] > 0).all()
+
+
 def test_torus_boundary_hook_identities():
     """fixPoles: the two active rows next to each pole copy the third (u^2,
     q, dP scaled by theta), ghosts mirror with u^2, B^2, q, dP flipped;
