@@ -216,6 +216,7 @@ SIGNATURES = {
     "grim_residual_tangent": (_i, [_P, _B, _vp, _vp, _i, _vp]),
     "grim_residual_adjoint": (_i, [_P, _B, _vp, _vp, _i, _vp]),
     "grim_batched_newton": (_i, [_P, _B, C.POINTER(NewtonOpts), C.POINTER(NewtonStats), _vp]),
+    "grim_batched_newton_noalloc": (_i, [_P, _B, C.POINTER(NewtonOpts), C.POINTER(NewtonStats), _vp]),
     "grim_count_above": (_i, [_i64, _vp, _d, C.POINTER(C.c_int64), _vp]),
     "grim_generate_kerr_schild_batch": (_i, [_P, _i64, C.c_uint64, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
     "grim_known_root_rhs": (_i, [_P, _B, _vp, _vp, _vp]),
@@ -416,6 +417,18 @@ def batched_newton(p: Params, batch: ZoneBatch, jacobian=JAC_FD, lu_precision=LU
     st = newton_stats(batch.n, iterations, residual_evals, final_l2)
     check(lib().grim_batched_newton(C.byref(p), C.byref(batch), C.byref(opts), C.byref(st), _stream(stream)),
           "grim_batched_newton")
+
+
+NOALLOC_MAX_ZONES = 8388608
+
+
+def batched_newton_noalloc(p: Params, batch: ZoneBatch, opts: NewtonOpts, iterations=None, residual_evals=None,
+                           final_l2=None, stream=None):
+    """grim_batched_newton_noalloc: the per-zone solve with the compaction work
+    list in the library's own device array (no allocation in the call)."""
+    st = newton_stats(batch.n, iterations, residual_evals, final_l2)
+    check(lib().grim_batched_newton_noalloc(C.byref(p), C.byref(batch), C.byref(opts), C.byref(st),
+                                            _stream(stream)), "grim_batched_newton_noalloc")
 
 
 def count_above(final_l2, tol, stream=None) -> int:

@@ -10,6 +10,9 @@
 // newton_fast.hip.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <mutex>
+
 #include "grim_hip.h"
 #include "newton_kernels.hpp"
 #include "runtime.hpp"
@@ -294,6 +297,39 @@ int launchGlobal(const grim_params& p, const grim_zone_batch& b, const grim_newt
 }
 
 }  // namespace
+
+// Scratch of the allocation-free solve: the compaction workspace of up to
+// GRIM_NOALLOC_MAX_ZONES zones, one copy per device for the life of the
+// process.  In host code `noAllocScratch` only names the symbol; its device
+// address comes from hipGetSymbolAddress, once per device.
+constexpr int64_t kNoAllocScratchBytes = compactionBytes(GRIM_NOALLOC_MAX_ZONES);
+static __device__ __align__(256) unsigned char noAllocScratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+std::atomic<void*> g_scratchPtr[kMaxDevices];
+
+// Device address of the current device's scratch (looked up on the first
+// call per device, with that device current).
+int noAllocScratchPtr(void** out) {
+  int dev = -1;
+  GRIM_HIP_TRY(hipGetDevice(&dev));
+  GRIM_REQUIRE(dev >= 0 && dev < kMaxDevices, "grim_batched_newton_noalloc: device ordinal out of range");
+  void* ptr = g_scratchPtr[dev].load(std::memory_order_acquire);
+  if (!ptr) {
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    ptr = g_scratchPtr[dev].load(std::memory_order_relaxed);
+    if (!ptr) {
+      GRIM_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(noAllocScratch)));
+      GRIM_REQUIRE(ptr != nullptr, "grim_batched_newton_noalloc: no device address for the scratch array");
+      g_scratchPtr[dev].store(ptr, std::memory_order_release);
+    }
+  }
+  *out = ptr;
+  return GRIM_OK;
+}
+}  // namespace
 }  // namespace grim
 
 using namespace grim;
@@ -482,6 +518,32 @@ int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
     }
   }
   return rc;
+}
+
+int grim_batched_newton_noalloc(const grim_params* p, const grim_zone_batch* b,
+                                const grim_newton_opts* opts, grim_newton_stats* stats,
+                                grim_stream_t stream) {
+  GRIM_REQUIRE(validBatch(p, b), "grim_batched_newton_noalloc: bad params or batch");
+  GRIM_REQUIRE(b->n <= GRIM_NOALLOC_MAX_ZONES,
+               "grim_batched_newton_noalloc: more than GRIM_NOALLOC_MAX_ZONES (2^23) zones");
+  grim_newton_opts o = {};
+  o.jacobian = GRIM_JAC_FD;
+  o.lu_precision = GRIM_LU_FP64;
+  o.semantics = GRIM_NEWTON_PER_ZONE;
+  if (opts) o = *opts;
+  GRIM_REQUIRE(o.semantics == GRIM_NEWTON_PER_ZONE,
+               "grim_batched_newton_noalloc: per-zone semantics only (the lock-step loop needs "
+               "per-zone state beyond the scratch array)");
+  GRIM_REQUIRE(o.workspace == nullptr && o.workspace_bytes == 0,
+               "grim_batched_newton_noalloc: takes no workspace (it uses the library's scratch array)");
+  if (b->n == 0) return GRIM_OK;
+  if (int rc = checkDevice()) return rc;
+  void* ws = nullptr;
+  if (int rc = noAllocScratchPtr(&ws)) return rc;
+  o.workspace = ws;
+  o.workspace_bytes = kNoAllocScratchBytes;
+  // the per-zone solve allocates nothing once the compaction workspace is given
+  return grim_batched_newton(p, b, &o, stats, stream);
 }
 
 }  // extern "C"

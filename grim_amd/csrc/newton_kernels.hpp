@@ -498,10 +498,10 @@ struct CompactionWorkspace {
   int* ctrl;                  // [0] list length, [1] grab counter
 };
 
-inline int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
+constexpr int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
 
 // Bytes of the workspace for n zones, and the carve-up of a buffer of that size.
-inline int64_t compactionBytes(int64_t n) {
+constexpr int64_t compactionBytes(int64_t n) {
   const int64_t nMasks = (n + 63) / 64, nScan = (nMasks + kScanBlock - 1) / kScanBlock;
   return align256(nMasks * 8) + align256(nMasks * 4) + align256(nScan * 4) + align256(n * 4) + 256;
 }

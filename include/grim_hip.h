@@ -190,6 +190,24 @@ int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
                         const grim_newton_opts* opts, grim_newton_stats* stats,
                         grim_stream_t stream);
 
+/* No counterpart in the reference.  grim_batched_newton for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): it calls no hipMalloc, hipFree or
+ * allocating library routine.  The compacting solve (compaction = 1, or
+ * GRIM_LU_FP32_FP64) keeps its work list in a library-owned device array of
+ * compaction workspace for GRIM_NOALLOC_MAX_ZONES zones (about 35 MB, one
+ * copy per device, held until the process ends).  Same options and results
+ * as grim_batched_newton, except: per-zone semantics only, opts->workspace
+ * must be NULL, and at most GRIM_NOALLOC_MAX_ZONES zones (GRIM_ERR_BAD_ARG
+ * otherwise).  Every call on one device shares that device's array: calls
+ * on one stream are ordered, calls that overlap on one device (other
+ * streams, other threads) race.  A call rewrites every part of the array it
+ * reads before reading it, so nothing left by an earlier call matters. */
+#define GRIM_NOALLOC_MAX_ZONES 8388608
+int grim_batched_newton_noalloc(const grim_params* p, const grim_zone_batch* b,
+                                const grim_newton_opts* opts, grim_newton_stats* stats,
+                                grim_stream_t stream);
+
 /* Project autotuner for the batched solve: times the candidate launch
  * configurations (workgroup size, compaction on/off, phase-1 budget,
  * phase-2 grid) on this batch -- on a scratch copy of batch->prim, so the
