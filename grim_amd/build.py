@@ -110,20 +110,32 @@ def build_io(verbose: bool = False) -> str:
 
 
 DRIVER = os.path.join(HERE, "grim_step")
+FACADE_CHECK = os.path.join(HERE, "facade_check")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 
-def build_driver(verbose: bool = False) -> str:
-    """The C++ driver grim_amd/grim_step (host code only: g++ against the C
-    ABI and the C++ interface grim_amd/host/timestepper.hpp)."""
-    tmp = DRIVER + ".tmp"
-    cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-Wall", "-Wextra",
-           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host"),
-           os.path.join(HERE, "host", "grim_main.cpp"), "-L", HERE, "-lgrim_hip", "-lgrim_io",
-           "-Wl,-rpath,$ORIGIN", "-o", tmp]
+def host_program(src: str, out: str, libs, verbose: bool = False) -> str:
+    """A host program over the C ABI and the C++ interfaces of grim_amd/host
+    (g++; the HIP runtime's host API for device memory)."""
+    tmp = out + ".tmp"
+    cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host"), "-isystem",
+           os.path.join(ROCM, "include"), os.path.join(HERE, "host", src), "-L", HERE, *libs,
+           "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib"),
+           "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, DRIVER)
+    os.replace(tmp, out)
+    return out
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The C++ driver grim_amd/grim_step and the interface check
+    grim_amd/facade_check (host code only: g++ against the C ABI and the C++
+    interfaces of grim_amd/host)."""
+    host_program("grim_main.cpp", DRIVER, ["-lgrim_hip", "-lgrim_io"], verbose)
+    host_program("facade_check.cpp", FACADE_CHECK, ["-lgrim_hip"], verbose)
     return DRIVER
 
 

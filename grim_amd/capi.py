@@ -163,6 +163,19 @@ class StepStats(C.Structure):
                 ("nonfinite", C.c_int64 * 2)]
 
 
+ELEMENT_FIELDS = ("rho", "u", "pressure", "temperature", "gammaLorentzFactor", "soundSpeed", "bSqr", "bNorm", "q",
+                  "deltaP", "tau", "chi_emhd", "nu_emhd", "uCon0", "uCon1", "uCon2", "uCon3", "uCov0", "uCov1",
+                  "uCov2", "uCov3", "bCon0", "bCon1", "bCon2", "bCon3", "bCov0", "bCov1", "bCov2", "bCov3")
+
+
+class StageArrays(C.Structure):
+    """grim_stage_arrays: device arrays of a context under grim's timeStepper names."""
+
+    _fields_ = [(name, C.c_void_p) for name in (
+        "primOld", "primHalfStep", "prim", "consOld", "sourcesExplicit", "sourcesImplicitOld", "rhs",
+        "residualMask", "fluxesX1", "fluxesX2", "fluxesX3")]
+
+
 class TorusObservables(C.Structure):
     """grim_torus_observables (ObserversTorus.hpp:6-251)."""
 
@@ -248,6 +261,20 @@ SIGNATURES = {
     "grim_torus_observers": (_i, [_vp, _vp, _vp]),
     "grim_dump_vars": (_i, [_vp, _vp, _vp, _i]),
     "grim_set_ghost_state": (_i, [_vp, _vp, _i]),
+    "grim_riemann_solve": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "grim_element_fluxes": (_i, [_vp, _vp, _i, _i, _vp]),
+    "grim_element_char_speeds": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
+    "grim_element_fields": (_i, [_vp, _vp, _i, _vp]),
+    "grim_element_implicit_sources": (_i, [_vp, _vp, _vp, _vp]),
+    "grim_element_time_deriv_sources": (_i, [_vp, _vp, _vp, _vp, _d, _vp]),
+    "grim_step_begin": (_i, [_vp]),
+    "grim_fold_rhs": (_i, [_vp, _i]),
+    "grim_solve": (_i, [_vp, _i, _vp, C.POINTER(StepStats)]),
+    "grim_stage_residual": (_i, [_vp, _i, _vp, _vp]),
+    "grim_stage_finish": (_i, [_vp, _i]),
+    "grim_step_end": (_i, [_vp]),
+    "grim_get_stage_arrays": (_i, [_vp, C.POINTER(StageArrays)]),
+    "grim_last_solve_stats": (_i, [_vp, _vp, _vp, _i]),
 }
 
 _lib = None

@@ -268,6 +268,12 @@ __global__ __launch_bounds__(kBlock) void reconstructKernel(Dims D, Box box, int
 // LDS.  The arithmetic follows grim's expression order exactly.
 constexpr int kFacesPerBlock = kBlock / 2;
 
+// RECON = GIVEN_STATES: the face values come ready-made, as
+// reconstruction::reconstruct leaves them (prim holds primRight on the left
+// side, i.e. the right face value of zone z - s, and primLeft on the right
+// side), which is riemannSolver::solve's own interface.
+constexpr int GIVEN_STATES = -1;
+
 template <class L, int DIR, int RECON, int SIDE>
 __device__ __forceinline__ void faceState(const StepArgs& a, int64_t z, const double* __restrict__ prim,
                                           const double* __restrict__ geomFace, double (&F)[L::DOF],
@@ -276,12 +282,16 @@ __device__ __forceinline__ void faceState(const StepArgs& a, int64_t z, const do
   double P[L::DOF];
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) {
-    double y[5];
+    if constexpr (RECON == GIVEN_STATES) {
+      P[v] = prim[v * n + z - (SIDE ? 0 : s)];
+    } else {
+      double y[5];
 #pragma unroll
-    for (int q = 0; q < 5; q++) y[q] = prim[v * n + z + (q - 3 + SIDE) * s];
-    double l, r;
-    reconstructZone<RECON>(y, a.theta, l, r);
-    P[v] = SIDE ? l : r;
+      for (int q = 0; q < 5; q++) y[q] = prim[v * n + z + (q - 3 + SIDE) * s];
+      double l, r;
+      reconstructZone<RECON>(y, a.theta, l, r);
+      P[v] = SIDE ? l : r;
+    }
   }
   GeomFull G;
   loadGeomFull(geomFace, n, z, G);
@@ -300,7 +310,7 @@ __device__ __forceinline__ void faceState(const StepArgs& a, int64_t z, const do
 template <class L, int DIR, int RECON>
 __global__ __launch_bounds__(kBlock) void faceFluxSplitKernel(
     StepArgs a, Box box, const double* __restrict__ prim, const double* __restrict__ geomFace,
-    double* __restrict__ flux) {
+    double* __restrict__ flux, const double* __restrict__ primRight = nullptr) {
   __shared__ double sh[2 * L::DOF + 2][kFacesPerBlock];
   const int side = threadIdx.x / kFacesPerBlock;
   const int lf = threadIdx.x % kFacesPerBlock;
@@ -315,7 +325,7 @@ __global__ __launch_bounds__(kBlock) void faceFluxSplitKernel(
     box.at(f, i, j, k);
     z = flatIndex(D, i, j, k);
     if (side == 0) {
-      faceState<L, DIR, RECON, 0>(a, z, prim, geomFace, F, U, mn, mx);
+      faceState<L, DIR, RECON, 0>(a, z, RECON == GIVEN_STATES ? primRight : prim, geomFace, F, U, mn, mx);
     } else {
       faceState<L, DIR, RECON, 1>(a, z, prim, geomFace, F, U, mn, mx);
 #pragma unroll
@@ -691,6 +701,111 @@ __global__ __launch_bounds__(kBlock) void sourcesKernel(StepArgs a, Box box,
   }
 #pragma unroll
   for (int v = 0; v < L::NF; v++) sExp[v * n + z] = src[v];
+}
+
+// ---------------------------------------------------------------------------
+// fluidElement's own outputs on a whole grid (physics.cpp:53-358)
+// ---------------------------------------------------------------------------
+enum { ELEM_FLUXES = 0, ELEM_SPEEDS = 1, ELEM_FIELDS = 2, ELEM_IMPLICIT = 3, ELEM_TIME_DERIV = 4 };
+
+// Every zone z of the grid, the element set from prim at the geometry of one
+// location (fluidElement::set, physics.cpp:53-183), then by `what`:
+//  ELEM_FLUXES      computeFluxes(dir) (physics.cpp:185-217), dof planes of out0
+//  ELEM_SPEEDS      computeMinMaxCharSpeeds(dir) (riemannsolver.cpp:54-200): out0, out1
+//  ELEM_FIELDS      the element's arrays (GRIM_ELEMENT_FIELDS planes of out0)
+//  ELEM_IMPLICIT    computeImplicitSources(sources, tauDamp = aux0) (physics.cpp:320-358)
+//  ELEM_TIME_DERIV  computeTimeDerivSources(elemOld = aux0, elemNew = aux1, dt)
+//                   (physics.cpp:219-318), this element the coefficient element
+template <class L>
+__global__ __launch_bounds__(kBlock) void elementKernel(StepArgs a, int what, int dir,
+                                                        const double* __restrict__ prim,
+                                                        const double* __restrict__ geom,
+                                                        const double* __restrict__ aux0,
+                                                        const double* __restrict__ aux1, double dt,
+                                                        double* __restrict__ out0, double* __restrict__ out1) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = a.D.n;
+  if (z >= n) return;
+  double P[L::DOF];
+#pragma unroll
+  for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
+  GeomFull G;
+  loadGeomFull(geom, n, z, G);
+  Kinematics kk;
+  Closure cc;
+  kinematics<L>(P, G.zg, a.prm, kk);
+  elementClosure<L>(P, kk, a.prm, G.r, cc);
+  const double g = G.zg.g;
+  if (what == ELEM_FLUXES) {
+    double F[L::DOF];
+    if (dir == 0) fluxes<L, 0>(P, kk, cc, g, F);
+    else if (dir == 1) fluxes<L, 1>(P, kk, cc, g, F);
+    else if (dir == 2) fluxes<L, 2>(P, kk, cc, g, F);
+    else fluxes<L, 3>(P, kk, cc, g, F);
+#pragma unroll
+    for (int v = 0; v < L::DOF; v++) out0[v * n + z] = F[v];
+  } else if (what == ELEM_SPEEDS) {
+    double mn, mx;
+    if (dir == 0) charSpeeds<L, 0>(kk, cc, G, a.prm, mn, mx);
+    else if (dir == 1) charSpeeds<L, 1>(kk, cc, G, a.prm, mn, mx);
+    else charSpeeds<L, 2>(kk, cc, G, a.prm, mn, mx);
+    out0[z] = mn;
+    out1[z] = mx;
+  } else if (what == ELEM_FIELDS) {
+    const double f[GRIM_ELEMENT_FIELDS] = {
+        kk.rho, kk.u, kk.pressure, kk.temperature, kk.gammaLF, cc.soundSpeed, kk.bSqr, cc.bNorm,
+        cc.q, cc.deltaP, cc.tau, cc.chi, cc.nu,
+        kk.uCon[0], kk.uCon[1], kk.uCon[2], kk.uCon[3], kk.uCov[0], kk.uCov[1], kk.uCov[2], kk.uCov[3],
+        kk.bCon[0], kk.bCon[1], kk.bCon[2], kk.bCon[3], kk.bCov[0], kk.bCov[1], kk.bCov[2], kk.bCov[3]};
+#pragma unroll
+    for (int q = 0; q < GRIM_ELEMENT_FIELDS; q++) out0[q * n + z] = f[q];
+  } else {
+    double S[L::DOF];
+#pragma unroll
+    for (int v = 0; v < L::DOF; v++) S[v] = 0.;
+    if constexpr (L::EMHD) {
+      if (what == ELEM_IMPLICIT) {
+        if constexpr (L::VISC_ON) S[L::DP] = g * (P[L::DP]) / aux0[z];
+        if constexpr (L::COND_ON) S[L::Q] = g * (P[L::Q]) / aux0[z];
+      } else {
+        // elemOld and elemNew at the same geometry
+        double Po[L::DOF], Pn[L::DOF];
+#pragma unroll
+        for (int v = 0; v < L::DOF; v++) {
+          Po[v] = aux0[v * n + z];
+          Pn[v] = aux1[v * n + z];
+        }
+        Kinematics ko, kn;
+        kinematics<L>(Po, G.zg, a.prm, ko);
+        kinematics<L>(Pn, G.zg, a.prm, kn);
+        double dtuCov[4];
+#pragma unroll
+        for (int mu = 0; mu < 4; mu++) dtuCov[mu] = (kn.uCov[mu] - ko.uCov[mu]) / dt;
+        double divuCov = 0.;
+#pragma unroll
+        for (int mu = 0; mu < 4; mu++) divuCov += G.zg.gcon0[mu] * dtuCov[mu];
+        if constexpr (L::VISC_ON) {
+          double dP0 = -divuCov * kk.rho * cc.nu;
+#pragma unroll
+          for (int mu = 0; mu < 4; mu++) dP0 += 3. * kk.rho * cc.nu * kk.bCon[0] * kk.bCon[mu] / kk.bSqr * dtuCov[mu];
+          if (a.prm.highOrderVisc) dP0 *= sqrt(cc.tau / kk.rho / cc.nu / kk.temperature);
+          S[L::DP] = -g * (dP0) / cc.tau;
+          if (a.prm.highOrderVisc) S[L::DP] -= 0.5 * g * divuCov * P[L::DP];
+        }
+        if constexpr (L::COND_ON) {
+          double q0 = -kk.rho * cc.chi * kk.bCon[0] / cc.bNorm * (kn.temperature - ko.temperature) / dt;
+#pragma unroll
+          for (int nu = 0; nu < 4; nu++)
+            q0 -= kk.rho * cc.chi * kk.temperature * kk.bCon[nu] / cc.bNorm * kk.uCon[0] * dtuCov[nu];
+          if (a.prm.highOrderCond) q0 *= sqrt(cc.tau / kk.rho / cc.chi) / kk.temperature;
+          S[L::Q] = -g * (q0) / cc.tau;
+          if (a.prm.highOrderCond) S[L::Q] -= 0.5 * g * divuCov * P[L::Q];
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < L::DOF; v++) out0[v * n + z] = S[v];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1484,6 +1599,16 @@ void launchFace(grim_ctx* x, const double* prim, double* flux, bool allFaces = t
   }
 }
 
+// riemannSolver::solve from reconstructed states (primLeft / primRight of
+// reconstruction::reconstruct) on the faces the fused launch covers.
+template <class L, int DIR>
+void launchFaceGiven(grim_ctx* x, const double* primLeft, const double* primRight, double* flux) {
+  const Box b = faceBox(x->D, DIR);
+  const double* gf = DIR == 0 ? x->geomX1 : (DIR == 1 ? x->geomX2 : x->geomC);
+  const unsigned nb = (unsigned)((b.count() + kFacesPerBlock - 1) / kFacesPerBlock);
+  faceFluxSplitKernel<L, DIR, GIVEN_STATES><<<nb, kBlock, 0, x->s>>>(x->a, b, primLeft, gf, flux, primRight);
+}
+
 // Riemann fluxes of every active direction into x->fluxes, then the EMFs.
 template <class L>
 int fluxesAndEmf(grim_ctx* x, const double* prim, bool allFaces) {
@@ -1700,17 +1825,29 @@ int torusBCs(grim_ctx* x, double* prim) {
 }
 
 // One stage of timeStep (timestep.cpp:14-107 half, 183-255 full).
+// The stage buffers have fixed roles (no pointer swaps), so that a captured
+// step reads and writes the same addresses on every replay: the half stage
+// advances primOld into primHalf, the full stage primHalf into primNew,
+// which the end of the step copies back into primOld.
+inline double* stageIn(grim_ctx* x, int st) { return st == 0 ? x->primOld : x->primHalf; }
+inline double* stageOut(grim_ctx* x, int st) { return st == 0 ? x->primHalf : x->primNew; }
+inline const double* stageDtPrime(grim_ctx* x, int st) {
+  return x->clock + (st == 0 ? CLOCK_HALF_DT : CLOCK_DT);
+}
+
+// Everything a stage does before its solve (timestep.cpp:15-89 / 184-237):
+// boundary conditions and the problem's hooks on the stage state, the
+// explicit and old implicit sources, the divergence of the fluxes with
+// flux-CT, the induction update of B, and the folded right-hand side
+// rhs = consOld - dt' (divF + S_expl + S_impl,old / 2) with the residual
+// mask.  The Newton guess (the stage state's fluid rows) goes into the
+// stage's output buffer.
 template <class L>
-int stage(grim_ctx* x, int st, grim_step_stats* stats) {
+int stagePrepare(grim_ctx* x, int st) {
   const Dims& D = x->D;
   const size_t bytes = sizeof(double) * (size_t)L::DOF * D.n;
-  // Fixed buffer roles (no pointer swaps), so that a captured step reads and
-  // writes the same addresses on every replay: the half stage advances
-  // primOld into primHalf, the full stage primHalf into primNew, which
-  // grim_time_step then copies back into primOld.
-  double* primStage = st == 0 ? x->primOld : x->primHalf;
-  double* out = st == 0 ? x->primHalf : x->primNew;
-  const double* dtPrime = x->clock + (st == 0 ? CLOCK_HALF_DT : CLOCK_DT);
+  double* primStage = stageIn(x, st);
+  double* out = stageOut(x, st);
   if (int rc = applyBoundaries(x, primStage, L::DOF, false)) return rc;
   if (x->torus)
     if (int rc = torusBCs<L>(x, primStage)) return rc;
@@ -1726,44 +1863,85 @@ int stage(grim_ctx* x, int st, grim_step_stats* stats) {
   if (int rc = fluxesAndEmf<L>(x, primStage, false)) return rc;
   GRIM_HIP_TRY(hipMemcpyAsync(out, primStage, bytes, hipMemcpyDeviceToDevice, x->s));
   const Box b = bulkBox(D);
-  launchFold<L, true>(x->a, b, x->s, x->fluxes, x->emf, x->geomC, x->consOld, x->sExp, x->sImp, dtPrime, nullptr,
-                      x->rhs, out, x->c.residualScaling ? x->mask : nullptr);
-  if (int rc = checkLaunch("fold")) return rc;
+  launchFold<L, true>(x->a, b, x->s, x->fluxes, x->emf, x->geomC, x->consOld, x->sExp, x->sImp,
+                      stageDtPrime(x, st), nullptr, x->rhs, out, x->c.residualScaling ? x->mask : nullptr);
+  return checkLaunch("fold");
+}
 
+// The stage's implicit system as a batch of independent zones (the frozen
+// terms folded into rhs; elemOld = primOld, the coefficient element primOld
+// on the half stage and primHalf on the full one).
+grim_zone_batch stageBatch(grim_ctx* x, int st, double* prim) {
   grim_zone_batch zb = {};
-  zb.n = D.n;
-  zb.prim = out;
+  zb.n = x->D.n;
+  zb.prim = prim;
   zb.geom = x->geomC;
   zb.rhs = x->rhs;
   zb.prim_old = x->primOld;
   zb.prim_coef = st == 0 ? x->primOld : x->primHalf;
   zb.closure = nullptr;
-  zb.radius = x->geomC + (size_t)LOC_R * D.n;
+  zb.radius = x->geomC + (size_t)LOC_R * x->D.n;
   zb.residual_mask = x->mask;
-  zb.dt_prime_device = dtPrime;
+  zb.dt_prime_device = stageDtPrime(x, st);
+  return zb;
+}
+
+// timeStepper::solve of the stage (solve.cpp:3-201) on prim; the per-zone
+// iteration counts and final residuals stay in the context.
+int stageSolve(grim_ctx* x, int st, double* prim) {
+  const grim_zone_batch zb = stageBatch(x, st, prim);
   grim_newton_stats ns = {x->iters, nullptr, x->l2};
-  if (int rc = grim_batched_newton(&x->p, &zb, &x->opts, &ns, x->s)) return rc;
-  if (stats) {
-    zeroCountersKernel<<<1, 64, 0, x->s>>>(x->counters, 3);
-    stageStatsKernel<<<kStatsBlocks, kBlock, 0, x->s>>>(D.n, x->mask, x->l2, x->iters,
-                                                          x->p.nonlinearsolve_atol, x->counters);
-    GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf, x->counters, 3 * sizeof(unsigned long long),
-                                hipMemcpyDeviceToHost, x->s));
-    GRIM_HIP_TRY(hipStreamSynchronize(x->s));
-    unsigned long long c[3];
-    std::memcpy(c, x->hostBuf, sizeof(c));
-    stats->unconverged[st] = (int64_t)c[0];
-    stats->max_iterations[st] = (int32_t)c[1];
-    stats->nonfinite[st] = (int64_t)c[2];
-  }
+  return grim_batched_newton(&x->p, &zb, &x->opts, &ns, x->s);
+}
+
+// Unconverged / non-finite bulk zones and the largest iteration count of the
+// last solve (synchronises the stream).
+int stageStats(grim_ctx* x, int st, grim_step_stats* stats) {
+  zeroCountersKernel<<<1, 64, 0, x->s>>>(x->counters, 3);
+  stageStatsKernel<<<kStatsBlocks, kBlock, 0, x->s>>>(x->D.n, x->mask, x->l2, x->iters, x->p.nonlinearsolve_atol,
+                                                        x->counters);
+  GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf, x->counters, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              x->s));
+  GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+  unsigned long long c[3];
+  std::memcpy(c, x->hostBuf, sizeof(c));
+  stats->unconverged[st] = (int64_t)c[0];
+  stats->max_iterations[st] = (int32_t)c[1];
+  stats->nonfinite[st] = (int64_t)c[2];
+  return GRIM_OK;
+}
+
+// After the solve: communicate, and the problem's half/full-step
+// diagnostics hooks (torus: NaN scrub on the full stage, applyFloor;
+// torus/problem.cpp:714-746).
+template <class L>
+int stageFinish(grim_ctx* x, int st) {
+  double* out = stageOut(x, st);
   if (int rc = communicate(x, out, L::DOF)) return rc;
-  // halfStepDiagnostics / fullStepDiagnostics of the torus (:714-741)
   if (x->torus) {
     if (st == 1)
       if (int rc = scrubNaNs(x, out, L::DOF)) return rc;
     if (int rc = applyFloor<L>(x, out)) return rc;
   }
   return GRIM_OK;
+}
+
+template <class L>
+int stage(grim_ctx* x, int st, grim_step_stats* stats) {
+  if (int rc = stagePrepare<L>(x, st)) return rc;
+  if (int rc = stageSolve(x, st, stageOut(x, st))) return rc;
+  if (stats)
+    if (int rc = stageStats(x, st, stats)) return rc;
+  return stageFinish<L>(x, st);
+}
+
+// The end of timeStepper::timeStep: primOld = the full stage's result, the
+// state the next computeDt reads, and time += dt (timestep.cpp:290-300).
+template <class L>
+int stepEnd(grim_ctx* x) {
+  const int64_t count = (int64_t)L::DOF * x->D.n;
+  copy2Kernel<<<blocksFor(count / 2 + 1), kBlock, 0, x->s>>>(count, x->primNew, x->primOld, x->primDt, x->clock);
+  return checkLaunch("end of step");
 }
 
 int validConfig(const grim_params* p, const grim_grid_config* c) {
@@ -2006,10 +2184,7 @@ int grim_time_step(grim_ctx* x, grim_step_stats* stats) {
     if (int rc = stage<L>(x, 1, stats)) return rc;
     // primOld = the full stage's result, and elemOld of the next computeDt
     // (the problem's fullStepDiagnostics set it from this step's primOld).
-    const int64_t count = (int64_t)L::DOF * x->D.n;
-    copy2Kernel<<<blocksFor(count / 2 + 1), kBlock, 0, x->s>>>(count, x->primNew, x->primOld, x->primDt,
-                                                               x->clock);
-    if (int rc = checkLaunch("grim_time_step")) return rc;
+    if (int rc = stepEnd<L>(x)) return rc;
     if (stats) {
       GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf + 4, x->clock, 2 * sizeof(double), hipMemcpyDeviceToHost, x->s));
       GRIM_HIP_TRY(hipStreamSynchronize(x->s));
@@ -2088,6 +2263,124 @@ int grim_explicit_sources(grim_ctx* x, const double* prim, double* sources) {
   return withLayout(x, [&]<class L>() -> int {
     return explicitSources<L>(x, prim, prim, sources, nullptr, nullptr);
   });
+}
+
+int grim_riemann_solve(grim_ctx* x, const double* prim_left, const double* prim_right, int dir, double* flux) {
+  GRIM_REQUIRE(x && prim_left && prim_right && flux && dir >= 0 && dir < x->D.dim,
+               "grim_riemann_solve: bad argument");
+  return withLayout(x, [&]<class L>() -> int {
+    if (dir == 0) launchFaceGiven<L, 0>(x, prim_left, prim_right, flux);
+    else if (dir == 1) launchFaceGiven<L, 1>(x, prim_left, prim_right, flux);
+    else launchFaceGiven<L, 2>(x, prim_left, prim_right, flux);
+    return checkLaunch("grim_riemann_solve");
+  });
+}
+
+namespace {
+const double* locationGeometry(grim_ctx* x, int location) {
+  return location == 0 ? x->geomC : (location == 1 ? x->geomX1 : (location == 2 ? x->geomX2 : nullptr));
+}
+
+int element(grim_ctx* x, const char* what, int kind, int dir, const double* prim, int location,
+            const double* aux0, const double* aux1, double dt, double* out0, double* out1) {
+  GRIM_REQUIRE(x && prim && out0, std::string(what) + ": null argument");
+  const double* geom = locationGeometry(x, location);
+  GRIM_REQUIRE(geom, std::string(what) + ": location must be 0 (centre), 1 (X1 faces) or 2 (X2 faces)");
+  GRIM_REQUIRE(location < 2 || x->D.dim > 1, std::string(what) + ": no X2 faces on a 1D grid");
+  return withLayout(x, [&]<class L>() -> int {
+    elementKernel<L><<<blocksFor(x->D.n), kBlock, 0, x->s>>>(x->a, kind, dir, prim, geom, aux0, aux1, dt, out0,
+                                                                out1);
+    return checkLaunch(what);
+  });
+}
+}  // namespace
+
+int grim_element_fluxes(grim_ctx* x, const double* prim, int location, int dir, double* out) {
+  GRIM_REQUIRE(dir >= 0 && dir <= 3, "grim_element_fluxes: dir must be 0..3");
+  return element(x, "grim_element_fluxes", ELEM_FLUXES, dir, prim, location, nullptr, nullptr, 0., out, nullptr);
+}
+
+int grim_element_char_speeds(grim_ctx* x, const double* prim, int location, int dir, double* min_speed,
+                             double* max_speed) {
+  GRIM_REQUIRE(x && dir >= 0 && dir < x->D.dim && max_speed, "grim_element_char_speeds: bad argument");
+  return element(x, "grim_element_char_speeds", ELEM_SPEEDS, dir, prim, location, nullptr, nullptr, 0., min_speed,
+                 max_speed);
+}
+
+int grim_element_fields(grim_ctx* x, const double* prim, int location, double* out) {
+  return element(x, "grim_element_fields", ELEM_FIELDS, 0, prim, location, nullptr, nullptr, 0., out, nullptr);
+}
+
+int grim_element_implicit_sources(grim_ctx* x, const double* prim, const double* tau_damp, double* sources) {
+  GRIM_REQUIRE(x && tau_damp, "grim_element_implicit_sources: null argument");
+  return element(x, "grim_element_implicit_sources", ELEM_IMPLICIT, 0, prim, 0, tau_damp, nullptr, 0., sources,
+                 nullptr);
+}
+
+int grim_element_time_deriv_sources(grim_ctx* x, const double* prim, const double* prim_old,
+                                    const double* prim_new, double dt, double* sources) {
+  GRIM_REQUIRE(x && prim_old && prim_new && dt > 0., "grim_element_time_deriv_sources: bad argument");
+  return element(x, "grim_element_time_deriv_sources", ELEM_TIME_DERIV, 0, prim, 0, prim_old, prim_new, dt,
+                 sources, nullptr);
+}
+
+int grim_step_begin(grim_ctx* x) {
+  GRIM_REQUIRE(x && x->clockSet, "grim_step_begin: no context or dt not set (grim_set_time)");
+  return withLayout(x, [&]<class L>() -> int { return computeDtOnDevice<L>(x, x->primDt); });
+}
+
+int grim_fold_rhs(grim_ctx* x, int stage) {
+  GRIM_REQUIRE(x && x->clockSet && (stage == 0 || stage == 1), "grim_fold_rhs: bad argument or dt not set");
+  return withLayout(x, [&]<class L>() -> int { return stagePrepare<L>(x, stage); });
+}
+
+int grim_solve(grim_ctx* x, int stage, double* prim, grim_step_stats* stats) {
+  GRIM_REQUIRE(x && x->clockSet && (stage == 0 || stage == 1), "grim_solve: bad argument or dt not set");
+  if (int rc = stageSolve(x, stage, prim ? prim : stageOut(x, stage))) return rc;
+  return stats ? stageStats(x, stage, stats) : GRIM_OK;
+}
+
+int grim_stage_residual(grim_ctx* x, int stage, const double* prim, double* residual) {
+  GRIM_REQUIRE(x && x->clockSet && (stage == 0 || stage == 1) && prim && residual,
+               "grim_stage_residual: bad argument");
+  const grim_zone_batch zb = stageBatch(x, stage, const_cast<double*>(prim));
+  return grim_residual(&x->p, &zb, residual, x->s);
+}
+
+int grim_stage_finish(grim_ctx* x, int stage) {
+  GRIM_REQUIRE(x && (stage == 0 || stage == 1), "grim_stage_finish: bad argument");
+  return withLayout(x, [&]<class L>() -> int { return stageFinish<L>(x, stage); });
+}
+
+int grim_step_end(grim_ctx* x) {
+  GRIM_REQUIRE(x && x->clockSet, "grim_step_end: no context or dt not set");
+  return withLayout(x, [&]<class L>() -> int { return stepEnd<L>(x); });
+}
+
+int grim_get_stage_arrays(grim_ctx* x, grim_stage_arrays* out) {
+  GRIM_REQUIRE(x && out, "grim_get_stage_arrays: null argument");
+  const size_t plane = (size_t)x->dof * x->D.n;
+  out->primOld = x->primOld;
+  out->primHalfStep = x->primHalf;
+  out->prim = x->primNew;
+  out->consOld = x->consOld;
+  out->sourcesExplicit = x->sExp;
+  out->sourcesImplicitOld = x->sImp;
+  out->rhs = x->rhs;
+  out->residualMask = x->mask;
+  out->fluxesX1 = x->fluxes;
+  out->fluxesX2 = x->D.dim > 1 ? x->fluxes + plane : nullptr;
+  out->fluxesX3 = x->D.dim > 2 ? x->fluxes + 2 * plane : nullptr;
+  return GRIM_OK;
+}
+
+int grim_last_solve_stats(grim_ctx* x, int32_t* iterations, double* final_l2, int on_device) {
+  GRIM_REQUIRE(x && (iterations || final_l2), "grim_last_solve_stats: null argument");
+  const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (iterations) GRIM_HIP_TRY(hipMemcpyAsync(iterations, x->iters, sizeof(int32_t) * (size_t)x->D.n, kind, x->s));
+  if (final_l2) GRIM_HIP_TRY(hipMemcpyAsync(final_l2, x->l2, sizeof(double) * (size_t)x->D.n, kind, x->s));
+  if (!on_device) GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+  return GRIM_OK;
 }
 
 int grim_compute_dt(grim_ctx* x, const double* prim, double dt, double* new_dt) {

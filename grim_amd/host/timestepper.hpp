@@ -12,15 +12,28 @@
 // MPI_Abort; here a failing C call throws std::runtime_error with the
 // library's message (exceptions never cross the C ABI itself).  Numerical
 // non-convergence is not an error: it is reported in lastStats().
+//
+// The stage-level members grim's timeStep is built from are here too:
+// solve(grid&), computeResidual, computeDivOfFluxes, batchLinearSolve, the
+// grids prim / primHalfStep / primOld / consOld / sourcesExplicit /
+// sourcesImplicitOld / divFluxes / fluxesX1..3 and the geometries
+// geomLeft / geomRight / geomBottom / geomTop / geomCenter, all on the
+// context's device arrays (grid.hpp, geometry.hpp, physics.hpp,
+// reconstruction.hpp mirror the classes they take).
 #pragma once
 
 #include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
+#include "geometry.hpp"
+#include "grid.hpp"
 #include "grim_hip.h"
+#include "physics.hpp"
+#include "reconstruction.hpp"
 
 namespace grim {
 
@@ -47,10 +60,6 @@ struct Params {
     newton.semantics = GRIM_NEWTON_PER_ZONE;
   }
 };
-
-inline void check(int rc, const char* what) {
-  if (rc != GRIM_OK) throw std::runtime_error(std::string(what) + ": " + grim_last_error());
-}
 
 class timeStepper {
   grim_ctx* ctx = nullptr;
@@ -114,8 +123,27 @@ class timeStepper {
     cfg.maxDtIncrement = params.maxDtIncrement;
     cfg.residualScaling = params.residualScaling;
     check(grim_create(&params.physics, &cfg, &params.newton, (grim_stream_t)stream, &ctx), "grim_create");
+    attachMembers();
   }
-  ~timeStepper() { grim_destroy(ctx); }
+  ~timeStepper() {
+    // the views go before the context that holds their memory
+    prim.reset();
+    primHalfStep.reset();
+    primOld.reset();
+    consOld.reset();
+    sourcesExplicit.reset();
+    sourcesImplicitOld.reset();
+    fluxesX1.reset();
+    fluxesX2.reset();
+    fluxesX3.reset();
+    divFluxes.reset();
+    geomLeft.reset();
+    geomRight.reset();
+    geomBottom.reset();
+    geomTop.reset();
+    geomCenter.reset();
+    grim_destroy(ctx);
+  }
   timeStepper(const timeStepper&) = delete;
   timeStepper& operator=(const timeStepper&) = delete;
 
@@ -221,10 +249,97 @@ class timeStepper {
                 o.mdot_out_unbound, o.mdot_out_relativistic);
   }
 
+  // ---- the stage-level members (timestepper.hpp:36-41, 58-90) -----------
+  // grids on the context's arrays: prim (the full stage's solution),
+  // primHalfStep, primOld (dof planes), consOld, sourcesExplicit,
+  // sourcesImplicitOld (numFluidVars of dof planes used), the face fluxes
+  // of the last stage and divFluxes (owned, written by computeDivOfFluxes)
+  std::unique_ptr<grid> prim, primHalfStep, primOld, consOld, sourcesExplicit, sourcesImplicitOld;
+  std::unique_ptr<grid> fluxesX1, fluxesX2, fluxesX3, divFluxes;
+  std::unique_ptr<geometry> geomLeft, geomRight, geomBottom, geomTop, geomCenter;
+  int currentStep = 0;  // timeStepperSwitches: 0 = HALF_STEP, 1 = FULL_STEP
+
+  // timeStepper::computeDivOfFluxes (fvmfluxes.cpp:3-184) of prim into
+  // divFluxes, with flux-CT and the problem's flux filter.
+  void computeDivOfFluxes(const grid& primIn, int& numReads, int& numWrites) {
+    check(grim_div_fluxes(ctx, primIn.vars, divFluxes->vars, nullptr), "timeStepper::computeDivOfFluxes");
+    numReads = 0;
+    numWrites = 0;
+  }
+
+  // The part of timeStep before solve in the current stage (timestep.cpp:
+  // 15-89 / 184-237): boundaries, sources, divergence of fluxes, induction,
+  // and the frozen terms folded into the residual's right-hand side; the
+  // guess goes into primHalfStep (half stage) or prim (full stage).
+  void foldFrozenTerms() { check(grim_fold_rhs(ctx, currentStep), "timeStepper: grim_fold_rhs"); }
+
+  // timeStepper::solve (solve.cpp:3-201) of the current stage on primGuess.
+  void solve(grid& primGuess) {
+    check(grim_solve(ctx, currentStep, primGuess.vars, &stats), "timeStepper::solve");
+  }
+
+  // timeStepper::computeResidual (residual.cpp:3-180) of the current stage.
+  void computeResidual(const grid& primIn, grid& residual, int& numReads, int& numWrites) {
+    check(grim_stage_residual(ctx, currentStep, primIn.vars, residual.vars), "timeStepper::computeResidual");
+    numReads = 0;
+    numWrites = 0;
+  }
+
+  // timeStepper::batchLinearSolve (solve.cpp:203-230): A x = b for n zones of
+  // nvar unknowns, device planes (A: nvar*nvar planes, entry (i, j) in plane
+  // i*nvar + j; b, x: nvar planes), LU with partial pivoting in FP64.
+  void batchLinearSolve(const double* A, const double* b, double* x, int nvar, int64_t n) {
+    check(grim_batch_linear_solve(n, nvar, A, b, x, GRIM_LU_FP64, nullptr), "timeStepper::batchLinearSolve");
+  }
+
+  // timeStep (timestep.cpp:3-334) written out as grim's control flow over
+  // the stage-level members: computeDt, then for each stage the frozen
+  // terms, solve, communicate and the diagnostics hooks, then primOld =
+  // prim and time += dt.  The same kernels as timeStep(), in the same order.
+  void timeStepByStages(int& numReads, int& numWrites) {
+    check(grim_step_begin(ctx), "timeStepper: grim_step_begin");
+    currentStep = 0;
+    foldFrozenTerms();
+    solve(*primHalfStep);
+    check(grim_stage_finish(ctx, 0), "timeStepper: grim_stage_finish");
+    currentStep = 1;
+    foldFrozenTerms();
+    solve(*prim);
+    check(grim_stage_finish(ctx, 1), "timeStepper: grim_stage_finish");
+    check(grim_step_end(ctx), "timeStepper: grim_step_end");
+    check(grim_get_time(ctx, &time, &dt), "grim_get_time");
+    numReads = 0;
+    numWrites = 0;
+  }
+
   // Statistics of the last timeStep: dt, unconverged zones and largest
   // Newton iteration count per stage, zones with a non-finite residual.
   const grim_step_stats& lastStats() const { return stats; }
   grim_ctx* context() const { return ctx; }
+
+ private:
+  void attachMembers() {
+    grim_stage_arrays a{};
+    check(grim_get_stage_arrays(ctx, &a), "grim_get_stage_arrays");
+    const int nf = 5 + phys.conduction + phys.viscosity;
+    prim = std::make_unique<grid>(ctx, cfg, numVars, a.prim);
+    primHalfStep = std::make_unique<grid>(ctx, cfg, numVars, a.primHalfStep);
+    primOld = std::make_unique<grid>(ctx, cfg, numVars, a.primOld);
+    consOld = std::make_unique<grid>(ctx, cfg, numVars, a.consOld);
+    sourcesExplicit = std::make_unique<grid>(ctx, cfg, nf, a.sourcesExplicit);
+    sourcesImplicitOld = std::make_unique<grid>(ctx, cfg, nf, a.sourcesImplicitOld);
+    fluxesX1 = std::make_unique<grid>(ctx, cfg, numVars, a.fluxesX1);
+    if (a.fluxesX2) fluxesX2 = std::make_unique<grid>(ctx, cfg, numVars, a.fluxesX2);
+    if (a.fluxesX3) fluxesX3 = std::make_unique<grid>(ctx, cfg, numVars, a.fluxesX3);
+    divFluxes = std::make_unique<grid>(ctx, cfg, numVars);
+    geomCenter = std::make_unique<geometry>(ctx, cfg, CENTER);
+    geomLeft = std::make_unique<geometry>(ctx, cfg, LEFT);
+    geomRight = std::make_unique<geometry>(ctx, cfg, RIGHT);
+    if (dim > 1) {
+      geomBottom = std::make_unique<geometry>(ctx, cfg, BOTTOM);
+      geomTop = std::make_unique<geometry>(ctx, cfg, TOP);
+    }
+  }
 };
 
 }  // namespace grim

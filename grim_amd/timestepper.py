@@ -246,6 +246,93 @@ class TimeStepper:
                    "grim_explicit_sources")
         return src
 
+    def riemannSolve(self, primLeft, primRight, dir: int):
+        """riemannSolver::solve from reconstructed states (reconstruct's output)."""
+        flux = self._empty(self.dof)
+        capi.check(capi.lib().grim_riemann_solve(self._ctx, self._grid_ptr(primLeft, self.dof),
+                                                 self._grid_ptr(primRight, self.dof), dir,
+                                                 self._grid_ptr(flux, self.dof)), "grim_riemann_solve")
+        return flux
+
+    # -- fluidElement on the grid (src/physics/physics.hpp:14-108) ----------
+    def elementFluxes(self, prim, dir: int, location: int = 0):
+        """computeFluxes(dir) of the element set from prim at a location
+        (0 centre, 1 X1 faces, 2 X2 faces); dir 0 = conserved variables."""
+        out = self._empty(self.dof)
+        capi.check(capi.lib().grim_element_fluxes(self._ctx, self._grid_ptr(prim, self.dof), location, dir,
+                                                  self._grid_ptr(out, self.dof)), "grim_element_fluxes")
+        return out
+
+    def elementCharSpeeds(self, prim, dir: int, location: int = 0):
+        mn, mx = self._empty(1), self._empty(1)
+        capi.check(capi.lib().grim_element_char_speeds(self._ctx, self._grid_ptr(prim, self.dof), location, dir,
+                                                       self._grid_ptr(mn, 1), self._grid_ptr(mx, 1)),
+                   "grim_element_char_speeds")
+        return mn[0], mx[0]
+
+    def elementFields(self, prim, location: int = 0) -> dict:
+        out = self._empty(len(capi.ELEMENT_FIELDS))
+        capi.check(capi.lib().grim_element_fields(self._ctx, self._grid_ptr(prim, self.dof), location,
+                                                  self._grid_ptr(out, len(capi.ELEMENT_FIELDS))),
+                   "grim_element_fields")
+        return {name: out[q] for q, name in enumerate(capi.ELEMENT_FIELDS)}
+
+    def elementImplicitSources(self, prim, tauDamp):
+        out = self._empty(self.dof)
+        capi.check(capi.lib().grim_element_implicit_sources(self._ctx, self._grid_ptr(prim, self.dof),
+                                                            self._grid_ptr(tauDamp.reshape(1, -1), 1),
+                                                            self._grid_ptr(out, self.dof)),
+                   "grim_element_implicit_sources")
+        return out
+
+    def elementTimeDerivSources(self, prim, primOld, primNew, dt: float):
+        out = self._empty(self.dof)
+        capi.check(capi.lib().grim_element_time_deriv_sources(self._ctx, self._grid_ptr(prim, self.dof),
+                                                              self._grid_ptr(primOld, self.dof),
+                                                              self._grid_ptr(primNew, self.dof), float(dt),
+                                                              self._grid_ptr(out, self.dof)),
+                   "grim_element_time_deriv_sources")
+        return out
+
+    # -- the step in its pieces (timestep.cpp:3-334) -----------------------
+    def stepBegin(self):
+        capi.check(capi.lib().grim_step_begin(self._ctx), "grim_step_begin")
+
+    def foldRHS(self, stage: int):
+        capi.check(capi.lib().grim_fold_rhs(self._ctx, stage), "grim_fold_rhs")
+
+    def solve(self, stage: int, prim=None, stats: bool = False):
+        st = capi.StepStats() if stats else None
+        capi.check(capi.lib().grim_solve(self._ctx, stage, self._grid_ptr(prim, self.dof),
+                                         C.byref(st) if stats else None), "grim_solve")
+        return st
+
+    def computeResidual(self, stage: int, prim):
+        out = self._empty(self.params.nf)
+        capi.check(capi.lib().grim_stage_residual(self._ctx, stage, self._grid_ptr(prim, self.dof),
+                                                  self._grid_ptr(out, self.params.nf)), "grim_stage_residual")
+        return out
+
+    def stageFinish(self, stage: int):
+        capi.check(capi.lib().grim_stage_finish(self._ctx, stage), "grim_stage_finish")
+
+    def stepEnd(self):
+        capi.check(capi.lib().grim_step_end(self._ctx), "grim_step_end")
+
+    def stageArrays(self) -> dict:
+        """Device addresses of the context's arrays under grim's names."""
+        a = capi.StageArrays()
+        capi.check(capi.lib().grim_get_stage_arrays(self._ctx, C.byref(a)), "grim_get_stage_arrays")
+        return {name: getattr(a, name) for name, _ in a._fields_}
+
+    def lastSolveStats(self):
+        """(iterations, final sum R^2) per zone of the last solve, host arrays."""
+        it = np.empty(self.n, dtype=np.int32)
+        l2 = np.empty(self.n)
+        capi.check(capi.lib().grim_last_solve_stats(self._ctx, it.ctypes.data, l2.ctypes.data, 0),
+                   "grim_last_solve_stats")
+        return it, l2
+
     def computeDt(self, prim, dt: float) -> float:
         out = C.c_double()
         capi.check(capi.lib().grim_compute_dt(self._ctx, self._grid_ptr(prim, self.dof), dt, C.byref(out)),

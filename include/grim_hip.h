@@ -358,6 +358,91 @@ int grim_div_fluxes(grim_ctx* ctx, const double* prim, double* div_f, double* fl
 int grim_explicit_sources(grim_ctx* ctx, const double* prim, double* sources);
 int grim_compute_dt(grim_ctx* ctx, const double* prim, double dt, double* new_dt);
 
+/* riemannSolver::solve (src/physics/riemannsolver.cpp:202-328) from
+ * reconstructed states: prim_left / prim_right are the face values
+ * grim_reconstruct writes (primLeft at the lower face of each zone,
+ * primRight at the upper face); flux at the lower face of each zone along
+ * dir, from primRight of zone z - 1 and primLeft of zone z at the face
+ * geometry (geomLeft/geomRight, geomBottom/geomTop, geomCenter for X3, as
+ * grim passes them, timestepper.cpp:236-280 and fvmfluxes.cpp:151). */
+int grim_riemann_solve(grim_ctx* ctx, const double* prim_left, const double* prim_right, int dir,
+                       double* flux);
+
+/* fluidElement on a whole grid (src/physics/physics.hpp:14-108): the element
+ * set from prim (dof planes) at the geometry of a location (0 = zone
+ * centres, 1 = X1 lower faces, 2 = X2 lower faces), then
+ *  grim_element_fluxes: computeFluxes(dir) (physics.cpp:185-217), dof planes;
+ *    dir 0 gives the conserved variables, 1..3 the fluxes along X1..X3;
+ *  grim_element_char_speeds: computeMinMaxCharSpeeds(dir) (riemannsolver.cpp:
+ *    54-200), dir 0..2 = X1..X3, one plane each;
+ *  grim_element_fields: the element's arrays, GRIM_ELEMENT_FIELDS planes in
+ *    the order rho, u, pressure, temperature, gammaLorentzFactor, soundSpeed,
+ *    bSqr, bNorm, q, deltaP, tau, chi_emhd, nu_emhd, uCon[4], uCov[4],
+ *    bCon[4], bCov[4];
+ *  grim_element_implicit_sources: computeImplicitSources(sources, tauDamp)
+ *    (physics.cpp:320-358) at the zone centres, dof planes;
+ *  grim_element_time_deriv_sources: computeTimeDerivSources(elemOld,
+ *    elemNew, dt, sources) (physics.cpp:219-318) of the element set from prim
+ *    (the coefficient element), elemOld / elemNew set from prim_old /
+ *    prim_new, at the zone centres, dof planes.
+ * Every zone of the grid is evaluated, ghosts included. */
+#define GRIM_ELEMENT_FIELDS 29
+int grim_element_fluxes(grim_ctx* ctx, const double* prim, int location, int dir, double* out);
+int grim_element_char_speeds(grim_ctx* ctx, const double* prim, int location, int dir, double* min_speed,
+                             double* max_speed);
+int grim_element_fields(grim_ctx* ctx, const double* prim, int location, double* out);
+int grim_element_implicit_sources(grim_ctx* ctx, const double* prim, const double* tau_damp,
+                                  double* sources);
+int grim_element_time_deriv_sources(grim_ctx* ctx, const double* prim, const double* prim_old,
+                                    const double* prim_new, double dt, double* sources);
+
+/* grim_time_step in its pieces, for a caller that runs timeStepper::timeStep
+ * (src/timestepper/timestep.cpp:3-334) itself; stage 0 = HALF_STEP, 1 =
+ * FULL_STEP.  dt lives in the context's device clock:
+ *  grim_step_begin: computeDt (timestep.cpp:8, 336-400) from the state the
+ *    last step left;
+ *  grim_fold_rhs: everything the stage does before timeStepper::solve --
+ *    boundary conditions and problem hooks on the stage state, explicit and
+ *    old implicit sources, the divergence of the fluxes with flux-CT, the
+ *    induction update of B, and the frozen right-hand side folded into one
+ *    array, rhs = consOld - dt' (divF + S_expl + S_impl,old / 2), with the
+ *    residual mask (residual.cpp:39-45, 173-177); the guess (the stage
+ *    state) goes into the stage's output (primHalfStep, then prim);
+ *  grim_solve: timeStepper::solve (solve.cpp:3-201) of the stage on prim
+ *    (dof planes, fluid rows updated; NULL = the stage's output); stats, if
+ *    given, receive the stage's counts (synchronises);
+ *  grim_stage_residual: timeStepper::computeResidual (residual.cpp:3-180) of
+ *    the stage at prim, numFluidVars planes;
+ *  grim_stage_finish: communicate and the problem's half/full-step
+ *    diagnostics on the stage's output;
+ *  grim_step_end: primOld = prim, time += dt.
+ * begin, fold/solve/finish for stage 0 then 1, and end are exactly
+ * grim_time_step. */
+int grim_step_begin(grim_ctx* ctx);
+int grim_fold_rhs(grim_ctx* ctx, int stage);
+int grim_solve(grim_ctx* ctx, int stage, double* prim, grim_step_stats* stats);
+int grim_stage_residual(grim_ctx* ctx, int stage, const double* prim, double* residual);
+int grim_stage_finish(grim_ctx* ctx, int stage);
+int grim_step_end(grim_ctx* ctx);
+
+/* Device arrays of the context under grim's timeStepper member names
+ * (timestepper.hpp:58-80): dof planes each for the states and consOld,
+ * numFluidVars planes for the sources, rhs and fluxes, one plane for the
+ * residual mask; fluxesX2/X3 are NULL below 2 / 3 dimensions.  The fluxes
+ * hold the last stage's (or grim_div_fluxes') face fluxes. */
+typedef struct grim_stage_arrays {
+  double *primOld, *primHalfStep, *prim;
+  double *consOld, *sourcesExplicit, *sourcesImplicitOld, *rhs, *residualMask;
+  double *fluxesX1, *fluxesX2, *fluxesX3;
+} grim_stage_arrays;
+int grim_get_stage_arrays(grim_ctx* ctx, grim_stage_arrays* out);
+
+/* Per-zone Newton iteration counts and final sums of squared residual rows
+ * of the context's last solve (the full stage after grim_time_step), all
+ * zones, into host (on_device = 0, synchronises) or device memory; either
+ * pointer may be NULL. */
+int grim_last_solve_stats(grim_ctx* ctx, int32_t* iterations, double* final_l2, int on_device);
+
 /* Initial conditions of the linear-mode problem
  * (src/problem/linear_modes/problem.cpp:10-111) into primOld:
  * mode 0 = ideal fast magnetosonic wave, k = (2 pi, 2 pi), on rho = 1,

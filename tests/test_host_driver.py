@@ -162,3 +162,132 @@ def test_driver_write_every_writes_primitives_and_vts(tmp_path):
     full = np.zeros((8, cfg.N1 + 2 * g, cfg.N2 + 2 * g))
     loaded = checkpoint.load(cfg, full.reshape(8, -1).copy(), str(h5[-1]))
     np.testing.assert_array_equal(loaded.reshape(8, 70, 70)[:, g:-g, g:-g].reshape(8, -1), prim)
+
+
+FACADE = os.path.join(os.path.dirname(HERE), "grim_amd", "facade_check")
+
+
+def read_facade(outdir):
+    out = {}
+    for line in open(os.path.join(outdir, "manifest.txt")):
+        name, count = line.split()
+        out[name] = np.fromfile(os.path.join(outdir, name + ".f64"), dtype=np.float64)
+        assert out[name].size == int(count)
+    return out
+
+
+def same(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return bool(((a == b) | (np.isnan(a) & np.isnan(b))).all())
+
+
+@pytest.mark.gpu
+def test_cpp_interfaces_match_python_path(tmp_path):
+    """grim's C++ interfaces of grim_amd/host (grid, geometry, fluidElement,
+    riemannSolver, reconstruction::reconstruct, timeStepper::solve /
+    computeResidual / computeDivOfFluxes / batchLinearSolve and the step
+    written out stage by stage), run by the g++-built grim_amd/facade_check
+    on a 48x32x8 EMHD torus, against the same operations taken through the
+    Python bindings (and, where one exists, through a second kernel path)."""
+    import torch
+
+    import bench
+    from grim_amd import capi
+    from grim_amd.timestepper import TimeStepper
+
+    assert os.path.exists(FACADE), "grim_amd/facade_check not built (run __graft_entry__.build())"
+    print("facade_check on a 48x32x8 EMHD torus", flush=True)
+    r = subprocess.run([FACADE, str(tmp_path)], capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stderr
+    F = read_facade(tmp_path)
+    p = bench.torus_params_emhd()
+    cfg = bench.torus_config(48, 32, 8)
+    ts = TimeStepper(p, cfg)
+    ts.initTorus(capi.torus_params())
+    ts.set_time(0., 0.005)
+    dof, n = p.dof, cfg.n
+    prim0 = ts.primOld
+    assert same(F["primOld"], prim0.ravel())
+    P = torch.from_numpy(prim0).cuda()
+    host = lambda t: t.cpu().numpy().ravel()  # noqa: E731
+    for d in range(3):
+        L, R = ts.reconstruct(P, d)
+        # the C++ grids start at zero where reconstruct writes nothing
+        wrote = ~np.isnan(host(L))
+        assert same(F[f"primLeft{d}"][wrote], host(L)[wrote]) and same(F[f"primRight{d}"][wrote], host(R)[wrote])
+        # riemannSolver::solve from the reconstructed states = the fused
+        # reconstruction + Riemann kernel, on every face it writes
+        fused = host(ts.riemannFlux(P, d))
+        faces = ~np.isnan(fused)
+        assert faces.sum() > 0 and same(F[f"riemann{d}"][faces], fused[faces]), d
+    fields = ts.elementFields(P)
+    for name, key in (("rho", "rho"), ("tau", "tau"), ("uCov2", "uCov2"), ("bCon0", "bCon0"),
+                      ("soundSpeed", "soundSpeed")):
+        assert same(F["elem_" + name], host(fields[key])), name
+    for d in range(4):
+        assert same(F[f"fluxes{d}"], host(ts.elementFluxes(P, d))), d
+    # conserved variables of the element = grim_conserved on the batch layout
+    U = capi.conserved(p, P, torch.from_numpy(ts.geometry(0)[:16].copy()).cuda(),
+                       radius=torch.from_numpy(ts.geometry(0)[22].copy()).cuda())
+    assert same(F["fluxes0"], host(U))
+    for d in range(3):
+        mn, mx = ts.elementCharSpeeds(P, d)
+        assert same(F[f"minSpeed{d}"], host(mn)) and same(F[f"maxSpeed{d}"], host(mx))
+    assert same(F["fluxesFaceX1"], host(ts.elementFluxes(P, 1, location=1)))
+    tau = fields["tau"].contiguous()
+    Si = ts.elementImplicitSources(P, tau)
+    assert same(F["implicitSources"], host(Si))
+    # closed form of computeImplicitSources: g qTilde / tau, zero elsewhere
+    g = ts.geometry(0)[15]
+    Sih = host(Si).reshape(dof, n)
+    np.testing.assert_array_equal(Sih[5], g * prim0[5] / host(tau))
+    assert (Sih[:5] == 0).all() and (Sih[7:] == 0).all()
+    Pn = prim0.copy()
+    Pn[1] *= 1.01
+    Pn[2] += 1e-3
+    Std = ts.elementTimeDerivSources(P, P, torch.from_numpy(Pn).cuda(), 0.005)
+    assert same(F["timeDerivSources"], host(Std))
+    sexp = host(ts.computeExplicitSources(P))
+    bulkmask = ~np.isnan(sexp)
+    assert same(F["explicitSources"][bulkmask], sexp[bulkmask])
+    div = host(ts.computeDivOfFluxes(P))
+    ok = ~np.isnan(div)
+    assert same(F["divFluxes"][ok], div[ok])
+    # the standalone geometry equals the context's
+    G = ts.geometry(0)
+    assert same(F["geom_gCov11"], G[4]) and same(F["geom_gCon23"], G[16 + 4]) and same(F["ts_gCon23"], G[20])
+    assert same(F["geom_alpha"], G[14]) and same(F["geom_g"], G[15])
+    assert same(F["geom_gamma123"], ts.geometry(3)[1 * 16 + 2 * 4 + 3])
+    GX1 = ts.geometry(1)[14]
+    assert same(F["ts_gammaRight_alpha"][:-1], GX1[1:])
+    # the step written out stage by stage = grim_time_step, bit for bit, and
+    # both equal the Python path's two steps
+    assert same(F["step_timeStep"], F["step_byStages"])
+    dts = F["step_dt"]
+    assert dts[0] == dts[1] and dts[2] == dts[3]
+    ts.timeStep()
+    ts.timeStep()
+    assert same(F["step_timeStep"], ts.primOld.ravel())
+    # the next half stage through the Python pieces: frozen terms, the
+    # residual at the guess, the solve, the residual at the solution
+    ts.stepBegin()
+    ts.foldRHS(0)
+    guess = torch.from_numpy(F["halfStep_guess"].reshape(dof, n).copy()).cuda()
+    assert same(F["residual_guess"], host(ts.computeResidual(0, guess)))
+    ts.solve(0, prim=guess)
+    assert same(F["halfStep_solution"], host(guess))
+    res_sol = ts.computeResidual(0, guess)
+    assert same(F["residual_solution"], host(res_sol))
+    # the solved half stage meets the relative residual test in the bulk
+    mask = F["residualMask"]
+    it, l2_stats = ts.lastSolveStats()
+    l2 = (host(res_sol).reshape(p.nf, n) ** 2).sum(axis=0)
+    bulk_zones = mask > 0
+    assert (l2_stats[bulk_zones] <= p.nonlinearsolve_atol).mean() > 0.97
+    np.testing.assert_allclose(l2[bulk_zones], l2_stats[bulk_zones], rtol=1e-12, atol=1e-300)
+    # batchLinearSolve against LAPACK
+    A = F["lu_A"].reshape(5, 5, 1000).transpose(2, 0, 1)
+    b = F["lu_b"].reshape(5, 1000).T
+    x = F["lu_x"].reshape(5, 1000).T
+    np.testing.assert_allclose(x, np.linalg.solve(A, b[..., None])[..., 0], rtol=1e-12, atol=1e-14)
+    ts.close()
