@@ -47,6 +47,10 @@ def parse():
     ap.add_argument("--torus-paths", default="fd_fp64_lockstep,fd_fp64,mixed",
                     help="comma-separated subset of the torus Newton paths (profiling)")
     ap.add_argument("--no-wave", action="store_true", help="skip the 64x64 linear-wave workload")
+    ap.add_argument("--only-torus", action="store_true",
+                    help="run only the torus paths of --torus-paths and print their JSON (profiling)")
+    ap.add_argument("--no-torus-error", action="store_true",
+                    help="skip the one-step mixed vs FD-FP64 comparison of the torus")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one elementwise negation of this many MiB first: reference bytes that calibrate "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -160,6 +164,10 @@ def main():
         torch.neg(src, out=dst)  # reads and writes copy_probe_mib MiB once each
         torch.cuda.synchronize()
         del src, dst
+    if args.only_torus:
+        args.no_wave = True
+        print(json.dumps({"full_step": full_step_bench(args)}), flush=True)
+        return
     n = args.zones
     p = capi.default_params(False)
     p.maxNonLinearIter = 30
@@ -178,6 +186,7 @@ def main():
     torch.cuda.synchronize()
     results = {}
     finals = {}
+    converged = {}
     selected = [q for q in PATHS if q in args.paths.split(",")]
     for path in selected:
         jac, lu, fused, kname = PATHS[path]
@@ -219,6 +228,7 @@ def main():
         t_peak = f64 / (FP64_PEAK_TFLOPS * 1e12) + f32 / (FP32_PEAK_TFLOPS * 1e12)
         rel = ((prim[: p.nf] - b["prim_star"][: p.nf]).abs() / b["prim_star"][: p.nf].abs()).amax(dim=0)
         finals[path] = prim[: p.nf].clone()
+        converged[path] = l2 <= p.nonlinearsolve_atol
         results[path] = {
             "zone_updates_per_s": n / (ms_per_step * 1e-3),
             "ms_per_step": ms_per_step,
@@ -243,6 +253,14 @@ def main():
             d = ((finals[path] - finals[ref]).abs() / finals[ref].abs()).amax(dim=0)
             results[path][f"err_vs_{ref}_p99"] = float(torch.quantile(d.float(), 0.99))
             results[path][f"err_vs_{ref}_max"] = float(d.max())
+            # split by convergence: zones that met the tolerance in both paths
+            both = converged[path] & converged[ref]
+            if bool(both.any()):
+                results[path][f"err_vs_{ref}_converged_max"] = float(d[both].max())
+                results[path][f"err_vs_{ref}_converged_p99"] = float(torch.quantile(d[both].float(), 0.99))
+            results[path][f"zones_unconverged_in_either_vs_{ref}"] = int((~both).sum())
+            if bool((~both).any()):
+                results[path][f"err_vs_{ref}_unconverged_max"] = float(d[~both].max())
 
     head = results[HEADLINE] if HEADLINE in results else results[selected[-1]]
     fd = results.get("fd_fp64", head)
@@ -296,7 +314,7 @@ def main():
         "paths": results,
     }
 
-    del b, rhs, mask, prim, batch, ws, finals
+    del b, rhs, mask, prim, batch, ws, finals, converged
     torch.cuda.empty_cache()
     if not args.no_emhd_batch:
         out["emhd_batch"] = emhd_batch_bench(args)
@@ -384,6 +402,8 @@ def torus_one_step_error(cfg, N1, N2, N3):
     from grim_amd.timestepper import TimeStepper
 
     out = {}
+    g = N1 + 6, N2 + 6, N3 + 6
+    bulk = lambda a, k: a.reshape(k, g[2], g[1], g[0])[:, 3:-3, 3:-3, 3:-3].reshape(k, -1)  # noqa: E731
     for name, opts in (("fd_fp64", capi.newton_opts()),
                        ("mixed", capi.newton_opts(jacobian=capi.JAC_ANALYTIC, lu_precision=capi.LU_FP32_FP64))):
         p = torus_params_emhd()
@@ -391,24 +411,65 @@ def torus_one_step_error(cfg, N1, N2, N3):
         ts.initTorus(capi.torus_params())
         ts.set_time(0., 0.005)
         st = ts.timeStep()
-        g = N1 + 6, N2 + 6, N3 + 6
-        out[name] = (ts.primOld.reshape(p.dof, g[2], g[1], g[0])[:, 3:-3, 3:-3, 3:-3].reshape(p.dof, -1),
-                     [int(st.unconverged[0]), int(st.unconverged[1])])
+        _, l2 = ts.lastSolveStats()  # the full stage's final sum R^2 per zone
+        out[name] = (bulk(ts.primOld, p.dof), [int(st.unconverged[0]), int(st.unconverged[1])],
+                     bulk(l2[None, :], 1)[0] <= p.nonlinearsolve_atol)
         ts.close()
         del ts
         torch.cuda.empty_cache()
     ref, mine = out["fd_fp64"][0], out["mixed"][0]
+    both = out["fd_fp64"][2] & out["mixed"][2]
     diff = np.abs(mine - ref)
     rel = diff[:2] / np.abs(ref[:2])
     scaled = diff / np.abs(ref).max(axis=1, keepdims=True)
-    return {"rho_u_rel_p99": float(np.percentile(rel, 99)), "rho_u_rel_median": float(np.median(rel)),
-            "rho_u_rel_max": float(rel.max()), "scaled_p99": float(np.percentile(scaled, 99)),
-            "scaled_max": float(scaled.max()), "unconverged_fd_fp64": out["fd_fp64"][1],
-            "unconverged_mixed": out["mixed"][1],
-            "note": "rho, u relative to themselves; every variable relative to its largest value"}
+    relz = rel.max(axis=0)
+    scaledz = scaled.max(axis=0)
+    res = {"rho_u_rel_p99": float(np.percentile(rel, 99)), "rho_u_rel_median": float(np.median(rel)),
+           "rho_u_rel_max": float(rel.max()), "scaled_p99": float(np.percentile(scaled, 99)),
+           "scaled_max": float(scaled.max()), "unconverged_fd_fp64": out["fd_fp64"][1],
+           "unconverged_mixed": out["mixed"][1],
+           "full_stage_converged_in_both": int(both.sum()), "full_stage_unconverged_in_either": int((~both).sum()),
+           "converged": {"rho_u_rel_max": float(relz[both].max()), "rho_u_rel_p99": float(np.percentile(relz[both], 99)),
+                         "scaled_max": float(scaledz[both].max()),
+                         "scaled_p99": float(np.percentile(scaledz[both], 99))},
+           "note": "rho, u relative to themselves; every variable relative to its largest value; 'converged' = "
+                   "bulk zones whose full-stage residual met the tolerance in both paths"}
+    if (~both).any():
+        res["unconverged"] = {"rho_u_rel_max": float(relz[~both].max()), "scaled_max": float(scaledz[~both].max())}
+        res["max_in"] = "unconverged zones" if relz[~both].max() >= relz[both].max() else "converged zones"
+    return res
 
 
 OBS_BYTES_PER_ZONE = (8 + 16) * 8  # rho, u, u~^i, B^i + gCov (10), gCon row 0, alpha, g
+# Algorithmic bytes of one zone-update of the 3D EMHD torus step (stored
+# geometry, interior zones; nP = 10 primitives, nf = 7 unknowns, G = 22
+# metric doubles, 40 connection coefficients): per stage 3 (2 nP + G) for the
+# face sweeps + (3 nP + nP + G + 40 + nf + 3) for divergence, CT, sources and
+# right-hand side + (2 nP + nf + G) for the Newton solve + 2 nP for the
+# floors = 307 doubles; two stages plus (nP + G) doubles of computeDt.
+TORUS_BYTES_PER_ZONE_STEP = (2 * 307 + 10 + 22) * 8
+TORUS_PROFILE_FILE = "profiles/torus_step_192x192x128.md"
+
+
+def torus_profile():
+    """Per-kernel durations and calibrated HBM traffic of the torus step's
+    kernels from the committed rocprofv3 summary (lines 'kernel: name us GB'),
+    or None when it is absent."""
+    import re
+
+    try:
+        text = open(os.path.join(ROOT, TORUS_PROFILE_FILE)).read()
+    except OSError:
+        return None
+    out = {}
+    for m in re.finditer(r"^kernel_summary:\s*(\S+)\s+([\d.]+)\s+([\d.]+)\s+([\d.]+)\s*$", text, re.M):
+        name, us, gb, valu = m.group(1), float(m.group(2)), float(m.group(3)), float(m.group(4))
+        out[name] = {"us": us, "hbm_GB": gb, "GBps": gb / (us * 1e-6), "frac_of_hbm_peak": gb / (us * 1e-6) /
+                     HBM_PEAK_GBS, "valu_issue_frac": valu}
+    return {"source": TORUS_PROFILE_FILE, "per_launch": out} if out else None
+
+
+TORUS_PROFILE = torus_profile()
 
 
 def observers_bench(ts, zones, calls=20):
@@ -449,9 +510,10 @@ def full_step_bench(args):
     cfg = torus_config(N1, N2, N3)
     zones = N1 * N2 * N3
     torus = {}
-    if "mixed" in args.torus_paths.split(","):
+    if "mixed" in args.torus_paths.split(",") and not args.no_torus_error:
         torus["one_step_error_mixed_vs_fd_fp64"] = torus_one_step_error(cfg, N1, N2, N3)
     states = {}
+    torus_state = {}
     for name, opts in (("fd_fp64_lockstep", capi.newton_opts(semantics=capi.NEWTON_GLOBAL)),
                        ("fd_fp64", capi.newton_opts()),
                        ("mixed", capi.newton_opts(jacobian=capi.JAC_ANALYTIC, lu_precision=capi.LU_FP32_FP64))):
@@ -465,6 +527,9 @@ def full_step_bench(args):
         ts = TimeStepper(p, cfg, opts=opts)
         ts.initTorus(capi.torus_params())
         ts.set_time(0., 0.005)
+        if not args.no_cpu and "cpu_state" not in torus_state:
+            # the initial state and geometry the CPU baseline starts from
+            torus_state["cpu_state"] = (p, ts.primOld, [ts.geometry(w) for w in range(4)], ts.geometry(4))
         ms, last = time_steps(ts, args.torus_steps, 2)
         torus[name] = {
             "ms_per_step": ms,
@@ -505,6 +570,26 @@ def full_step_bench(args):
                       ("fd_fp64", "speedup_mixed_vs_fd_fp64_per_zone")):
         if "mixed" in torus and base in torus:
             torus[key] = torus["mixed"]["zone_updates_per_s"] / torus[base]["zone_updates_per_s"]
+    # HBM roofline of the whole step: algorithmic bytes per zone-step (stored
+    # geometry, interior zones) over the measured time per step
+    for name in ("mixed", "fd_fp64", "fd_fp64_lockstep"):
+        if name in torus:
+            gbps = TORUS_BYTES_PER_ZONE_STEP * zones / (torus[name]["ms_per_step"] * 1e-3) / 1e9
+            torus[name]["roofline"] = {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": gbps / HBM_PEAK_GBS,
+                                       "bytes_per_zone_step": TORUS_BYTES_PER_ZONE_STEP,
+                                       "kernels": TORUS_PROFILE}
+    if "cpu_state" in torus_state:
+        del states
+        torch.cuda.empty_cache()
+        pc, prim0, geoms, floors = torus_state.pop("cpu_state")
+        print("# cpu baseline: torus step", file=sys.stderr, flush=True)
+        cb = cpu_baseline_step(cfg, pc, prim0, geoms, floors, steps=1, dt0=0.005)
+        cb["sample"] = (f"1 step (of the GPU run's {args.torus_steps}) of the oracle's full time step, FD-FP64 per-zone "
+                        f"Newton, from the same initial state and geometry, {cb['cores']} threads")
+        torus["cpu_baseline"] = cb
+        del prim0, geoms, floors
+        states = {}
     torus["config"] = (f"3D MKS torus {N1}x{N2}x{N3} (a=0.9375, r_in=6, r_max=12, SANE beta_min=100), EMHD "
                        f"(torus closure), WENO5+LLF, {args.torus_steps} steps after 2 warm-up, Newton to "
                        "relative residual 1e-10 (maxNonLinearIter 10); fd_fp64_lockstep = grim's solve loop "
@@ -526,9 +611,22 @@ def full_step_bench(args):
                                         riemannSolver=capi.RIEMANN_LLF), stream=torch.cuda.Stream())
         ts.initLinearMode(0, 1e-4)
         ts.set_time(0., 0.5 / 64)
+        if mode == "eager" and not args.no_cpu:
+            wave_ic = (ts.primOld, [ts.geometry(w) for w in range(4)])
         ms, _ = time_steps(ts, 10, 2, graph=(mode == "graph"))
         wave[mode] = {"ms_per_step": ms, "zone_updates_per_s": 64 * 64 / (ms * 1e-3)}
         ts.close()
+    if not args.no_cpu:
+        wcfg = grid_config(N1=64, N2=64, reconstruction=capi.RECON_WENO5, riemannSolver=capi.RIEMANN_LLF)
+        cb = cpu_baseline_step(wcfg, p, wave_ic[0], wave_ic[1], None, steps=10, dt0=0.5 / 64, torus=False)
+        cb["sample"] = f"the same 10 steps in the oracle (FD-FP64 per-zone Newton), {cb['cores']} threads"
+        from oracle import oracle as O
+
+        O.set_threads(1)
+        one = cpu_baseline_step(wcfg, p, wave_ic[0], wave_ic[1], None, steps=10, dt0=0.5 / 64, torus=False)
+        O.set_threads(host_cores())
+        cb["one_core"] = {"value": one["value"], "cores": 1, "sample": "the same 10 steps on one core"}
+        wave["cpu_baseline"] = cb
     res["linear_wave_64x64"] = wave
 
     # grim's 1D EMHD shock tube (src/problem/shock_tests/, N1 = 512, MINMOD +
@@ -668,6 +766,8 @@ def emhd_batch_bench(args):
         d = torch.cat([d[:5] / star[:5].abs(), d[5:] / scale[5:]]).amax(dim=0)
         res[name]["err_vs_root_p99"] = float(torch.quantile(d.float(), 0.99))
         res[name]["err_vs_root_median"] = float(d.median())
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_emhd(p, b, rhs, mask, n)
     res["speedup_tuned_vs_fd_fp64"] = res["mixed_tuned_graph"]["zone_updates_per_s"] / res["fd_fp64"]["zone_updates_per_s"]
     res["speedup_tuned_vs_mixed_one_pass"] = (res["mixed_tuned_graph"]["zone_updates_per_s"]
                                               / res["mixed"]["zone_updates_per_s"])
@@ -676,36 +776,112 @@ def emhd_batch_bench(args):
     return res
 
 
-def cpu_baseline(p, b, rhs, mask, n, reps):
-    """The oracle (FP64 FD Jacobian, grim's algorithm) timed on the host
-    cores over the same batch: median of `reps` full passes after one
-    warm-up pass, OpenMP over zones on every available core."""
-    import numpy as np
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    from oracle import oracle as O
 
-    O.lib()
-    cores = len(os.sched_getaffinity(0))
+def host_cores() -> int:
+    return len(os.sched_getaffinity(0))
+
+
+def oracle_params(O, p):
     q = O.Params()
     for name, _ in O.Params._fields_:
         setattr(q, name, getattr(p, name))
+    return q
+
+
+def cpu_newton_rate(O, q, guess, geom, rhs, mask, prim_old=None, closure=None, reps=3, threads=0):
+    """zone-updates/s of the oracle's FD-FP64 per-zone solve on these zones:
+    median of `reps` passes after one warm-up pass."""
+    import numpy as np
+
+    n = guess.shape[-1]
+    times = []
+    for r in range(reps + 1):
+        B = O.Batch(guess, geom, rhs, prim_old, None, closure, residualMask=mask)
+        t = time.perf_counter()
+        O.newton(q, B, O.NEWTON_PER_ZONE, nthreads=threads)
+        dt = time.perf_counter() - t
+        if r > 0:
+            times.append(dt)
+    return n / float(np.median(times))
+
+
+def cpu_baseline(p, b, rhs, mask, n, reps):
+    """The oracle (FP64 FD Jacobian, grim's algorithm) timed on the host
+    cores over the same batch: median of `reps` full passes after one
+    warm-up pass, OpenMP over zones on every available core; then the same
+    on one core over the first 2^16 zones."""
+    from oracle import oracle as O
+
+    O.lib()
+    cores = host_cores()
+    q = oracle_params(O, p)
     g = b["geom"].cpu().numpy()
     guess = b["guess"].cpu().numpy()
     rh = rhs.cpu().numpy()
     mk = mask.cpu().numpy()
-    times = []
-    for r in range(reps + 1):
-        B = O.Batch(guess, g, rh, residualMask=mk)
-        t = time.perf_counter()
-        O.newton(q, B, O.NEWTON_PER_ZONE, nthreads=cores)
-        dt = time.perf_counter() - t
-        print(f"# cpu pass {r}: {dt:.2f} s", file=sys.stderr, flush=True)
-        if r > 0:
-            times.append(dt)
-    med = float(np.median(times))
-    return {"value": n / med, "unit": "zone-updates/s", "cores": cores, "kind": "port",
-            "sample": f"full {n}-zone batch, median of {reps} passes after 1 warm-up, "
-                      f"OpenMP over zones, {cores} threads"}
+    print("# cpu baseline: headline batch", file=sys.stderr, flush=True)
+    value = cpu_newton_rate(O, q, guess, g, rh, mk, reps=reps, threads=cores)
+    m = min(n, 1 << 16)
+    one = cpu_newton_rate(O, q, guess[:, :m].copy(), g[:, :m].copy(), rh[:, :m].copy(), mk[:m].copy(), reps=3,
+                          threads=1)
+    O.set_threads(cores)
+    return {"value": value, "unit": "zone-updates/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": f"full {n}-zone batch, median of {reps} passes after 1 warm-up, OpenMP over zones, "
+                      f"{cores} threads",
+            "one_core": {"value": one, "cores": 1, "sample": f"first {m} zones, median of 3 passes"}}
+
+
+def cpu_baseline_emhd(p, b, rhs, mask, n):
+    """The oracle's FD-FP64 solve of the EMHD batch on a 2^18-zone sample
+    (every host core) and a 2^14-zone sample (one core)."""
+    from oracle import oracle as O
+
+    O.lib()
+    cores = host_cores()
+    q = oracle_params(O, p)
+    m = min(n, 1 << 18)
+    host = lambda t, k: t[..., :k].cpu().numpy().copy()  # noqa: E731
+    print("# cpu baseline: EMHD batch", file=sys.stderr, flush=True)
+    value = cpu_newton_rate(O, q, host(b["guess"], m), host(b["geom"], m), host(rhs, m), host(mask, m),
+                            host(b["prim_old"], m), host(b["closure"], m), reps=3, threads=cores)
+    k = min(n, 1 << 14)
+    one = cpu_newton_rate(O, q, host(b["guess"], k), host(b["geom"], k), host(rhs, k), host(mask, k),
+                          host(b["prim_old"], k), host(b["closure"], k), reps=3, threads=1)
+    O.set_threads(cores)
+    return {"value": value, "unit": "zone-updates/s", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"first {m} zones of the batch, median of 3 passes after 1 warm-up, {cores} threads",
+            "one_core": {"value": one, "cores": 1, "sample": f"first {k} zones, median of 3 passes"}}
+
+
+def cpu_baseline_step(cfg, p, prim, geoms, floors, steps, dt0, torus=True):
+    """The oracle's full time step (both stages, FD-FP64 per-zone Newton, the
+    torus hooks when torus) on the same grid from the same state and
+    geometry as the GPU run, timed over `steps` steps on every host core."""
+    from oracle import oracle as O
+
+    O.lib()
+    O.set_threads(host_cores())
+    g = O.grid(**{f: getattr(cfg, f) for f, _ in cfg._fields_})
+    gm = O.Geometry(g, center=geoms[0], faceX1=geoms[1], faceX2=geoms[2], gamma=geoms[3])
+    t = O.torus_params(minRho=floors[0].ctypes.data, minU=floors[1].ctypes.data) if torus else None
+    sim = O.Simulation(g, oracle_params(O, p), prim, dt=dt0, gm=gm, torus=t)
+    zones = cfg.N1 * cfg.N2 * cfg.N3
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    sec = (time.perf_counter() - t0) / steps
+    return {"value": zones / sec, "unit": "zone-updates/s", "cores": host_cores(), "kind": "port",
+            "cpu_model": cpu_model(), "seconds_per_step": sec}
 
 
 if __name__ == "__main__":

@@ -481,6 +481,8 @@ void grim_oracle_count_ops(const grim_oracle_params* p,
   luOps[4] = c.other;
 }
 
+void grim_oracle_set_threads(int nthreads) { setThreads(nthreads); }
+
 void grim_oracle_count_live_ops(const grim_oracle_params* p, const grim_oracle_batch* b,
                                 int64_t* residualOps) {
   const Vars vars(*p);

@@ -158,6 +158,9 @@ void grim_oracle_count_ops(const grim_oracle_params* p,
                            const grim_oracle_batch* b, int64_t* residualOps,
                            int64_t* luOps);
 
+/* OpenMP threads of every later parallel loop of the oracle (n > 0). */
+void grim_oracle_set_threads(int nthreads);
+
 /* Live operations of one residual evaluation, from the tracing scalar type:
  * those the residual rows depend on and that depend on the guess (work a
  * compiled evaluation repeats per call; dead code and per-zone constants

@@ -173,6 +173,7 @@ def lib():
         L.grim_oracle_lu_solve.restype = C.c_int
         L.grim_oracle_count_ops.argtypes = [P, B, vp, vp]
         L.grim_oracle_count_live_ops.argtypes = [P, B, vp]
+        L.grim_oracle_set_threads.argtypes = [C.c_int]
         Gp = C.POINTER(Grid)
         Gm = C.POINTER(_Geometry)
         L.grim_oracle_grid_zones.argtypes = [Gp]
@@ -369,6 +370,11 @@ def count_ops(p: Params, b: Batch):
     cb = b._c()
     lib().grim_oracle_count_ops(C.byref(p), C.byref(cb), _ptr(r), _ptr(l))
     return r, l
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's later parallel loops."""
+    lib().grim_oracle_set_threads(int(n))
 
 
 def count_live_ops(p: Params, b: Batch):
