@@ -142,7 +142,7 @@ class GridConfig(C.Structure):
         ("boundaryTop", C.c_int), ("boundaryBack", C.c_int), ("boundaryFront", C.c_int),
         ("reconstruction", C.c_int), ("riemannSolver", C.c_int), ("slopeLimTheta", C.c_double),
         ("CourantFactor", C.c_double), ("maxDtIncrement", C.c_double),
-        ("residualScaling", C.c_int),
+        ("residualScaling", C.c_int), ("explicitFast", C.c_int),
     ]
 
     @property

@@ -26,6 +26,7 @@
 #include "grim_hip.h"
 #include "runtime.hpp"
 #include "batchgen.hpp"
+#include "step_fast.hpp"
 #include "step_physics.hpp"
 #include "torus.hpp"
 #include "zone_physics.hpp"
@@ -35,21 +36,6 @@ struct grim_ctx;
 namespace grim {
 namespace {
 
-constexpr int kBlock = 256;
-
-struct StepArgs {
-  Dims D;
-  DevParams prm;
-  double dX[3];
-  double theta;
-  int riemann;
-  int metric;
-  int fluxFilter;  // torus: applyProblemSpecificFluxFilter
-};
-
-__device__ __forceinline__ int64_t flatIndex(const Dims& D, int i, int j, int k) {
-  return i + (int64_t)D.NT[0] * (j + (int64_t)D.NT[1] * k);
-}
 
 // (i, j, k) of zone z, in 32-bit arithmetic (grids hold fewer than 2^31 zones)
 __device__ __forceinline__ void zoneIndex(const Dims& D, int64_t z, int& i, int& j, int& k) {
@@ -266,7 +252,6 @@ __global__ __launch_bounds__(kBlock) void reconstructKernel(Dims D, Box box, int
 // needs 256 VGPRs plus spills and runs one wavefront per SIMD).  The right
 // states' fluxes, conserved variables and speeds meet the left states in
 // LDS.  The arithmetic follows grim's expression order exactly.
-constexpr int kFacesPerBlock = kBlock / 2;
 
 // RECON = GIVEN_STATES: the face values come ready-made, as
 // reconstruction::reconstruct leaves them (prim holds primRight on the left
@@ -1475,6 +1460,7 @@ struct grim_ctx {
   double *primOld, *primHalf, *primNew, *primDt, *consOld;
   double *fluxes, *emf, *sExp, *sImp, *uT, *rhs, *mask;
   double *geomC, *geomX1, *geomX2, *gamma;
+  double* gamma40;  // explicitFast: the 40 symmetric connection planes
   double* red;
   double* obs;  // observers: per-workgroup partials, then the result
   int* iters;
@@ -1586,6 +1572,10 @@ template <class L, int DIR>
 void launchFace(grim_ctx* x, const double* prim, double* flux, bool allFaces = true) {
   const Box b = allFaces ? faceBox(x->D, DIR) : stageFaceBox(x->D, DIR);
   const double* gf = DIR == 0 ? x->geomX1 : (DIR == 1 ? x->geomX2 : x->geomC);
+  if (x->c.explicitFast) {
+    (void)launchFaceFast(x->p, x->a, DIR, x->c.reconstruction, b, prim, gf, flux, x->s);
+    return;
+  }
   const unsigned nb = (unsigned)((b.count() + kFacesPerBlock - 1) / kFacesPerBlock);
   switch (x->c.reconstruction) {
     case GRIM_RECON_WENO5:
@@ -1629,6 +1619,9 @@ int explicitSources(grim_ctx* x, const double* prim, const double* primOld, doub
   const Box b = bulkBox(x->D);
   if constexpr (L::EMHD)
     uTKernel<L><<<blocksFor(x->D.n), kBlock, 0, x->s>>>(x->a, prim, x->geomC, x->uT);
+  if (x->c.explicitFast)
+    return launchSourcesFast(x->p, x->a, x->c.reconstruction, b, prim, primOld, x->geomC, x->gamma40, x->uT, sExp,
+                             sImp, consOld, x->s);
   const unsigned nb = blocksFor(b.count());
   switch (x->c.reconstruction) {
     case GRIM_RECON_WENO5:
@@ -1961,6 +1954,7 @@ int validConfig(const grim_params* p, const grim_grid_config* c) {
   if (c->reconstruction < 0 || c->reconstruction > 2 || c->riemannSolver < 0 || c->riemannSolver > 1)
     return 0;
   if ((p->conduction || p->viscosity) && p->closure == GRIM_CLOSURE_PER_ZONE) return 0;
+  if (c->explicitFast != 0 && c->explicitFast != 1) return 0;
   return 1;
 }
 
@@ -2084,6 +2078,15 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
   geometryKernel<<<nb, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, 1, x->geomX1);
   geometryKernel<<<nb, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, 2, x->geomX2);
   connectionKernel<<<nb, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, x->gamma);
+  if (c->explicitFast) {
+    hipError_t e = hipMalloc((void**)&x->gamma40, sizeof(double) * 40 * (size_t)n);
+    if (e != hipSuccess) {
+      setError(std::string("grim_create: hipMalloc: ") + hipGetErrorString(e));
+      grim_destroy(x);
+      return GRIM_ERR_OOM;
+    }
+    launchGamma40(n, x->gamma, x->gamma40, x->s);
+  }
   maskKernel<<<nb, kBlock, 0, x->s>>>(x->D, x->mask);
   if (int rc = checkLaunch("grim_create")) {
     grim_destroy(x);
@@ -2105,7 +2108,7 @@ int grim_destroy(grim_ctx* x) {
   void* ptrs[] = {x->primOld, x->primHalf, x->primNew, x->primDt, x->consOld, x->fluxes, x->emf,
                   x->sExp,    x->sImp,     x->uT,      x->rhs,    x->mask,    x->geomC,  x->geomX1,
                   x->geomX2,  x->gamma,    x->red,     x->iters,  x->l2,      x->counters,
-                  x->newtonWs, x->clock,  x->obs};
+                  x->newtonWs, x->clock,  x->obs,  x->gamma40};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (x->hostBuf) (void)hipHostFree(x->hostBuf);

@@ -92,6 +92,23 @@ __device__ __forceinline__ void loadGeomFull(const double* __restrict__ gl, int6
   G.r = gl[LOC_R * n + z];
 }
 
+constexpr int kBlock = 256;
+
+// Everything a step kernel reads besides its arrays.
+struct StepArgs {
+  Dims D;
+  DevParams prm;
+  double dX[3];
+  double theta;
+  int riemann;
+  int metric;
+  int fluxFilter;  // torus: applyProblemSpecificFluxFilter
+};
+
+__device__ __forceinline__ int64_t flatIndex(const Dims& D, int i, int j, int k) {
+  return i + (int64_t)D.NT[0] * (j + (int64_t)D.NT[1] * k);
+}
+
 // ---------------------------------------------------------------------------
 // Element quantities beyond the kinematics: sound speed, |b|, EMHD closure
 // (fluidElement::set, physics.cpp:53-183).

@@ -52,6 +52,8 @@ struct Params {
   // Newton path of the two solves and the relative residual test
   grim_newton_opts newton{};
   int residualScaling = 0;
+  // the explicit stage with shared reciprocals and fused multiply-adds
+  int explicitFast = 0;
 
   explicit Params(bool emhd = false) {
     grim_default_params(&physics, emhd ? 1 : 0);
@@ -122,6 +124,7 @@ class timeStepper {
     cfg.CourantFactor = params.CourantFactor;
     cfg.maxDtIncrement = params.maxDtIncrement;
     cfg.residualScaling = params.residualScaling;
+    cfg.explicitFast = params.explicitFast;
     check(grim_create(&params.physics, &cfg, &params.newton, (grim_stream_t)stream, &ctx), "grim_create");
     attachMembers();
   }

@@ -288,6 +288,12 @@ typedef struct grim_grid_config {
    * to the zone's conserved variables (no grim counterpart: an absolute
    * 1e-20 lies below the FP64 rounding floor of zones with large U). */
   int residualScaling;
+  /* 0: the explicit stage (face fluxes, explicit sources) in grim's
+   * expression order, one rounding per operation, bit-reproducible by a CPU
+   * FP64 evaluation.  1: the same mathematics with shared reciprocals, fused
+   * multiply-adds and the 40 symmetric connection coefficients (+40 planes
+   * of memory); results agree with 0 to rounding (tested to 1e-12). */
+  int explicitFast;
 } grim_grid_config;
 
 typedef struct grim_ctx grim_ctx;

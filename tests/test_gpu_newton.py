@@ -387,6 +387,69 @@ def test_graph_captured_step_matches_eager(capi, torch_cuda):
         assert torch.equal(prim, eager[0])
 
 
+def test_emhd_tuned_graph_replay_equals_one_pass_mixed(capi, torch_cuda):
+    """The EMHD batch's tuned path: per-zone closure alphas over three decades,
+    10 % near-floor zones, maxNonLinearIter 20, analytic Jacobian + FP32 LU,
+    in the configuration the autotuner picks, captured in a HIP graph and
+    replayed.  Every replay is bitwise the one-pass mixed solve (no
+    compaction, default workgroup), and the solution is within the
+    known-root bounds of that batch: median error 1e-9, p99 1e-3 (the tail
+    is the near-floor zones, whose conditioning times the relative tolerance
+    1e-10 allows it), unconverged below 0.1 % of the zones."""
+    torch = torch_cuda
+    n = 1 << 18
+    p, b, rhs, mask = make_problem(capi, torch, n, True)
+    p.maxNonLinearIter = 20
+    prim = b["guess"].clone()
+    it = torch.zeros(n, dtype=torch.int32, device="cuda")
+    l2 = torch.zeros(n, dtype=torch.float64, device="cuda")
+    ws = torch.empty(capi.workspace_bytes(n), dtype=torch.uint8, device="cuda")
+    batch = capi.make_batch(prim, b["geom"], rhs, b["prim_old"], None, b["closure"], residual_mask=mask)
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    tuned = capi.autotune_newton(p, batch, capi.newton_opts(capi.JAC_ANALYTIC, capi.LU_FP32, workspace=ws),
+                                 stream=stream)
+    torch.cuda.synchronize()
+    assert torch.equal(prim, b["guess"])
+    print(f"autotuned: block {tuned.block_size} compaction {tuned.compaction} phase1 {tuned.phase1_iters} "
+          f"phase2 group {tuned.phase2_group}", flush=True)
+
+    def step(opts):
+        prim[: p.nf].copy_(b["guess"][: p.nf])
+        capi.batched_newton(p, batch, iterations=it, final_l2=l2, opts=opts, stream=torch.cuda.current_stream())
+
+    with torch.cuda.stream(stream):
+        step(capi.newton_opts(capi.JAC_ANALYTIC, capi.LU_FP32))
+    torch.cuda.synchronize()
+    one_pass = (prim.clone(), it.clone(), l2.clone())
+    with torch.cuda.stream(stream):
+        step(tuned)  # warm-up on the capture stream
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=stream):
+        step(tuned)
+    torch.cuda.synchronize()
+    for r in range(3):
+        prim[: p.nf].fill_(0.0)
+        it.fill_(-1)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(prim, one_pass[0]) and torch.equal(it, one_pass[1]) and torch.equal(l2, one_pass[2]), r
+        with torch.cuda.stream(stream):
+            step(capi.newton_opts(capi.JAC_ANALYTIC, capi.LU_FP32))  # eager work between replays
+        torch.cuda.synchronize()
+    del g
+    star = b["prim_star"][: p.nf]
+    scale = star.abs().amax(dim=1, keepdim=True)
+    d = (one_pass[0][: p.nf] - star).abs()
+    d = torch.cat([d[:5] / star[:5].abs(), d[5:] / scale[5:]]).amax(dim=0)
+    unconverged = capi.count_above(one_pass[2], p.nonlinearsolve_atol)
+    med, p99 = float(d.median()), float(torch.quantile(d.float(), 0.99))
+    print(f"EMHD tuned graph: unconverged {unconverged} of {n}, error vs root median {med:.2e} p99 {p99:.2e}")
+    assert unconverged < 1e-3 * n and med < 1e-9 and p99 < 1e-3
+
+
 # ------------------------------------------------ tangent and adjoint products
 @pytest.mark.parametrize("emhd", [False, True])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])

@@ -376,7 +376,7 @@ def test_mixed_newton_step_close_to_fd(torch):
 # ---------------------------------------------------------------------------
 # Torus problem
 # ---------------------------------------------------------------------------
-def torus_case(emhd, N1=96, N2=48, N3=4, opts=None, scaling=1, **pkw):
+def torus_case(emhd, N1=96, N2=48, N3=4, opts=None, scaling=1, fast=0, **pkw):
     from grim_amd import capi
 
     a = 0.9375
@@ -385,7 +385,7 @@ def torus_case(emhd, N1=96, N2=48, N3=4, opts=None, scaling=1, **pkw):
                X2Start=1e-8, X2End=1 - 1e-8, X3Start=0., X3End=2 * np.pi, DerefineThetaHorizon=1,
                DoCylindrify=1, X1cyl=np.log(4 * Rin), X2cyl=1. / N2, boundaryLeft=1, boundaryRight=1,
                boundaryBottom=2, boundaryTop=2, boundaryBack=0, boundaryFront=0, reconstruction=1,
-               riemannSolver=1, residualScaling=scaling)
+               riemannSolver=1, residualScaling=scaling, explicitFast=fast)
     ts, g, p, po = make(cfg, emhd=emhd, closure=2, opts=opts, adiabaticIndex=5. / 3, **pkw)
     gm = oracle_geometry(ts, g)
     ts.initTorus(capi.torus_params())
@@ -717,4 +717,101 @@ def test_bad_tensors_raise_before_any_launch(torch):
     ts.set_ghost_state(good)
     ts.set_ghost_state(None)
     assert ts.dump_vars(good).shape == (p.dof + 5, 16 * 16)
+    ts.close()
+
+
+# ------------------------------------------------------ the fast explicit stage
+def scaled_diff(a, b, groups=None):
+    """max over zones of |a - b| / max|b| per variable (finite entries of b).
+
+    The rows of a group share one scale, the largest of theirs: the torus
+    starts with B^3 = 0, so its flux g (b^3 u^d - b^d u^3) is the rounding
+    of two equal products, which sets no scale of its own."""
+    a = np.asarray(a).reshape(a.shape[0], -1)
+    b = np.asarray(b).reshape(b.shape[0], -1)
+    ok = np.isfinite(b).all(axis=0) & np.isfinite(a).all(axis=0)
+    scale = np.abs(b[:, ok]).max(axis=1, keepdims=True) + 1e-300
+    if groups is None:  # the field rows B^1..B^3, the last three of a primitive-shaped array
+        groups = ((b.shape[0] - 3, b.shape[0] - 2, b.shape[0] - 1),)
+    for grp in groups:
+        rows = [r for r in grp if r < b.shape[0]]
+        scale[rows] = scale[rows].max()
+    return (np.abs(a[:, ok] - b[:, ok]) / scale).max(axis=1), ok
+
+
+@pytest.mark.parametrize("emhd", [False, True])
+def test_fast_explicit_pieces_match_exact(torch, emhd):
+    """explicitFast = 1 (shared reciprocals, fused multiply-adds, 40-plane
+    connection) against grim's expression order on the same torus state:
+    every face flux and every explicit source within 1e-12 of the variable's
+    largest magnitude.  The largest source difference (a few 1e-13) comes
+    from the connection: grim's central differences (geometry.cpp:61-121)
+    sum the three metric derivatives in an order that leaves
+    Gamma^l_{k n} and Gamma^l_{n k} a rounding of about 1e-11 apart, and the
+    40-plane connection keeps the k <= n one of each pair."""
+    exact = torus_case(emhd, N1=64, N2=48, N3=8)
+    fast = torus_case(emhd, N1=64, N2=48, N3=8, fast=1)
+    ts0, g, p, po = exact[:4]
+    ts1 = fast[0]
+    rng = np.random.default_rng(3)
+    prim = ts0.primOld
+    assert np.array_equal(prim, ts1.primOld)
+    # stir the state so that every branch of the closure and limiter runs
+    prim[2:5] *= 1 + 0.2 * rng.standard_normal((3, g.n))
+    if emhd:
+        prim[5:7] = 1e-3 * rng.standard_normal((2, g.n)) * prim[1]
+    P = dev(torch, prim)
+    for d in range(3):
+        a, b = ts1.riemannFlux(P, d).cpu().numpy(), ts0.riemannFlux(P, d).cpu().numpy()
+        err, ok = scaled_diff(a, b)
+        print(f"face fluxes dir {d}: max scaled difference {err.max():.2e} over {ok.sum()} faces", flush=True)
+        assert ok.sum() > 0.5 * g.n and err.max() < 1e-12, (d, err)
+    a, b = ts1.computeExplicitSources(P).cpu().numpy(), ts0.computeExplicitSources(P).cpu().numpy()
+    # t and phi are Killing directions of Kerr: the energy and u_phi sources
+    # T^k_l Gamma^l_{nu k} = T^{k m} d_nu g_{k m} / 2 vanish there up to the
+    # rounding of cancelling pairs, so the four rows share the momentum scale
+    err, ok = scaled_diff(a, b, groups=((1, 2, 3, 4),))
+    print(f"explicit sources: max scaled difference per row {np.array2string(err, precision=1)}", flush=True)
+    r = int(err.argmax())
+    d = np.where(ok, np.abs(a[r] - b[r]).reshape(-1), 0.)
+    z = int(d.argmax())
+    zi = np.unravel_index(z, g.shape)
+    print(f"  largest in row {r} at zone (k, j, i) {tuple(int(v) for v in zi)}: fast {a[r].reshape(-1)[z]:.17e} "
+          f"exact {b[r].reshape(-1)[z]:.17e}, row max {np.abs(b[r].reshape(-1)[ok]).max():.3e}", flush=True)
+    assert err.max() < 1e-12
+    a, b = ts1.computeDivOfFluxes(P).cpu().numpy(), ts0.computeDivOfFluxes(P).cpu().numpy()
+    err, ok = scaled_diff(a, b)
+    print(f"divergence of fluxes: max scaled difference per row {np.array2string(err, precision=1)}", flush=True)
+    assert err.max() < 1e-10
+    ts0.close()
+    ts1.close()
+
+
+@pytest.mark.parametrize("solver", ["fd_fp64", "mixed_escalating"])
+def test_fast_explicit_torus_steps_match_oracle(torch, solver):
+    """Two EMHD torus steps with the fast explicit stage against the oracle's
+    steps from the same state: the same bar as the exact stage's test."""
+    from grim_amd import capi
+
+    opts = None
+    if solver == "mixed_escalating":
+        opts = capi.newton_opts(capi.JAC_ANALYTIC, capi.LU_FP32_FP64)
+    ts, g, p, po, gm, t, floors = torus_case(True, opts=opts, fast=1, maxNonLinearIter=12)
+    prim0 = ts.primOld
+    ts.set_time(0., 0.005)
+    sim = O.Simulation(g, po, prim0, dt=0.005, gm=gm, torus=t)
+    for s in range(2):
+        st = ts.timeStep()
+        sim.step()
+    a, b = bulk(g, ts.primOld), bulk(g, sim.prim)
+    scale = np.abs(b).reshape(po.dof, -1).max(axis=1).reshape(-1, 1, 1, 1)
+    rel = np.abs(a - b) / (np.abs(b) + 1e-12 * scale + 1e-300)
+    ideal = np.r_[0:5, po.nf:po.dof]
+    scaled = np.abs(a - b)[5:po.nf] / scale[5:po.nf]
+    print(f"fast explicit, torus EMHD {solver}: ideal vars rel p99 {np.percentile(rel[ideal], 99):.2e} "
+          f"max {rel[ideal].max():.2e}; q/dP scaled p99 {np.percentile(scaled, 99):.2e}; "
+          f"unconverged {list(st.unconverged)} non-finite {list(st.nonfinite)}")
+    assert list(st.nonfinite) == [0, 0] or solver == "fd_fp64"
+    assert st.unconverged[0] + st.unconverged[1] < 1e-2 * g.N1 * g.N2 * g.N3
+    assert np.percentile(rel[ideal], 99) < 5e-7 and np.percentile(scaled, 99) < 1e-9
     ts.close()
