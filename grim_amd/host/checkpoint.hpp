@@ -5,6 +5,8 @@
 //                   WriteData dumps, torus/problem.cpp:778-840)
 //  dumpVTS          the dump variables of primOld as dump%06d.vts
 //                   (torus/problem.cpp:808-876)
+//  dumpMetric       the metric files of the first WriteData (gCov.h5, gCon.h5,
+//                   sqrtDetg.h5, xCoords.h5; torus/problem.cpp:780-792)
 //  loadPrimitives   grid::load into primOld (src/grid/grid.cpp:482-499)
 //  writeRestart     CheckWallClockTermination's restart dump and the two
 //                   text files naming it (torus/problem.cpp:880-906)
@@ -14,6 +16,7 @@
 
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -72,6 +75,56 @@ inline void dumpVTS(const timeStepper& ts, const std::string& file) {
   const std::vector<double> vars = ts.dumpVars();
   checkIo(grim_io_dump_vts(&ts.config(), (int)names.size(), names.data(), vars.data(), file.c_str()),
           "grim_io_dump_vts");
+}
+
+// The metric files the first WriteData writes (torus/problem.cpp:780-792):
+// geomCenter's gCovGrid, gConGrid (16 variables each, index nu + 4 mu;
+// geometry.cpp:686-714), gGrid (sqrt(-g)) and xCoordsGrid (the zone centres'
+// x coordinates; geometry.cpp:753-764) as datasets "gCov", "gCon",
+// "sqrtDetg", "xCoords" of gCov.h5, gCon.h5, sqrtDetg.h5 and xCoords.h5.
+inline void dumpMetric(const timeStepper& ts, const std::string& dir = ".") {
+  const int64_t n = ts.zones();
+  std::vector<double> geom((size_t)GRIM_LOC_PLANES * (size_t)n);
+  check(grim_get_geometry(ts.context(), 0, geom.data(), 0), "grim_get_geometry");
+  auto packed = [](int a, int b, int m) {
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    return lo * m - (lo * (lo + 1)) / 2 + hi;
+  };
+  std::vector<double> gCov((size_t)16 * n), gCon((size_t)16 * n);
+  for (int mu = 0; mu < 4; mu++)
+    for (int nu = 0; nu < 4; nu++) {
+      const int cov = packed(mu, nu, 4);
+      const int con = (mu == 0 || nu == 0) ? 10 + (mu == 0 ? nu : mu) : 16 + packed(mu - 1, nu - 1, 3);
+      std::copy(geom.begin() + (size_t)cov * n, geom.begin() + (size_t)(cov + 1) * n,
+                gCov.begin() + (size_t)(nu + 4 * mu) * n);
+      std::copy(geom.begin() + (size_t)con * n, geom.begin() + (size_t)(con + 1) * n,
+                gCon.begin() + (size_t)(nu + 4 * mu) * n);
+    }
+  const std::vector<double> g(geom.begin() + (size_t)15 * n, geom.begin() + (size_t)16 * n);
+  // the x coordinates of the bulk zones, as grim_dump_vars gives them,
+  // placed into ghosted planes (grid::dump writes the bulk only)
+  const grim_grid_config& c = ts.config();
+  const std::vector<double> vars = ts.dumpVars();
+  const size_t nb = (size_t)c.N1 * c.N2 * c.N3;
+  std::vector<double> x((size_t)3 * n, 0.);
+  const int g1 = c.numGhost, g2 = c.dim > 1 ? c.numGhost : 0, g3 = c.dim > 2 ? c.numGhost : 0;
+  const int64_t N1T = c.N1 + 2 * g1, N2T = c.N2 + 2 * g2;
+  const double dX3 = (c.X3End - c.X3Start) / c.N3;
+  for (int d = 0; d < 3; d++)
+    for (int k = 0; k < c.N3; k++)
+      for (int j = 0; j < c.N2; j++)
+        for (int i = 0; i < c.N1; i++) {
+          // x^3 = X^3 in both metrics (phi = X3); the dump variables' third
+          // point coordinate is 0 below three dimensions, as PETSc's VTS
+          // points are, so it is formed here from the zone centre
+          const size_t b = (size_t)(i + (int64_t)c.N1 * (j + (int64_t)c.N2 * k));
+          x[(size_t)d * n + (size_t)((i + g1) + N1T * ((j + g2) + N2T * (k + g3)))] =
+              d < 2 ? vars[(size_t)(ts.numVars + 2 + d) * nb + b] : c.X3Start + (k + 0.5) * dX3;
+        }
+  checkIo(grim_io_dump(&c, 16, gCov.data(), "gCov", (dir + "/gCov.h5").c_str()), "grim_io_dump");
+  checkIo(grim_io_dump(&c, 16, gCon.data(), "gCon", (dir + "/gCon.h5").c_str()), "grim_io_dump");
+  checkIo(grim_io_dump(&c, 1, g.data(), "sqrtDetg", (dir + "/sqrtDetg.h5").c_str()), "grim_io_dump");
+  checkIo(grim_io_dump(&c, 3, x.data(), "xCoords", (dir + "/xCoords.h5").c_str()), "grim_io_dump");
 }
 
 // restartVarsT%08d.h5 with index floor(100 time), then the file name and

@@ -18,8 +18,9 @@
 // whenever time crosses a multiple of DT (fullStepDiagnostics,
 // torus/problem.cpp:752-776).  --write-every DT writes primOld to
 // primVarsT%06d.h5 and the dump variables to dump%06d.vts whenever time
-// crosses a multiple of DT (WriteData, :777-876; the metric files of the
-// first write are not produced).  A run restarts as grim's constructor does
+// crosses a multiple of DT (WriteData, :777-876), and at the start the
+// metric files gCov.h5, gCon.h5, sqrtDetg.h5 and xCoords.h5 with the first
+// dumps (the initial conditions' WriteIdx 0, :780-792).  A run restarts as grim's constructor does
 // (timestepper.cpp:387-441) when restartFileName.txt and restartFileTime.txt
 // exist in the working directory, or from restartFile.h5 with --restart;
 // --max-wall-time stops after the first step that ends past the limit and
@@ -177,7 +178,14 @@ int run(const Options& o) {
   } else {
     ts.initialConditionsLinearMode(o.problem == "emhdmode" ? 1 : 0, o.problem == "emhdmode" ? 1e-8 : 1e-4);
   }
-  grim::restart(ts, o.restart);
+  const bool restarted = grim::restart(ts, o.restart);
+  // the initial conditions' fullStepDiagnostics (torus/problem.cpp:539):
+  // WriteData at WriteIdx 0 writes the metric files and the first dumps
+  if (o.writeEvery > 0. && !restarted) {
+    grim::dumpMetric(ts);
+    grim::dumpPrimitives(ts, grim::primVarsName(ts.time, o.writeEvery));
+    grim::dumpVTS(ts, grim::dumpVTSName(ts.time, o.writeEvery));
+  }
   const auto start = std::chrono::steady_clock::now();
   const double zones = (double)N1 * N2 * N3;
   double total = 0.;

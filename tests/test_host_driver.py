@@ -129,8 +129,10 @@ def test_driver_restart_continues_the_run(tmp_path):
 
 @pytest.mark.gpu
 def test_driver_write_every_writes_primitives_and_vts(tmp_path):
-    """WriteData (torus/problem.cpp:777-876): every time the run crosses a
-    multiple of --write-every it writes primVarsT%06d.h5 and dump%06d.vts;
+    """WriteData (torus/problem.cpp:777-876): at the start (the initial
+    conditions' fullStepDiagnostics, :539, with the metric files) and every
+    time the run crosses a multiple of --write-every it writes
+    primVarsT%06d.h5 and dump%06d.vts;
     after the last step they hold the final state (rho and u of the VTS equal
     the primitives, the field within the 1e-10 that dx/dX by central
     differences leaves, the points the zone centres)."""
@@ -144,7 +146,27 @@ def test_driver_write_every_writes_primitives_and_vts(tmp_path):
     assert r.returncode == 0, r.stderr
     vts = sorted(tmp_path.glob("dump*.vts"))
     h5 = sorted(tmp_path.glob("primVarsT*.h5"))
-    assert len(vts) == len(h5) == 2
+    # the initial conditions' WriteIdx 0 and one write per step
+    assert len(vts) == len(h5) == 3
+    assert h5[0].name == "primVarsT000000.h5" and vts[0].name == "dump000000.vts"
+    # the metric files of the first write (torus/problem.cpp:780-792):
+    # Minkowski, so gCov = gCon = diag(-1, 1, 1, 1), sqrt(-g) = 1, and the
+    # zone centres' x = X (X3 = 1/2 on the 2D grid)
+    from test_checkpoint import read_dataset
+
+    eta = np.diag([-1., 1., 1., 1.]).ravel()
+    for name in ("gCov", "gCon"):
+        data = read_dataset(str(tmp_path / f"{name}.h5"), name)
+        assert data.shape[-1] == 16 and int(np.prod(data.shape[:-1])) == 64 * 64
+        np.testing.assert_array_equal(data.reshape(-1, 16), np.tile(eta, (64 * 64, 1)))
+    data = read_dataset(str(tmp_path / "sqrtDetg.h5"), "sqrtDetg")
+    np.testing.assert_array_equal(data.ravel(), 1.)
+    data = read_dataset(str(tmp_path / "xCoords.h5"), "xCoords")
+    xc = data.reshape(64 * 64, 3)
+    Xc = (np.arange(64) + 0.5) / 64
+    np.testing.assert_array_equal(xc[:, 0], np.tile(Xc, 64))
+    np.testing.assert_array_equal(xc[:, 1], np.repeat(Xc, 64))
+    np.testing.assert_array_equal(xc[:, 2], 0.5)
     g = 3
     prim = np.fromfile(tmp_path / "prim.bin").reshape(8, 64 + 2 * g, 64 + 2 * g)[:, g:-g, g:-g].reshape(8, -1)
     extent, pts, arrays = checkpoint.read_vts(str(vts[-1]))
