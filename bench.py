@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--no-emhd-batch", action="store_true", help="skip the EMHD uneven-convergence batch")
     ap.add_argument("--torus", type=int, nargs=3, default=[192, 192, 128], metavar=("N1", "N2", "N3"))
     ap.add_argument("--torus-steps", type=int, default=20)
-    ap.add_argument("--torus-paths", default="fd_fp64_lockstep,fd_fp64,mixed",
+    ap.add_argument("--torus-paths", default="fd_fp64_lockstep,fd_fp64,mixed,fd_fp64_fast,mixed_fast",
                     help="comma-separated subset of the torus Newton paths (profiling)")
     ap.add_argument("--no-wave", action="store_true", help="skip the 64x64 linear-wave workload")
     ap.add_argument("--only-torus", action="store_true",
@@ -332,7 +332,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def torus_config(N1, N2, N3):
+def torus_config(N1, N2, N3, explicit_fast=0):
     """Grid of the 3D Kerr torus workload (src/problem/torus/params.cpp:51-105 with
     a = 0.9375, WENO5 + LLF)."""
     import numpy as np
@@ -347,7 +347,8 @@ def torus_config(N1, N2, N3):
                        DerefineThetaHorizon=1, DoCylindrify=1, X1cyl=np.log(4 * Rin), X2cyl=1. / N2,
                        boundaryLeft=capi.BC_OUTFLOW, boundaryRight=capi.BC_OUTFLOW, boundaryBottom=capi.BC_MIRROR,
                        boundaryTop=capi.BC_MIRROR, boundaryBack=capi.BC_PERIODIC, boundaryFront=capi.BC_PERIODIC,
-                       reconstruction=capi.RECON_WENO5, riemannSolver=capi.RIEMANN_LLF, residualScaling=1)
+                       reconstruction=capi.RECON_WENO5, riemannSolver=capi.RIEMANN_LLF, residualScaling=1,
+                       explicitFast=explicit_fast)
 
 
 def time_steps(ts, steps, warmup, graph=False):
@@ -514,11 +515,15 @@ def full_step_bench(args):
         torus["one_step_error_mixed_vs_fd_fp64"] = torus_one_step_error(cfg, N1, N2, N3)
     states = {}
     torus_state = {}
+    mixed_opts = lambda: capi.newton_opts(jacobian=capi.JAC_ANALYTIC, lu_precision=capi.LU_FP32_FP64)  # noqa: E731
     for name, opts in (("fd_fp64_lockstep", capi.newton_opts(semantics=capi.NEWTON_GLOBAL)),
                        ("fd_fp64", capi.newton_opts()),
-                       ("mixed", capi.newton_opts(jacobian=capi.JAC_ANALYTIC, lu_precision=capi.LU_FP32_FP64))):
+                       ("mixed", mixed_opts()),
+                       ("fd_fp64_fast", capi.newton_opts()),
+                       ("mixed_fast", mixed_opts())):
         if name not in args.torus_paths.split(","):
             continue
+        cfg = torus_config(N1, N2, N3, explicit_fast=1 if name.endswith("_fast") else 0)
         p = capi.default_params(True)
         p.adiabaticIndex = 5. / 3
         p.closure = capi.CLOSURE_TORUS
@@ -566,13 +571,16 @@ def full_step_bench(args):
         torus[name]["vs_fd_fp64_rho_u_rel_median"] = float(np.median(rel))
         torus[name]["vs_fd_fp64_scaled_p99"] = float(np.percentile(scaled, 99))
         torus[name]["vs_fd_fp64_scaled_max"] = float(scaled.max())
-    for base, key in (("fd_fp64_lockstep", "speedup_mixed_vs_fd_fp64_lockstep"),
-                      ("fd_fp64", "speedup_mixed_vs_fd_fp64_per_zone")):
-        if "mixed" in torus and base in torus:
-            torus[key] = torus["mixed"]["zone_updates_per_s"] / torus[base]["zone_updates_per_s"]
+    for head, base, key in (("mixed", "fd_fp64_lockstep", "speedup_mixed_vs_fd_fp64_lockstep"),
+                            ("mixed", "fd_fp64", "speedup_mixed_vs_fd_fp64_per_zone"),
+                            ("mixed_fast", "fd_fp64_lockstep", "speedup_mixed_fast_vs_fd_fp64_lockstep"),
+                            ("mixed_fast", "fd_fp64", "speedup_mixed_fast_vs_fd_fp64_per_zone"),
+                            ("mixed_fast", "fd_fp64_fast", "speedup_mixed_fast_vs_fd_fp64_fast")):
+        if head in torus and base in torus:
+            torus[key] = torus[head]["zone_updates_per_s"] / torus[base]["zone_updates_per_s"]
     # HBM roofline of the whole step: algorithmic bytes per zone-step (stored
     # geometry, interior zones) over the measured time per step
-    for name in ("mixed", "fd_fp64", "fd_fp64_lockstep"):
+    for name in ("mixed", "fd_fp64", "fd_fp64_lockstep", "fd_fp64_fast", "mixed_fast"):
         if name in torus:
             gbps = TORUS_BYTES_PER_ZONE_STEP * zones / (torus[name]["ms_per_step"] * 1e-3) / 1e9
             torus[name]["roofline"] = {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -596,7 +604,10 @@ def full_step_bench(args):
                        "(every zone updated while any zone is unconverged, solve.cpp:75-78), fd_fp64 = the same "
                        "FD-FP64 algorithm stopping each zone at its own convergence, mixed = analytic Jacobian "
                        "with FP32 assembly and LU for the first two updates of each zone and FP64 for the "
-                       "compacted stragglers (FP64 residual and update throughout)")
+                       "compacted stragglers (FP64 residual and update throughout); the *_fast paths run the "
+                       "same Newton with the explicit stage in its fast form (shared reciprocals, fused "
+                       "multiply-adds, 40-plane connection; equal to grim's order to rounding), the others with "
+                       "the explicit stage in grim's expression order")
     res["torus_emhd"] = torus
     if args.no_wave:
         return res
