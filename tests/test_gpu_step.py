@@ -256,7 +256,8 @@ def test_fast_mode_steps_bitwise(torch, semantics):
     run_pair(ts, g, po, O.Geometry(g), 10, semantics=semantics)
 
 
-@pytest.mark.parametrize("test", ["shock_tube_2", "collision", "fast_shock", "alfven_wave"])
+@pytest.mark.parametrize("test", ["fast_shock", "slow_shock", "switch_on_slow", "switch_off_fast", "alfven_wave",
+                                  "shock_tube_1", "shock_tube_2", "collision", "stationary_shock"])
 def test_shock_tube_steps_bitwise(torch, test):
     """grim's 1D EMHD shock tests (src/problem/shock_tests/: MINMOD + LLF,
     outflow, conduction and viscosity with the shock closure, grim's three
