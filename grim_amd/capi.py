@@ -24,6 +24,7 @@ CLOSURE_LINEAR_MODES = 0
 CLOSURE_PER_ZONE = 1
 CLOSURE_TORUS = 2
 CLOSURE_SHOCK_TESTS = 3
+CLOSURE_BUOYANCY = 4
 JAC_FD = 0
 JAC_ANALYTIC = 1
 LU_FP64 = 0
@@ -261,6 +262,7 @@ SIGNATURES = {
     "grim_torus_observers": (_i, [_vp, _vp, _vp]),
     "grim_dump_vars": (_i, [_vp, _vp, _vp, _i]),
     "grim_set_ghost_state": (_i, [_vp, _vp, _i]),
+    "grim_set_x1_side_state": (_i, [_vp, _vp, _i]),
     "grim_riemann_solve": (_i, [_vp, _vp, _vp, _i, _vp]),
     "grim_element_fluxes": (_i, [_vp, _vp, _i, _i, _vp]),
     "grim_element_char_speeds": (_i, [_vp, _vp, _i, _i, _vp, _vp]),

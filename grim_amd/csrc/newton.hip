@@ -258,7 +258,9 @@ int validBatch(const grim_params* p, const grim_zone_batch* b) {
   if (!b->prim || !b->geom || !b->rhs) return 0;
   if ((p->conduction || p->viscosity) && !b->prim_old) return 0;
   if (p->closure == GRIM_CLOSURE_PER_ZONE && (p->conduction || p->viscosity) && !b->closure) return 0;
-  if (p->closure == GRIM_CLOSURE_TORUS && (p->conduction || p->viscosity) && !b->radius) return 0;
+  if ((p->closure == GRIM_CLOSURE_TORUS || p->closure == GRIM_CLOSURE_BUOYANCY) && (p->conduction || p->viscosity) &&
+      !b->radius)
+    return 0;
   if (!b->dt_prime_device && !(b->dt_prime > 0.)) return 0;
   return 1;
 }

@@ -50,7 +50,8 @@ inline int checkLaunch(const char* what) {
 inline int validParams(const grim_params* p) {
   if (!p) return 0;
   if (p->conduction < 0 || p->conduction > 1 || p->viscosity < 0 || p->viscosity > 1) return 0;
-  if (p->closure < GRIM_CLOSURE_LINEAR_MODES || p->closure > GRIM_CLOSURE_SHOCK_TESTS) return 0;
+  if (p->closure < GRIM_CLOSURE_LINEAR_MODES || p->closure > GRIM_CLOSURE_BUOYANCY) return 0;
+  if (p->closure == GRIM_CLOSURE_BUOYANCY && p->viscosity) return 0;
   if (p->maxNonLinearIter < 0 || p->maxLineSearchIters < 0) return 0;
   return 1;
 }

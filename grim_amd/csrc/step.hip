@@ -1474,6 +1474,7 @@ struct grim_ctx {
   double* floors;  // 2 planes: rho and u floor values
   double* dump;    // dump variables staged for a host copy (first such call)
   double* ghostState;  // dof planes whose ghost zones the half stage imposes, or null
+  double* sideState;   // rho, u, u~^1 planes the X1 sides take in both stages, or null
 };
 
 namespace grim {
@@ -1795,6 +1796,35 @@ __global__ __launch_bounds__(kBlock) void ghostStateKernel(Dims D, int nvars, co
   for (int v = 0; v < nvars; v++) prim[v * D.n + z] = ic[v * D.n + z];
 }
 
+// setProblemSpecificBCs of buoyancy_instabilities (problem.cpp:221-293):
+// the zones of X1 index 0..g (inclusive, as af::seq(0, numGhost)) and
+// N1+g..N1+2g-1 take rho, u, u~^1 of the stored state, zero transverse
+// velocity, zero field and (conduction) zero q~.  One thread per zone of
+// the two X1 slabs, (2g + 1) NT2 NT3 threads; the switch follows the three
+// planes, as ghostStateKernel's.
+template <class L>
+__global__ __launch_bounds__(kBlock) void sideStateKernel(Dims D, const double* __restrict__ ic,
+                                                          double* __restrict__ prim) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = 2 * D.g[0] + 1;
+  if (t >= (int64_t)w * D.NT[1] * D.NT[2]) return;
+  if (ic[3 * D.n] == 0.) return;
+  const int ii = (int)(t % w);
+  const int64_t jk = t / w;
+  const int i = ii <= D.g[0] ? ii : D.N[0] + D.g[0] + (ii - D.g[0] - 1);
+  const int64_t z = i + (int64_t)D.NT[0] * jk;
+  const int64_t n = D.n;
+  prim[RHO * n + z] = ic[z];
+  prim[UU * n + z] = ic[n + z];
+  prim[U1 * n + z] = ic[2 * n + z];
+  prim[U2 * n + z] = 0.;
+  prim[U3 * n + z] = 0.;
+  prim[L::B1 * n + z] = 0.;
+  prim[L::B2 * n + z] = 0.;
+  prim[L::B3 * n + z] = 0.;
+  if constexpr (L::COND_ON) prim[L::Q * n + z] = 0.;
+}
+
 // setProblemSpecificBCs of the torus: inflowCheck on both radial ghost
 // slabs, then fixPoles (problem.cpp:1217-1241).
 template <class L>
@@ -1849,6 +1879,12 @@ int stagePrepare(grim_ctx* x, int st) {
   if (x->ghostState && st == 0) {
     ghostStateKernel<<<blocksFor(D.n), kBlock, 0, x->s>>>(D, L::DOF, x->ghostState, primStage);
     if (int rc = checkLaunch("ghost state")) return rc;
+  }
+  // buoyancy_instabilities' hook writes the stage state in both stages
+  if (x->sideState) {
+    const int64_t count = (int64_t)(2 * D.g[0] + 1) * D.NT[1] * D.NT[2];
+    sideStateKernel<L><<<blocksFor(count), kBlock, 0, x->s>>>(D, x->sideState, primStage);
+    if (int rc = checkLaunch("X1 side state")) return rc;
   }
   if (int rc = explicitSources<L>(x, primStage, x->primOld, x->sExp, L::EMHD ? x->sImp : nullptr,
                                   st == 0 ? x->consOld : nullptr))
@@ -2115,6 +2151,7 @@ int grim_destroy(grim_ctx* x) {
   if (x->floors) (void)hipFree(x->floors);
   if (x->dump) (void)hipFree(x->dump);
   if (x->ghostState) (void)hipFree(x->ghostState);
+  if (x->sideState) (void)hipFree(x->sideState);
   delete x;
   return GRIM_OK;
 }
@@ -2493,6 +2530,23 @@ int grim_set_ghost_state(grim_ctx* x, const double* prim_ic, int on_device) {
     GRIM_HIP_TRY(hipMemcpyAsync(x->ghostState, prim_ic, bytes,
                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, x->s));
   GRIM_HIP_TRY(hipMemcpyAsync(x->ghostState + (size_t)x->dof * x->D.n, x->hostBuf, sizeof(double),
+                              hipMemcpyHostToDevice, x->s));
+  GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+  return GRIM_OK;
+}
+
+int grim_set_x1_side_state(grim_ctx* x, const double* prim_ic, int on_device) {
+  GRIM_REQUIRE(x, "grim_set_x1_side_state: null context");
+  // rho, u, u~^1 planes, then the on/off flag, kept until grim_destroy as
+  // the ghost state's
+  const size_t bytes = sizeof(double) * 3 * (size_t)x->D.n;
+  if (!prim_ic && !x->sideState) return GRIM_OK;
+  if (!x->sideState) GRIM_HIP_TRY(hipMalloc(&x->sideState, bytes + sizeof(double)));
+  x->hostBuf[0] = prim_ic ? 1. : 0.;
+  if (prim_ic)
+    GRIM_HIP_TRY(hipMemcpyAsync(x->sideState, prim_ic, bytes,
+                                on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, x->s));
+  GRIM_HIP_TRY(hipMemcpyAsync(x->sideState + 3 * (size_t)x->D.n, x->hostBuf, sizeof(double),
                               hipMemcpyHostToDevice, x->s));
   GRIM_HIP_TRY(hipStreamSynchronize(x->s));
   return GRIM_OK;

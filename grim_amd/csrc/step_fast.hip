@@ -139,6 +139,9 @@ __device__ __forceinline__ void closureFast(const double* P, const Kinematics& k
       c.tau = 0.1;
       c.chi = prm.condAlpha * cs2 * c.tau;
       c.nu = prm.viscAlpha * cs2 * c.tau;
+    } else if (prm.closure == GRIM_CLOSURE_BUOYANCY) {
+      c.chi = sqrt(radius);
+      c.tau = 1.2 * c.chi / cs2;
     } else {
       const double dyn = radius * sqrt(radius);
       double tau = dyn;

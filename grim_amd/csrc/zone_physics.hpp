@@ -193,6 +193,12 @@ __device__ __forceinline__ void closure(const double* P, const Kinematics& k,
     c.tau = 0.1;
     c.chi = prm.condAlpha * cs * cs * c.tau;
     c.nu = prm.viscAlpha * cs * cs * c.tau;
+  } else if (prm.closure == GRIM_CLOSURE_BUOYANCY) {
+    // src/problem/buoyancy_instabilities/problem.cpp:11-21; pow(r, 0.5) as
+    // sqrt(r), correctly rounded on host and device
+    c.chi = sqrt(radius);
+    c.tau = 1.2 * c.chi / (cs * cs);
+    c.nu = 0.;
   } else {
     // pow(r, 1.5) and pow(c_s, 3) of problem.cpp:7-55 as r sqrt(r) and
     // c_s c_s c_s: equal to the last bit, reproducible on host and device,

@@ -3,16 +3,22 @@
 // cycles per second) for the two problems of this path, through the C++
 // interface of timestepper.hpp.
 //
-//   grim_step [--problem fastmode|emhdmode|torus] [--N N1 N2 N3] [--steps K]
-//             [--solver fd|fd_lockstep|mixed] [--maxiter M] [--dump FILE]
-//             [--observe-every DT] [--write-every DT] [--restart]
-//             [--max-wall-time SECONDS] [--dt0 DT]
+//   grim_step [--problem fastmode|emhdmode|torus|buoyancy] [--N N1 N2 N3]
+//             [--steps K] [--solver fd|fd_lockstep|mixed] [--maxiter M]
+//             [--dump FILE] [--observe-every DT] [--write-every DT]
+//             [--restart] [--max-wall-time SECONDS] [--dt0 DT]
+//             [--atmosphere DIR]
 //
 // fastmode: 2D Minkowski ideal MHD, 64x64 periodic, WENO5 + LLF, the fast
 //           magnetosonic eigenmode at amplitude 1e-4, dt0 = 0.5/N1;
 // emhdmode: grim's EMHD linear mode (MINMOD + HLL, amplitude 1e-8);
 // torus:    3D MKS torus (a = 0.9375, r_in = 6, r_max = 12, SANE beta 100),
-//           EMHD with the torus closure, WENO5 + LLF, relative residual test.
+//           EMHD with the torus closure, WENO5 + LLF, relative residual test;
+// buoyancy: buoyancy_instabilities' 2D atmosphere (r in [200, 300], a = 0),
+//           conduction with the problem's closure, MINMOD + LLF, from grim's
+//           four atmosphere_soln_*.txt files in --atmosphere DIR (default
+//           the working directory), 4 % noise in u~^1, the X1-side hook;
+//           maxNonLinearIter 3 unless --maxiter.
 // --dump writes primOld (numVars planes of N1T*N2T*N3T doubles, X1 fastest)
 // after the last step.  --observe-every DT prints the torus observers
 // whenever time crosses a multiple of DT (fullStepDiagnostics,
@@ -45,13 +51,14 @@ struct Options {
   int N[3] = {0, 0, 0};
   int steps = 10;
   std::string solver = "fd";
-  int maxIter = 10;
+  int maxIter = 0;  // 0: the problem's maxNonLinearIter
   std::string dump;
   double observeEvery = 0.;
   double writeEvery = 0.;
   bool restart = false;
   double maxWallTime = -1.;
   double dt0 = 0.;
+  std::string atmosphere = ".";
 };
 
 Options parse(int argc, char** argv) {
@@ -84,6 +91,8 @@ Options parse(int argc, char** argv) {
       o.maxWallTime = std::atof(next());
     } else if (a == "--dt0") {
       o.dt0 = std::atof(next());
+    } else if (a == "--atmosphere") {
+      o.atmosphere = next();
     } else {
       throw std::runtime_error("unknown argument " + a);
     }
@@ -106,16 +115,19 @@ void setSolver(grim::Params& prm, const std::string& solver) {
 
 int run(const Options& o) {
   const bool torus = o.problem == "torus";
-  const bool emhd = torus || o.problem == "emhdmode";
+  const bool buoyancy = o.problem == "buoyancy";
+  const bool emhd = torus || buoyancy || o.problem == "emhdmode";
+  if (!torus && !buoyancy && o.problem != "fastmode" && o.problem != "emhdmode")
+    throw std::runtime_error("unknown problem " + o.problem);
   grim::Params prm(emhd);
   setSolver(prm, o.solver);
-  prm.physics.maxNonLinearIter = o.maxIter;
+  prm.physics.maxNonLinearIter = o.maxIter > 0 ? o.maxIter : (buoyancy ? 3 : 10);
   int N1, N2, N3, dim;
   double X1s = 0., X1e = 1., X2s = 0., X2e = 1., X3s = 0., X3e = 1., dt0;
   int bl = GRIM_BC_PERIODIC, br = GRIM_BC_PERIODIC, bb = GRIM_BC_PERIODIC, bt = GRIM_BC_PERIODIC,
       bk = GRIM_BC_PERIODIC, bf = GRIM_BC_PERIODIC;
   int metric = GRIM_METRIC_MINKOWSKI;
-  const double a = 0.9375;
+  double a = 0.9375, hSlope = 0.3;
   if (torus) {
     N1 = o.N[0] ? o.N[0] : 192;
     N2 = o.N[1] ? o.N[1] : 192;
@@ -139,6 +151,48 @@ int run(const Options& o) {
     prm.physics.closure = GRIM_CLOSURE_TORUS;
     prm.residualScaling = 1;
     dt0 = 0.005;
+  } else if (buoyancy) {
+    // buoyancy_instabilities/params.cpp:8-96 (a = 0, one X3 zone of grim's
+    // N3 = 128 over [0, pi])
+    N1 = o.N[0] ? o.N[0] : 64;
+    N2 = o.N[1] ? o.N[1] : 64;
+    N3 = 1;
+    dim = 2;
+    metric = GRIM_METRIC_MKS;
+    const double Rin = 200.;
+    X1s = std::log(Rin);
+    X1e = std::log(300.);
+    X2s = 1e-8;
+    X2e = 1. - 1e-8;
+    X3s = 0.;
+    X3e = M_PI / 128.;
+    bl = br = GRIM_BC_OUTFLOW;
+    bb = bt = GRIM_BC_MIRROR;
+    a = 0.;
+    hSlope = 1.;
+    prm.DerefineThetaHorizon = 1;
+    prm.DoCylindrify = 1;
+    prm.X1cyl = std::log(4. * Rin);
+    prm.X2cyl = 3. / N2;
+    prm.reconstruction = GRIM_RECON_MINMOD;
+    prm.riemannSolver = GRIM_RIEMANN_LLF;
+    prm.slopeLimTheta = 2.;
+    prm.CourantFactor = 0.5;
+    grim_params& ph = prm.physics;
+    ph.conduction = 1;
+    ph.viscosity = 0;
+    ph.highOrderTermsConduction = ph.highOrderTermsViscosity = 1;
+    ph.ConductionAlpha = ph.ViscosityAlpha = 1.;
+    ph.ConductionClosureFactor = ph.ViscosityClosureFactor = 1.;
+    ph.adiabaticIndex = 5. / 3.;
+    ph.rhoFloorInFluidElement = ph.uFloorInFluidElement = 1e-20;
+    ph.bSqrFloorInFluidElement = ph.temperatureFloorInFluidElement = 1e-20;
+    ph.closure = GRIM_CLOSURE_BUOYANCY;
+    ph.maxLineSearchIters = 3;
+    ph.nonlinearsolve_atol = 1e-6;
+    ph.JacobianAssembleEpsilon = 4e-8;
+    ph.linesearchfloor = 1e-24;
+    dt0 = 0.01;
   } else {
     N1 = o.N[0] ? o.N[0] : 64;
     N2 = o.N[1] ? o.N[1] : 64;
@@ -155,7 +209,7 @@ int run(const Options& o) {
   }
   const int numVars = 8 + prm.physics.conduction + prm.physics.viscosity;
   // grim's constructor order: ..., boundaryTop, boundaryBottom, boundaryFront, boundaryBack
-  grim::timeStepper ts(N1, N2, N3, dim, numVars, 3, 0., dt0, bl, br, bt, bb, bf, bk, metric, a, 0.3, X1s,
+  grim::timeStepper ts(N1, N2, N3, dim, numVars, 3, 0., dt0, bl, br, bt, bb, bf, bk, metric, a, hSlope, X1s,
                        X1e, X2s, X2e, X3s, X3e, prm);
   if (torus) {
     grim_torus_params t{};
@@ -175,6 +229,8 @@ int run(const Options& o) {
     t.BsqrOverUMax = 500.;
     t.seed = 20261202ull;
     ts.initialConditionsTorus(t);
+  } else if (buoyancy) {
+    ts.initialConditionsBuoyancy(o.atmosphere, 4e-2, 20261203ull);
   } else {
     ts.initialConditionsLinearMode(o.problem == "emhdmode" ? 1 : 0, o.problem == "emhdmode" ? 1e-8 : 1e-4);
   }

@@ -22,8 +22,11 @@
 // reconstruction.hpp mirror the classes they take).
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
+#include <fstream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -169,6 +172,85 @@ class timeStepper {
     check(grim_init_torus(ctx, &torus), "grim_init_torus");
     isTorus = true;
     setTime(time, dt);
+  }
+
+  // timeStepper::initialConditions of buoyancy_instabilities (src/problem/
+  // buoyancy_instabilities/problem.cpp:22-216) and from then on its
+  // setProblemSpecificBCs (:221-293, grim_set_x1_side_state):
+  //  * rho, u and (conduction) q~ of X1 zone i, ghosts included, from grim's
+  //    input files in dir (atmosphere_soln_rho.txt, _u.txt, _phi.txt,
+  //    _rCoords.txt: N1 + 2 numGhost values, one per line), after checking
+  //    |r(i) - rCoords[i]| <= 1e-10 on the first X2 / X3 line;
+  //  * u~^1 of the static Boyer-Lindquist observer, (c u^r / r - sqrt(-a b^4
+  //    - a b^2 c (u^r)^2 / r^2 - b^2 c)) / (a b^2 + c) with a = g_11,
+  //    b = g^01, c = g^00, u^r = 0, times 1 + amplitude (U - 1/2) on the
+  //    bulk zones (U: the library's counter-based stream in place of
+  //    PETSc's RAND48); u~^2 = u~^3 = 0;
+  //  * B^1 = B^2 = 0, B^3 = 1e-5 / sqrt(-g).
+  // Errors carry grim's messages.
+  void initialConditionsBuoyancy(const std::string& dir, double amplitude, uint64_t seed) {
+    const int64_t n = zones();
+    const int ng = cfg.numGhost;
+    const int N1T = cfg.N1 + 2 * ng;
+    const int N2T = cfg.dim > 1 ? cfg.N2 + 2 * ng : 1;
+    const int N3T = cfg.dim > 2 ? cfg.N3 + 2 * ng : 1;
+    static const char* names[4] = {"atmosphere_soln_rho.txt", "atmosphere_soln_u.txt", "atmosphere_soln_phi.txt",
+                                   "atmosphere_soln_rCoords.txt"};
+    std::vector<double> prof[4];
+    for (int f = 0; f < 4; f++) {
+      std::ifstream in(dir + "/" + names[f]);
+      if (!in) throw std::runtime_error("Input data files not found!");
+      std::string line;
+      for (int i = 0; i < N1T; i++) {
+        if (!std::getline(in, line))
+          throw std::runtime_error(
+              "Found the input data files but error in reading them! Check number of grid zones in python script");
+        prof[f].push_back(std::atof(line.c_str()));
+      }
+    }
+    std::vector<double> geom((size_t)GRIM_LOC_PLANES * n);
+    check(grim_get_geometry(ctx, 0, geom.data(), 0), "grim_get_geometry");
+    const double* r = geom.data() + 22 * n;
+    for (int i = 0; i < N1T; i++)
+      if (std::fabs(r[i] - prof[3][i]) > 1e-10) {
+        char msg[200];
+        std::snprintf(msg, sizeof msg,
+                      "Mismatch in rCoords! Check r coords in python script. r = %.18f, rCoords = %.18f, i = %d", r[i],
+                      prof[3][i], i);
+        throw std::runtime_error(msg);
+      }
+    std::vector<double> prim((size_t)numVars * n, 0.), side((size_t)3 * n);
+    const int B3 = numVars - 1;
+    for (int k = 0; k < N3T; k++)
+      for (int j = 0; j < N2T; j++)
+        for (int i = 0; i < N1T; i++) {
+          const int64_t z = i + (int64_t)N1T * (j + (int64_t)N2T * k);
+          const double a = geom[4 * n + z], b = geom[11 * n + z], c = geom[10 * n + z], rz = r[z];
+          const double uCon1 = 0. * prim[2 * n + z];
+          const double vUpr =
+              (c * uCon1 / rz - std::sqrt(-a * b * b * b * b - a * b * b * c * uCon1 * uCon1 / (rz * rz) - b * b * c)) /
+              (a * b * b + c);
+          const bool bulk = i >= ng && i < cfg.N1 + ng && (cfg.dim < 2 || (j >= ng && j < cfg.N2 + ng)) &&
+                            (cfg.dim < 3 || (k >= ng && k < cfg.N3 + ng));
+          prim[z] = prof[0][i];
+          prim[n + z] = prof[1][i];
+          if (phys.conduction) prim[5 * n + z] = prof[2][i];
+          prim[2 * n + z] = bulk ? vUpr * (1. + amplitude * (uniform(seed, (uint64_t)z) - 0.5)) : vUpr;
+          prim[B3 * n + z] = 1e-5 / geom[15 * n + z];
+          for (int v = 0; v < 3; v++) side[v * n + z] = prim[v * n + z];
+        }
+    setPrim(prim);
+    check(grim_set_x1_side_state(ctx, side.data(), 0), "grim_set_x1_side_state");
+    setTime(time, dt);
+  }
+  // The library's counter-based uniform in [0, 1) of zone z: splitmix64 of
+  // seed + G (32 z + 1), G = 0x9E3779B97F4A7C15 (the torus's noise stream).
+  static double uniform(uint64_t seed, uint64_t z) {
+    uint64_t x = seed + 0x9E3779B97F4A7C15ull * (32ull * z + 1ull);
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    x = x ^ (x >> 31);
+    return (double)(x >> 11) * (1.0 / 9007199254740992.0);
   }
 
   // grid::setVars / getVars: numVars planes of zones() doubles on the host.

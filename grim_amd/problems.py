@@ -223,3 +223,178 @@ def bondi_prim(grid: capi.GridConfig, p: capi.Params, **kw) -> np.ndarray:
         prim[2][..., i] = (ur + ut * 2. / (R + 2.)) / R
         prim[b1][..., i] = bMag / R ** 3
     return prim.reshape(p.dof, -1)
+
+
+# ---------------------------------------------------------------------------
+# Buoyancy instabilities of a conducting atmosphere (src/problem/
+# buoyancy_instabilities/)
+# ---------------------------------------------------------------------------
+# params.cpp:8-96.  grim's 2D grid has one X3 zone of width (X3End -
+# X3Start) / N3 with N3 = 128 (the DMDA of a 2D run ignores N3, the
+# coordinates do not): X3 = pi / 256 at the zone centre, which only the
+# dumped coordinates see.
+BUOYANCY = dict(Rin=200., Rout=300., InitialPerturbationAmplitude=4e-2, N3=128, X3End=np.pi)
+BUOYANCY_FILES = ("atmosphere_soln_rho.txt", "atmosphere_soln_u.txt", "atmosphere_soln_phi.txt",
+                  "atmosphere_soln_rCoords.txt")
+
+
+def buoyancy_grid(N1: int = 64, N2: int = 64) -> capi.GridConfig:
+    """params.cpp: 2D modified Kerr-Schild, spin 0, hSlope 1, r in [Rin, Rout]
+    with Rin = 100 (1 + sqrt(1 - a^2)) = 200 and Rout = 300, X2 in [1e-8,
+    1 - 1e-8], theta derefined and cylindrified (X1cyl = ln 4 Rin, X2cyl =
+    3 / N2), outflow in X1, mirror in X2, MINMOD (theta 2) + LLF, Courant
+    0.5."""
+    from .timestepper import grid_config
+
+    Rin = BUOYANCY["Rin"]
+    return grid_config(N1=N1, N2=N2, N3=1, dim=2, metric=capi.METRIC_MKS, blackHoleSpin=0., hSlope=1.,
+                       X1Start=np.log(Rin), X1End=np.log(BUOYANCY["Rout"]), X2Start=1e-8, X2End=1. - 1e-8,
+                       X3Start=0., X3End=BUOYANCY["X3End"] / BUOYANCY["N3"], DerefineThetaHorizon=1,
+                       DoCylindrify=1, X1cyl=np.log(4. * Rin), X2cyl=3. / N2, boundaryLeft=capi.BC_OUTFLOW,
+                       boundaryRight=capi.BC_OUTFLOW, boundaryBottom=capi.BC_MIRROR, boundaryTop=capi.BC_MIRROR,
+                       boundaryBack=capi.BC_PERIODIC, boundaryFront=capi.BC_PERIODIC,
+                       reconstruction=capi.RECON_MINMOD, riemannSolver=capi.RIEMANN_LLF, slopeLimTheta=2.,
+                       CourantFactor=0.5, maxDtIncrement=1.3)
+
+
+def buoyancy_params() -> capi.Params:
+    """params.cpp: conduction with the high-order rescaling, no viscosity,
+    ConductionAlpha 1, Gamma 5/3, fluid-element floors 1e-20, 3 Newton
+    iterations of at most 3 line-search steps to sum R^2 <= 1e-6, and the
+    problem's closure chi = sqrt(r), tau = 1.2 chi / c_s^2
+    (GRIM_CLOSURE_BUOYANCY, problem.cpp:11-21)."""
+    p = capi.default_params(True)
+    p.conduction, p.viscosity = 1, 0
+    p.highOrderTermsConduction = p.highOrderTermsViscosity = 1
+    p.ConductionAlpha = p.ViscosityAlpha = 1.
+    p.ConductionClosureFactor = p.ViscosityClosureFactor = 1.
+    p.adiabaticIndex = 5. / 3
+    p.rhoFloorInFluidElement = p.uFloorInFluidElement = 1e-20
+    p.bSqrFloorInFluidElement = p.temperatureFloorInFluidElement = 1e-20
+    p.closure = capi.CLOSURE_BUOYANCY
+    p.maxNonLinearIter, p.maxLineSearchIters = 3, 3
+    p.nonlinearsolve_atol, p.JacobianAssembleEpsilon, p.linesearchfloor = 1e-6, 4e-8, 1e-24
+    return p
+
+
+def zone_radii(grid: capi.GridConfig) -> np.ndarray:
+    """r = e^X1 of the N1 + 2 numGhost zone centres of an X1 line (MKS)."""
+    X1 = grid.X1Start + (np.arange(grid.N1 + 2 * grid.numGhost) - grid.numGhost + 0.5) * (
+        (grid.X1End - grid.X1Start) / grid.N1)
+    return np.exp(X1)
+
+
+def hydrostatic_atmosphere(grid: capi.GridConfig, p: capi.Params, rho0: float = 1., T0: float = 5e-3):
+    """The four input profiles of the problem (rho, u, q~, r per X1 zone,
+    ghosts included), whose generator the reference does not hold: a static
+    atmosphere in hydrostatic equilibrium around a Schwarzschild hole (M =
+    1) with T = p / rho = T0 r0 / r, r0 = the first profile radius.  For a
+    static perfect fluid dp/dr = -(rho + u + p) / (r^2 (1 - 2/r)) with u =
+    p / (Gamma - 1), which integrates in closed form:
+
+        p(r) = p0 ((r - 2) / (r0 - 2))^(-1 / (T0 r0))
+                  (((r - 2) / r) / ((r0 - 2) / r0))^(-Gamma / (2 (Gamma - 1)))
+
+    with p0 = rho0 T0; q~ = 0 (no heat flux along the toroidal field)."""
+    r = zone_radii(grid)
+    r0 = r[0]
+    gam = p.adiabaticIndex
+    pres = rho0 * T0 * ((r - 2.) / (r0 - 2.)) ** (-1. / (T0 * r0)) * (
+        ((r - 2.) / r) / ((r0 - 2.) / r0)) ** (-gam / (2. * (gam - 1.)))
+    T = T0 * r0 / r
+    return pres / T, pres / (gam - 1.), np.zeros_like(r), r
+
+
+def write_atmosphere_files(directory: str, rho, u, q, rCoords) -> None:
+    """The problem's input files (problem.cpp:3-9): one value per line, X1
+    zones -numGhost .. N1 + numGhost - 1, printed to 17 significant digits so
+    that atof reads the same doubles back."""
+    import os
+
+    for name, a in zip(BUOYANCY_FILES, (rho, u, q, rCoords)):
+        with open(os.path.join(directory, name), "w") as f:
+            f.writelines(f"{v:.17e}\n" for v in np.asarray(a, dtype=np.float64))
+
+
+def read_atmosphere_files(grid: capi.GridConfig, directory: str = "."):
+    """The file reading of initialConditions (problem.cpp:24-114): N1 + 2
+    numGhost lines of each file; a missing file or a short one is an error,
+    with grim's messages."""
+    import os
+
+    count = grid.N1 + 2 * grid.numGhost
+    out = []
+    for name in BUOYANCY_FILES:
+        path = os.path.join(directory, name)
+        if not os.path.exists(path):
+            raise FileNotFoundError("Input data files not found!")
+        with open(path) as f:
+            lines = f.readlines()
+        if len(lines) < count:
+            raise ValueError("Found the input data files but error in reading them! "
+                             "Check number of grid zones in python script")
+        out.append(np.array([float(v) for v in lines[:count]]))
+    return tuple(out)
+
+
+def noise_stream(seed: int, n: int) -> np.ndarray:
+    """Uniforms in [0, 1) of zones 0..n-1 from the library's counter-based
+    stream (splitmix64 of seed + G (32 z + 1), G = 0x9E3779B97F4A7C15), the
+    stream the torus perturbation draws; it replaces PETSc's RAND48."""
+    z = np.arange(n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = np.uint64(seed) + np.uint64(0x9E3779B97F4A7C15) * (np.uint64(32) * z + np.uint64(1))
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        x = x ^ (x >> np.uint64(31))
+    return (x >> np.uint64(11)).astype(np.float64) * (1. / 9007199254740992.)
+
+
+def buoyancy_prim(grid: capi.GridConfig, p: capi.Params, geom_center: np.ndarray, rho1D, u1D, q1D, rCoords1D,
+                  amplitude: float = BUOYANCY["InitialPerturbationAmplitude"], seed: int = 20261203):
+    """timeStepper::initialConditions (problem.cpp:22-216) on every zone:
+
+    * rho, u and (conduction) q~ of X1 zone i from the profiles, after
+      checking |r(i) - rCoords1D[i]| <= 1e-10 on the first X2 / X3 line;
+    * u~^1 = (c u^r / r - sqrt(-a b^4 - a b^2 c (u^r)^2 / r^2 - b^2 c)) /
+      (a b^2 + c) with a = g_11, b = g^01, c = g^00 and u^r = 0 (the
+      static Boyer-Lindquist observer, in grim's expression order), times
+      1 + amplitude (uniform - 1/2) on the bulk zones (the noise stream in
+      place of PETSc's RAND48); u~^2 = u~^3 = 0;
+    * B^1 = B^2 = 0, B^3 = 1e-5 / sqrt(-g).
+
+    geom_center: the zone-centre geometry planes (TimeStepper.geometry(0)).
+    Returns (prim, side_state): side_state holds rho, u and u~^1 of primIC,
+    the planes setProblemSpecificBCs imposes on the X1 sides
+    (TimeStepper.set_x1_side_state)."""
+    n3, n2, n1 = grid.shape
+    n = n1 * n2 * n3
+    geom = np.asarray(geom_center).reshape(-1, n)
+    r = geom[22].reshape(n3, n2, n1)
+    rC = np.asarray(rCoords1D, dtype=np.float64)
+    for i in range(n1):
+        if abs(r[0, 0, i] - rC[i]) > 1e-10:
+            raise ValueError(f"Mismatch in rCoords! Check r coords in python script. r = {r[0, 0, i]:.18f}, "
+                             f"rCoords = {rC[i]:.18f}, i = {i}")
+    prim = np.zeros((p.dof, n3, n2, n1))
+    prim[0] = np.asarray(rho1D, dtype=np.float64)[None, None, :]
+    prim[1] = np.asarray(u1D, dtype=np.float64)[None, None, :]
+    if p.conduction:
+        prim[5] = np.asarray(q1D, dtype=np.float64)[None, None, :]
+    a = geom[4].reshape(n3, n2, n1)   # gCov[1][1]
+    b = geom[11].reshape(n3, n2, n1)  # gCon[0][1]
+    c = geom[10].reshape(n3, n2, n1)  # gCon[0][0]
+    uCon1 = 0. * prim[2]
+    vUpr = (c * uCon1 / r - np.sqrt(-a * b * b * b * b - a * b * b * c * uCon1 * uCon1 / (r * r) - b * b * c)) / (
+        a * b * b + c)
+    u1 = vUpr.copy()
+    g = grid.numGhost
+    bulk = (slice(0, 1) if grid.dim < 3 else slice(g, n3 - g), slice(g, n2 - g) if grid.dim > 1 else slice(0, 1),
+            slice(g, n1 - g))
+    rand = noise_stream(seed, n).reshape(n3, n2, n1)
+    u1[bulk] = vUpr[bulk] * (1. + amplitude * (rand[bulk] - 0.5))
+    prim[2] = u1
+    b1 = p.dof - 3
+    prim[b1 + 2] = 1e-5 / geom[15].reshape(n3, n2, n1)
+    side = np.stack([prim[0], prim[1], prim[2]]).reshape(3, n)
+    return prim.reshape(p.dof, -1), side

@@ -180,6 +180,27 @@ class TimeStepper:
             ptr = prim_ic.ctypes.data
         capi.check(capi.lib().grim_set_ghost_state(self._ctx, ptr, int(on_device)), "grim_set_ghost_state")
 
+    def set_x1_side_state(self, prim_ic):
+        """setProblemSpecificBCs of buoyancy_instabilities (problem.cpp:
+        221-293): in both stages the X1 sides of the stage state (X1 index
+        0..numGhost, the first bulk zone included, and the right ghosts) take
+        rho, u, u~^1 of prim_ic (3 x zones, host array or device tensor),
+        zero u~^2, u~^3, B and q~; None switches it off."""
+        if prim_ic is None:
+            capi.check(capi.lib().grim_set_x1_side_state(self._ctx, None, 0), "grim_set_x1_side_state")
+            return
+        on_device = hasattr(prim_ic, "is_cuda") and prim_ic.is_cuda
+        if on_device:
+            if prim_ic.numel() != 3 * self.n:
+                raise ValueError("set_x1_side_state: expected 3 x zones values")
+            ptr = capi._ptr(prim_ic, numel=3 * self.n)
+        else:
+            prim_ic = np.ascontiguousarray(prim_ic, dtype=np.float64)
+            if prim_ic.size != 3 * self.n:
+                raise ValueError("set_x1_side_state: expected 3 x zones values")
+            ptr = prim_ic.ctypes.data
+        capi.check(capi.lib().grim_set_x1_side_state(self._ctx, ptr, int(on_device)), "grim_set_x1_side_state")
+
     def dump_vars(self, prim=None):
         """fullStepDiagnostics' dump variables (torus/problem.cpp:808-875) of
         prim (None: primOld) as a device tensor (dof + 5, N1*N2*N3), X1

@@ -37,8 +37,12 @@ typedef enum grim_closure {
   GRIM_CLOSURE_LINEAR_MODES = 0, /* src/problem/linear_modes/problem.cpp:3-8 */
   GRIM_CLOSURE_PER_ZONE = 1,     /* same form, per-zone conduction/viscosity alpha */
   GRIM_CLOSURE_TORUS = 2,        /* src/problem/torus/problem.cpp:7-55 */
-  GRIM_CLOSURE_SHOCK_TESTS = 3   /* src/problem/shock_tests/problem.cpp:13-18: tau = 0.1,
+  GRIM_CLOSURE_SHOCK_TESTS = 3,  /* src/problem/shock_tests/problem.cpp:13-18: tau = 0.1,
                                     chi (nu) = ConductionAlpha (ViscosityAlpha) c_s^2 tau */
+  GRIM_CLOSURE_BUOYANCY = 4      /* src/problem/buoyancy_instabilities/problem.cpp:11-21:
+                                    chi = sqrt(r), tau = 1.2 chi / c_s^2 (r: the radius
+                                    of the zone's xCoords); sets no viscosity coefficient,
+                                    so it takes conduction only */
 } grim_closure;
 
 /* The part of grim's `namespace params` (src/params.hpp:96-210) that the
@@ -82,7 +86,7 @@ typedef struct grim_zone_batch {
   const double* prim_old;     /* dof planes (EMHD: elemOld) or NULL */
   const double* prim_coef;    /* dof planes (EMHD: elemHalfStep) or NULL = prim_old */
   const double* closure;      /* 2 planes (conduction, viscosity alpha) or NULL */
-  const double* radius;       /* 1 plane (torus closure) or NULL */
+  const double* radius;       /* 1 plane (torus and buoyancy closures) or NULL */
   const double* residual_mask;/* 1 plane or NULL (= 1) */
   double dt_prime;            /* dt/2 (half step) or dt (full step) */
   const double* dt_prime_device; /* optional: device address holding dt_prime,
@@ -528,6 +532,20 @@ int grim_torus_observers(grim_ctx* ctx, const double* prim, grim_torus_observabl
  * before that call does not contain the hook.  Synchronises the context's
  * stream. */
 int grim_set_ghost_state(grim_ctx* ctx, const double* prim_ic, int on_device);
+
+/* Replaces setProblemSpecificBCs of buoyancy_instabilities (src/problem/
+ * buoyancy_instabilities/problem.cpp:221-293): from the next step on, in
+ * both stages after the boundary conditions, the stage state (primOld in
+ * the half stage, primHalfStep in the full stage) takes on the X1 sides
+ * rho, u and u~^1 from prim_ic, u~^2 = u~^3 = 0, B^1..3 = 0 and (conduction
+ * on) q~ = 0.  The sides are the zones of X1 index 0..numGhost (the first
+ * bulk zone included: af::seq(0, numGhost) is inclusive, :233) and
+ * N1 + numGhost .. N1 + 2 numGhost - 1, every X2 and X3 index.  prim_ic: 3
+ * planes (rho, u, u~^1) of grim_grid_zones doubles, host memory or device
+ * memory when on_device != 0, copied; NULL switches the hook off.  The
+ * state buffer and its on/off switch follow the rules of
+ * grim_set_ghost_state.  Synchronises the context's stream. */
+int grim_set_x1_side_state(grim_ctx* ctx, const double* prim_ic, int on_device);
 
 /* Replaces the dumpVars fill of fullStepDiagnostics (torus/problem.cpp:
  * 808-875, layout params.cpp:136-147).  out: device buffer of (dof + 5)

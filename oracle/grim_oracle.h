@@ -23,6 +23,7 @@
 #define GRIM_ORACLE_CLOSURE_PER_ZONE 1
 #define GRIM_ORACLE_CLOSURE_TORUS 2
 #define GRIM_ORACLE_CLOSURE_SHOCK_TESTS 3
+#define GRIM_ORACLE_CLOSURE_BUOYANCY 4
 
 #define GRIM_ORACLE_NEWTON_PER_ZONE 0
 #define GRIM_ORACLE_NEWTON_GLOBAL 1
@@ -344,6 +345,14 @@ void grim_oracle_time_step(const grim_oracle_grid* c, const grim_oracle_params* 
  * planes, kept by the caller) after the half stage's boundary conditions;
  * NULL switches the hook off.  Process-wide. */
 void grim_oracle_set_ghost_state(const double* prim_ic);
+
+/* setProblemSpecificBCs of buoyancy_instabilities (src/problem/
+ * buoyancy_instabilities/problem.cpp:221-293): in both stages after the
+ * boundary conditions, the stage state's zones of X1 index 0..numGhost and
+ * N1+numGhost..N1+2 numGhost-1 take rho, u, u~^1 from prim_ic (3 planes,
+ * kept by the caller), u~^2 = u~^3 = B^i = 0 and q~ = 0; NULL switches the
+ * hook off.  Process-wide. */
+void grim_oracle_set_x1_side_state(const double* prim_ic);
 
 /* The two stages as separate calls, for stage-level parity tests.
  * stage 0 (half): boundaries on primOld, consOld, sources, divF, induction,

@@ -110,6 +110,13 @@ struct FluidElement {
       tau = T(0.1);
       chi_emhd = p.ConductionAlpha * soundSpeed * soundSpeed * tau;
       nu_emhd = p.ViscosityAlpha * soundSpeed * soundSpeed * tau;
+    } else if (p.closure == GRIM_ORACLE_CLOSURE_BUOYANCY) {
+      // src/problem/buoyancy_instabilities/problem.cpp:11-21: af::pow(r,
+      // 0.5) evaluated as sqrt(r) (correctly rounded; a library pow need not
+      // be); the problem sets no nu_emhd and runs without viscosity
+      chi_emhd = sqrt(geom->r);
+      tau = 1.2 * chi_emhd / (soundSpeed * soundSpeed);
+      nu_emhd = T(0.);
     } else {
       // torus closure, src/problem/torus/problem.cpp:7-55.  pow(r, 1.5) and
       // pow(c_s, 3) are evaluated as r sqrt(r) and c_s c_s c_s (equal up to

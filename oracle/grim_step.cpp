@@ -1135,6 +1135,28 @@ void ghostStateBCs(const Dims& D, int nv, const double* ic, double* prim) {
       }
 }
 
+// setProblemSpecificBCs of buoyancy_instabilities (problem.cpp:221-293) on
+// the stage state, both stages (null: hook off).  af::seq(0, numGhost) is
+// inclusive: the first bulk zone of each X1 line is reset too.
+const double* g_sideState = nullptr;
+
+void sideStateBCs(const Dims& D, const Vars& vars, bool conduction, const double* ic, double* prim) {
+  const int64_t n = D.n;
+  for (int k = 0; k < D.NT[2]; k++)
+    for (int j = 0; j < D.NT[1]; j++)
+      for (int i = 0; i < D.NT[0]; i++) {
+        if (i > D.g[0] && i < D.N[0] + D.g[0]) continue;
+        const int64_t z = D.idx(i, j, k);
+        prim[RHO * n + z] = ic[z];
+        prim[UU * n + z] = ic[n + z];
+        prim[U1 * n + z] = ic[2 * n + z];
+        prim[U2 * n + z] = 0.;
+        prim[U3 * n + z] = 0.;
+        for (int d = 0; d < 3; d++) prim[(vars.B1 + d) * n + z] = 0.;
+        if (conduction) prim[vars.Q * n + z] = 0.;
+      }
+}
+
 void stage(const grim_oracle_grid& c, const grim_oracle_params& p, const Dims& D,
            const grim_oracle_geometry& gm, const grim_oracle_torus* torus, int st, double dt,
            double* primOld, double* primHalf, double* consOld, double* primNew, int semantics,
@@ -1151,6 +1173,7 @@ void stage(const grim_oracle_grid& c, const grim_oracle_params& p, const Dims& D
   applyBCs(c, D, nv, primStage);
   if (torus) torusBCs(c, p, *torus, D, gm, primStage);
   if (g_ghostState && st == 0) ghostStateBCs(D, nv, g_ghostState, primStage);
+  if (g_sideState) sideStateBCs(D, vars, p.conduction != 0, g_sideState, primStage);
   const double dtPrime = (st == 0) ? dt / 2. : dt;
 
   std::vector<double> Sexp((size_t)nv * n, 0.), divF((size_t)nv * n, 0.);
@@ -1488,6 +1511,7 @@ double grim_oracle_compute_dt(const grim_oracle_grid* c, const grim_oracle_param
 }
 
 void grim_oracle_set_ghost_state(const double* prim_ic) { g_ghostState = prim_ic; }
+void grim_oracle_set_x1_side_state(const double* prim_ic) { g_sideState = prim_ic; }
 
 void grim_oracle_stage(const grim_oracle_grid* c, const grim_oracle_params* p,
                        const grim_oracle_geometry* gm, const grim_oracle_torus* t, int st,

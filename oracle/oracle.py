@@ -22,6 +22,7 @@ CLOSURE_LINEAR_MODES = 0
 CLOSURE_PER_ZONE = 1
 CLOSURE_TORUS = 2
 CLOSURE_SHOCK_TESTS = 3
+CLOSURE_BUOYANCY = 4
 NEWTON_PER_ZONE = 0
 NEWTON_GLOBAL = 1
 MAX_VARS = 10
@@ -198,6 +199,7 @@ def lib():
         L.grim_oracle_torus_observers.argtypes = [Gp, P, Gm, vp, vp]
         L.grim_oracle_dump_vars.argtypes = [Gp, P, Gm, vp, vp]
         L.grim_oracle_set_ghost_state.argtypes = [vp]
+        L.grim_oracle_set_x1_side_state.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -491,10 +493,12 @@ class Simulation:
     (and the torus problem's hooks when torus is given)."""
 
     def __init__(self, g: Grid, p: Params, prim, dt: float, gm: Geometry | None = None, time: float = 0.,
-                 torus: "Torus | None" = None, ghost_state=None):
+                 torus: "Torus | None" = None, ghost_state=None, side_state=None):
         self.g, self.p = g, p
         # bondi_inflow's hook: ghost zones of primOld from this state
         self.ghost_state = None if ghost_state is None else _f64(ghost_state).copy()
+        # buoyancy_instabilities' hook: rho, u, u~^1 of the X1 sides (3 planes)
+        self.side_state = None if side_state is None else _f64(side_state).copy()
         self.gm = gm if gm is not None else Geometry(g)
         self.prim = _f64(prim).copy()
         self.prim_dt = self.prim.copy()
@@ -505,6 +509,7 @@ class Simulation:
     def step(self, semantics=NEWTON_PER_ZONE) -> StepStats:
         st = StepStats()
         lib().grim_oracle_set_ghost_state(None if self.ghost_state is None else _ptr(self.ghost_state))
+        lib().grim_oracle_set_x1_side_state(None if self.side_state is None else _ptr(self.side_state))
         try:
             lib().grim_oracle_time_step(C.byref(self.g), C.byref(self.p), C.byref(self.gm._c()),
                                         C.byref(self.torus) if self.torus else None, _ptr(self.prim),
@@ -512,6 +517,7 @@ class Simulation:
                                         C.byref(st))
         finally:
             lib().grim_oracle_set_ghost_state(None)
+            lib().grim_oracle_set_x1_side_state(None)
         return st
 
 

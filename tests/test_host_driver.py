@@ -90,6 +90,45 @@ def test_driver_matches_python_interface(tmp_path, case):
 
 
 @pytest.mark.gpu
+def test_driver_buoyancy_reads_grim_atmosphere_files(tmp_path):
+    """--problem buoyancy (buoyancy_instabilities/problem.cpp:22-216): the
+    driver reads the four atmosphere_soln_*.txt files written by
+    problems.write_atmosphere_files, sets the state through the C++
+    interface and its X1-side hook, and after 4 steps holds the Python
+    path's state bitwise (same profiles, noise seed and amplitude); without
+    the files it stops with grim's message."""
+    import torch  # noqa: F401
+
+    from grim_amd import problems as PR
+    from grim_amd.timestepper import TimeStepper
+
+    r = subprocess.run([DRIVER, "--problem", "buoyancy", "--steps", "1", "--atmosphere", str(tmp_path)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 1 and "Input data files not found!" in r.stderr
+    cfg, p = PR.buoyancy_grid(64, 64), PR.buoyancy_params()
+    prof = PR.hydrostatic_atmosphere(cfg, p)
+    PR.write_atmosphere_files(str(tmp_path), *prof)
+    assert np.array_equal(np.array(PR.read_atmosphere_files(cfg, str(tmp_path))), np.array(prof))
+    dump = str(tmp_path / "prim.bin")
+    r = subprocess.run([DRIVER, "--problem", "buoyancy", "--steps", "4", "--atmosphere", str(tmp_path),
+                        "--dump", dump], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(dump, dtype=np.float64)
+    ts = TimeStepper(p, cfg)
+    prim, side = PR.buoyancy_prim(cfg, p, ts.geometry(0), *PR.read_atmosphere_files(cfg, str(tmp_path)),
+                                  amplitude=4e-2, seed=20261203)
+    ts.primOld = prim
+    ts.set_x1_side_state(side)
+    ts.set_time(0., 0.01)
+    for _ in range(4):
+        ts.timeStep()
+    ref = ts.primOld.ravel()
+    ts.close()
+    assert got.shape == ref.shape and np.array_equal(got, ref)
+
+
+@pytest.mark.gpu
 def test_driver_restart_continues_the_run(tmp_path):
     """CheckWallClockTermination + the constructor's restart branch: a run
     stopped after its first step (restartVarsT*.h5, restartFileName.txt,

@@ -38,6 +38,12 @@ except the boundary identities and the EMHD linear-mode error table):
 * Bondi inflow (src/problem/bondi_inflow/): the set-up's Bondi invariants
   (normalisation in Kerr-Schild, mass flux, Bernoulli constant, monopole
   field), second-order stationarity, and the ghost-zone hook.
+* buoyancy instabilities (src/problem/buoyancy_instabilities/): the
+  closure's closed form, the set-up's velocity as the static observer (u^r
+  = 0 through the fluid element; the Schwarzschild closed form of u~^1),
+  the hydrostatic profiles against an independent ODE integration, the
+  X1-side hook's identities (the first bulk zone included) and the
+  atmosphere's second-order stationarity.
 * dump variables (torus/problem.cpp:808-875): Minkowski closed forms (v, B,
   gamma, b^2, q and dP with and without the high-order factors, floors,
   points); on MKS the analytic dx/dX applied to v^i and B^i and the
@@ -1361,3 +1367,174 @@ def test_noise_stream_is_splitmix64():
     np.testing.assert_array_equal(O.uniform(seed, n), want)
     u = O.uniform(seed, 1 << 16)
     assert 0. <= u.min() and u.max() < 1. and abs(u.mean() - 0.5) < 5e-3
+
+
+# ---------------------------------------------------------------------------
+# Buoyancy instabilities (src/problem/buoyancy_instabilities/)
+# ---------------------------------------------------------------------------
+def buoyancy_case(N1, N2, **kw):
+    """grim's grid at N1 x N2.  X2cyl = 3 / N2 needs N2 of about 64: the
+    cylindrified theta takes asin of 4 Rin sin(theta(X2cyl)) / r, above 1
+    (NaN geometry) on this r range once X2cyl nears 0.1."""
+    from grim_amd import problems as PR
+
+    cfg = PR.buoyancy_grid(N1, N2)
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    pc = PR.buoyancy_params()
+    g = O.grid(**{f: getattr(cfg, f) for f, _ in cfg._fields_})
+    p = O.default_params(True)
+    for name, _ in O.Params._fields_:
+        setattr(p, name, getattr(pc, name))
+    return PR, cfg, pc, g, p
+
+
+def test_buoyancy_closure_closed_form():
+    """problem.cpp:11-21: chi = sqrt(r) whatever the state, tau = 1.2 chi /
+    c_s^2 (high-order rescaling off in the helper, so q is the primitive)."""
+    p = closure_params(True, False)
+    p.closure = O.CLOSURE_BUOYANCY
+    r = np.geomspace(2.5, 3000., 40)
+    rho = np.linspace(0.1, 3., 40)
+    u = np.geomspace(1e-4, 1., 40)
+    el = closure_zones(p, rho, u, q=0.3 * u, r=r)
+    np.testing.assert_allclose(el["chi"], np.sqrt(r), rtol=2e-16)
+    np.testing.assert_allclose(el["tau"], 1.2 * np.sqrt(r) / sound_speed_sq(p, rho, u), rtol=1e-14)
+
+
+def test_buoyancy_initial_velocity_is_the_static_observer():
+    """With no perturbation the set-up's u~^1 is the static Boyer-Lindquist
+    observer: through the fluid element u^1 = u~^1 - gamma alpha g^01
+    vanishes to rounding on every zone (derefined and cylindrified metric
+    included), and on a plain MKS grid (hSlope 1, a = 0, X1 = ln r) it is
+    the Schwarzschild closed form u~^1 = (2 / r^2) / ((1 + 2/r) sqrt(1 - 2/r))."""
+    for plain in (False, True):
+        kw = dict(DerefineThetaHorizon=0, DoCylindrify=0) if plain else {}
+        PR, cfg, pc, g, p = buoyancy_case(32, 64, **kw)
+        gm = O.Geometry(g)
+        prof = PR.hydrostatic_atmosphere(cfg, pc)
+        prim, side = PR.buoyancy_prim(cfg, pc, gm.center, *prof, amplitude=0.)
+        el = O.element(p, prim, gm.center[:16], radius=gm.center[22])
+        assert np.abs(el["uCon1"]).max() <= 1e-14 * np.abs(prim[2]).max()
+        assert np.abs(prim[2]).min() > 0.
+        if plain:
+            r = gm.center[22]
+            np.testing.assert_allclose(prim[2], (2. / r ** 2) / ((1. + 2. / r) * np.sqrt(1. - 2. / r)), rtol=1e-13)
+        # rho, u, q~ from the profiles along X1, B^3 = 1e-5 / sqrt(-g), no other field
+        P = planes(g, prim)
+        np.testing.assert_array_equal(P[0], np.broadcast_to(prof[0], g.shape))
+        np.testing.assert_array_equal(P[1], np.broadcast_to(prof[1], g.shape))
+        np.testing.assert_array_equal(P[5], 0.)
+        np.testing.assert_array_equal(prim[p.dof - 1], 1e-5 / gm.center[15])
+        np.testing.assert_array_equal(prim[p.dof - 3:p.dof - 1], 0.)
+        np.testing.assert_array_equal(side, prim[:3])
+
+
+def test_buoyancy_perturbation_and_radius_check():
+    """The bulk zones' u~^1 carries 1 + A (U - 1/2) with U the noise stream
+    (uniform, mean 1/2 to 1e-2 over 4,096 zones); ghosts stay unperturbed;
+    profiles whose radii miss the grid's by more than 1e-10 are refused."""
+    PR, cfg, pc, g, p = buoyancy_case(64, 64)
+    gm = O.Geometry(g)
+    prof = PR.hydrostatic_atmosphere(cfg, pc)
+    p0, _ = PR.buoyancy_prim(cfg, pc, gm.center, *prof, amplitude=0.)
+    p1, side = PR.buoyancy_prim(cfg, pc, gm.center, *prof, amplitude=4e-2, seed=11)
+    ratio = planes(g, p1)[2] / planes(g, p0)[2]
+    U = planes(g, PR.noise_stream(11, g.n)[None, :])[0]
+    b = g.bulk()
+    np.testing.assert_allclose(ratio[b], 1. + 4e-2 * (U[b] - 0.5), rtol=1e-15)
+    gh = np.ones(g.shape, bool)
+    gh[b] = False
+    np.testing.assert_array_equal(ratio[gh], 1.)
+    assert abs(U[b].mean() - 0.5) < 1e-2 and U.min() >= 0. and U.max() < 1.
+    np.testing.assert_array_equal(side[2], p1[2])
+    bad = list(prof)
+    bad[3] = prof[3].copy()
+    bad[3][5] += 2e-10
+    with pytest.raises(ValueError, match="Mismatch in rCoords"):
+        PR.buoyancy_prim(cfg, pc, gm.center, *bad)
+
+
+def test_buoyancy_atmosphere_is_hydrostatic():
+    """The closed-form profiles against a numerical integration of
+    dp/dr = -(rho + u + p) / (r^2 (1 - 2/r)) with rho = p / T, u = p /
+    (Gamma - 1), T = T0 r0 / r (scipy's DOP853 from the first radius)."""
+    from scipy.integrate import solve_ivp
+
+    PR, cfg, pc, g, p = buoyancy_case(64, 8)
+    rho, u, q, r = PR.hydrostatic_atmosphere(cfg, pc, rho0=1., T0=5e-3)
+    gam = pc.adiabaticIndex
+    T0, r0 = 5e-3, r[0]
+
+    def rhs(R, y):
+        P = y[0]
+        T = T0 * r0 / R
+        return [-(P / T + P / (gam - 1.) + P) / (R * R * (1. - 2. / R))]
+
+    sol = solve_ivp(rhs, (r[0], r[-1]), [T0], t_eval=r, method="DOP853", rtol=1e-12, atol=0.)
+    pres = (gam - 1.) * u
+    np.testing.assert_allclose(pres, sol.y[0], rtol=1e-9)
+    np.testing.assert_allclose(rho, pres / (T0 * r0 / r), rtol=1e-15)
+    np.testing.assert_array_equal(q, 0.)
+    np.testing.assert_allclose(r, PR.zone_radii(cfg))
+
+
+def test_buoyancy_side_hook_identities():
+    """A half stage with the hook: the zones of X1 index 0..numGhost (the
+    inclusive af::seq of problem.cpp:233 takes the first bulk zone too) and
+    the right ghosts hold rho, u, u~^1 of the state, u~^2 = u~^3 = B = q~ = 0;
+    every other zone is what the same stage leaves without the hook."""
+    PR, cfg, pc, g, p = buoyancy_case(16, 64)
+    gm = O.Geometry(g)
+    prof = PR.hydrostatic_atmosphere(cfg, pc)
+    prim, side = PR.buoyancy_prim(cfg, pc, gm.center, *prof)
+    rng = np.random.default_rng(5)
+    junk = prim * (1. + 1e-3 * rng.standard_normal(prim.shape))
+    junk[5] = 1e-4
+    plain = junk.copy()
+    O.stage(g, p, gm, 0, 1e-2, plain)
+    hooked = junk.copy()
+    O.lib().grim_oracle_set_x1_side_state(O._ptr(side))
+    try:
+        O.stage(g, p, gm, 0, 1e-2, hooked)
+    finally:
+        O.lib().grim_oracle_set_x1_side_state(None)
+    ng = g.numGhost
+    sides = np.zeros(g.shape, bool)
+    sides[..., : ng + 1] = True
+    sides[..., g.N1 + ng:] = True
+    H, Pn, S = planes(g, hooked), planes(g, plain), planes(g, side)
+    for v in range(3):
+        np.testing.assert_array_equal(H[v][sides], S[v][sides])
+    for v in range(3, p.dof):
+        np.testing.assert_array_equal(H[v][sides], 0.)
+    np.testing.assert_array_equal(H[:, ~sides], Pn[:, ~sides])
+    assert not np.array_equal(Pn[0][..., ng], S[0][..., ng])  # the first bulk zone is reset only with the hook
+
+
+def test_buoyancy_atmosphere_is_held_static_by_the_side_hook():
+    """The unperturbed atmosphere (A = 0) on the plain MKS grid (no
+    derefinement or cylindrification, so the state has no theta
+    dependence), stepped to t = 40 M: with the X1-side hook the largest
+    radial 4-velocity u^1 through the fluid element stays below 1e-4 of the
+    static u~^1 (measured 1.7e-5 and 2.6e-5, in the polar rows next to the
+    inner boundary), and the median falls by more than 2.5 when both
+    resolutions double (measured 2.8); with grim's outflow boundaries alone
+    the atmosphere falls in at the inner edge (7e-2)."""
+    out = {}
+    for N, hook in ((32, True), (64, True), (32, False)):
+        PR, cfg, pc, g, p = buoyancy_case(N, N, DerefineThetaHorizon=0, DoCylindrify=0)
+        gm = O.Geometry(g)
+        prof = PR.hydrostatic_atmosphere(cfg, pc)
+        prim, side = PR.buoyancy_prim(cfg, pc, gm.center, *prof, amplitude=0.)
+        sim = O.Simulation(g, p, prim, dt=1., gm=gm, side_state=side if hook else None)
+        while sim.time.value < 40.:
+            st = sim.step()
+            assert list(st.nonfinite) == [0, 0]
+        el = O.element(p, sim.prim, gm.center[:16], radius=gm.center[22])
+        u1 = np.abs(planes(g, el["uCon1"][None, :])[0][g.bulk()]) / np.abs(prim[2]).max()
+        out[N, hook] = (u1.max(), np.median(u1))
+    print("buoyancy atmosphere max, median |u^1| / u~^1 after t = 40:", out)
+    assert out[32, True][0] < 1e-4 and out[64, True][0] < 1e-4
+    assert out[32, True][1] / out[64, True][1] > 2.5
+    assert out[32, False][0] > 1e-2
