@@ -52,6 +52,23 @@ struct ModeTraits {
   static constexpr bool analytic = (MODE == MODE_ANALYTIC64 || MODE == MODE_MIXED || MODE == MODE_ANALYTIC64F);
 };
 
+// The EMHD mixed solve keeps each zone's metric (16 doubles) and EMHD
+// weights (12) in LDS slots instead of VGPRs: held in registers across the
+// Newton loop they pushed it past 256 VGPRs into scratch (224 bytes per
+// lane).  Slots are 17 and 13 doubles apart, so that the 64 lanes of a b64
+// read cover every bank.  Every residual evaluation and Jacobian assembly
+// starts behind a compiler fence, so the slots are read where they are used
+// instead of being hoisted back into registers for the whole loop.
+template <class L, int MODE>
+constexpr bool kLdsState = L::EMHD && MODE == MODE_MIXED;
+constexpr int kGeomSlot = 17, kWeightSlot = 13;
+constexpr size_t kLdsStateBytes = (size_t)(kGeomSlot + kWeightSlot) * sizeof(double);  // per lane
+
+template <class L, int MODE>
+__device__ __forceinline__ void slotFence() {
+  if constexpr (kLdsState<L, MODE>) asm volatile("" ::: "memory");
+}
+
 // Per-zone quantities of the residual formed once per zone.
 struct ZoneScale {
   double invAlpha;  // 1/alpha
@@ -64,6 +81,7 @@ __device__ __forceinline__ void evalResidual(const double* P, const ZoneGeom& G,
                                              const ZoneConst<L>& c, const DevParams& prm,
                                              double dt, const ZoneScale& zs, const EmhdWeights& ew,
                                              double* R) {
+  slotFence<L, MODE>();
   if constexpr (ModeTraits<MODE>::fastResidual && !L::EMHD)
     residualFastIdeal<L>(P, G, c, prm, zs.invAlpha, zs.scale, R);
   else if constexpr (ModeTraits<MODE>::fastResidual)
@@ -180,6 +198,7 @@ __device__ __forceinline__ bool newtonUpdate(double* P, const double* R, double 
     luSolve<double, L::NF>(A, d);
   } else {
     float A[L::NF][L::NF], df[L::NF];
+    slotFence<L, MODE>();
     analyticJacobian<float, L>(P, G, prm, c.mask / dt, c, ew, dt, A);
 #pragma unroll
     for (int v = 0; v < L::NF; v++) df[v] = (float)(-R[v]);
@@ -242,6 +261,12 @@ __device__ __forceinline__ bool newtonUpdate(double* P, const double* R, double 
 // iteration is a function of P alone, so stopping and resuming changes no
 // result).  WIDE: the zone's kWide lanes run it together; only the first
 // writes.
+template <class L, int MODE, bool WIDE>
+__device__ __forceinline__ bool solveZoneCore(const DevParams& prm, const grim_zone_batch& b,
+                                              const grim_newton_stats& st, int64_t z, int budget,
+                                              bool accumulate, bool lastPhase, double dt, const ZoneGeom& G,
+                                              const ZoneConst<L>& c, const ZoneScale& zs, const EmhdWeights& ew);
+
 template <class L, int MODE, bool WIDE = false>
 __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_batch& b,
                                           const grim_newton_stats& st, int64_t z, int budget,
@@ -273,6 +298,29 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
   EmhdWeights ew;
   if constexpr (L::EMHD && (ModeTraits<MODE>::analytic || ModeTraits<MODE>::fastResidual))
     emhdWeights<L>(G, c, prm, dt, ew);
+  if constexpr (kLdsState<L, MODE>) {
+    static_assert(sizeof(ZoneGeom) <= kGeomSlot * sizeof(double) &&
+                  sizeof(EmhdWeights) <= kWeightSlot * sizeof(double), "LDS slot sizes");
+    // the launch gives kLdsStateBytes of dynamic LDS per lane
+    extern __shared__ double zoneSlots[];
+    ZoneGeom* gs = reinterpret_cast<ZoneGeom*>(zoneSlots + (size_t)kGeomSlot * threadIdx.x);
+    EmhdWeights* ws =
+        reinterpret_cast<EmhdWeights*>(zoneSlots + (size_t)kGeomSlot * blockDim.x + (size_t)kWeightSlot * threadIdx.x);
+    *gs = G;
+    *ws = ew;
+    return solveZoneCore<L, MODE, WIDE>(prm, b, st, z, budget, accumulate, lastPhase, dt, *gs, c, zs, *ws);
+  } else {
+    return solveZoneCore<L, MODE, WIDE>(prm, b, st, z, budget, accumulate, lastPhase, dt, G, c, zs, ew);
+  }
+}
+
+template <class L, int MODE, bool WIDE>
+__device__ __forceinline__ bool solveZoneCore(const DevParams& prm, const grim_zone_batch& b,
+                                              const grim_newton_stats& st, int64_t z, int budget,
+                                              bool accumulate, bool lastPhase, double dt, const ZoneGeom& G,
+                                              const ZoneConst<L>& c, const ZoneScale& zs, const EmhdWeights& ew) {
+  const int64_t n = b.n;
+  const bool lead = !WIDE || (threadIdx.x % kWide) == 0;
   double P[L::DOF];
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) P[v] = b.prim[v * n + z];
@@ -534,6 +582,12 @@ struct LaunchCfg {
   CompactionWorkspace ws;    // masks, prefix sums, work list, counters
 };
 
+// Dynamic LDS of a per-zone solve workgroup of `block` lanes.
+template <class L, int MODE>
+constexpr size_t ldsBytes(int block) {
+  return kLdsState<L, MODE> ? kLdsStateBytes * (size_t)block : 0;
+}
+
 // Enqueue the per-zone solve of mode MODE on stream s (phase 2 of the
 // compacting solve in MODE2).
 template <class L, int MODE, int MODE2 = MODE>
@@ -548,12 +602,12 @@ void launchSolve(const DevParams& prm, const grim_zone_batch& b, const grim_newt
         return;
       }
     }
-    newtonPerZone<L, MODE><<<grid, cfg.block, 0, s>>>(prm, b, st);
+    newtonPerZone<L, MODE><<<grid, cfg.block, ldsBytes<L, MODE>(cfg.block), s>>>(prm, b, st);
     return;
   }
   static_assert(MODE2 == MODE || (MODE == MODE_MIXED && MODE2 == MODE_ANALYTIC64F), "phase-2 mode");
   const CompactionWorkspace& w = cfg.ws;
-  newtonPhase1<L, MODE><<<grid, cfg.block, 0, s>>>(prm, b, st, cfg.budget1, w.masks);
+  newtonPhase1<L, MODE><<<grid, cfg.block, ldsBytes<L, MODE>(cfg.block), s>>>(prm, b, st, cfg.budget1, w.masks);
   const int64_t nMasks = (b.n + 63) / 64;
   const int64_t nScan = (nMasks + kScanBlock - 1) / kScanBlock;
   scanMasksKernel<<<(unsigned)nScan, kScanBlock, 0, s>>>(nMasks, w.masks, w.maskOff, w.blockSum);
@@ -561,8 +615,8 @@ void launchSolve(const DevParams& prm, const grim_zone_batch& b, const grim_newt
   scatterListKernel<<<(unsigned)nScan, kScanBlock, 0, s>>>(nMasks, w.masks, w.maskOff, w.blockSum, w.list);
   const int64_t waves = std::min<int64_t>(cfg.persistentWaves, nMasks);
   const unsigned grid2 = (unsigned)((waves * 64 + cfg.block - 1) / cfg.block);
-  newtonPhase2<L, MODE2><<<grid2, cfg.block, 0, s>>>(prm, b, st, prm.maxIter - cfg.budget1, w.list, w.ctrl,
-                                                     cfg.group);
+  newtonPhase2<L, MODE2><<<grid2, cfg.block, ldsBytes<L, MODE2>(cfg.block), s>>>(
+      prm, b, st, prm.maxIter - cfg.budget1, w.list, w.ctrl, cfg.group);
 }
 
 // Host launcher of the fast translation unit (newton_fast.hip).
