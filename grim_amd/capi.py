@@ -90,6 +90,7 @@ class ZoneBatch(C.Structure):
         ("residual_mask", C.c_void_p),
         ("dt_prime", C.c_double),
         ("dt_prime_device", C.c_void_p),
+        ("guess", C.c_void_p),
     ]
 
 
@@ -358,11 +359,14 @@ def device_info() -> str:
 
 
 def make_batch(prim, geom, rhs, prim_old=None, prim_coef=None, closure=None, radius=None,
-               residual_mask=None, dt_prime=1.0) -> ZoneBatch:
+               residual_mask=None, dt_prime=1.0, guess=None) -> ZoneBatch:
+    """guess: optional starting fluid rows (numFluidVars x n) that the solve
+    reads instead of prim's; the result still goes to prim."""
     n = prim.shape[-1]
     return ZoneBatch(n, _ptr(prim, numel=8 * n), _ptr(geom, numel=GEOM_PLANES * n), _ptr(rhs, numel=5 * n),
                      _ptr(prim_old, numel=8 * n), _ptr(prim_coef, numel=8 * n), _ptr(closure, numel=2 * n),
-                     _ptr(radius, numel=n), _ptr(residual_mask, numel=n), float(dt_prime))
+                     _ptr(radius, numel=n), _ptr(residual_mask, numel=n), float(dt_prime), None,
+                     _ptr(guess, numel=5 * n))
 
 
 def conserved(p: Params, prim, geom, closure=None, radius=None, out=None, stream=None):

@@ -474,7 +474,14 @@ int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
   if (b->n == 0) return GRIM_OK;
   if (int rc = checkDevice()) return rc;
   hipStream_t s = (hipStream_t)stream;
-  if (o.semantics == GRIM_NEWTON_GLOBAL) return launchGlobal(*p, *b, st, s);
+  if (o.semantics == GRIM_NEWTON_GLOBAL) {
+    // grim's lock-step loop works in prim from the start: copy the guess in
+    if (b->guess && b->guess != b->prim)
+      GRIM_HIP_TRY(hipMemcpyAsync(b->prim, b->guess, sizeof(double) * (size_t)(5 + p->conduction + p->viscosity) *
+                                                         (size_t)b->n,
+                                  hipMemcpyDeviceToDevice, s));
+    return launchGlobal(*p, *b, st, s);
+  }
 
   LaunchCfg cfg;
   cfg.block = o.block_size ? o.block_size : kDefaultBlock;

@@ -265,7 +265,8 @@ template <class L, int MODE, bool WIDE>
 __device__ __forceinline__ bool solveZoneCore(const DevParams& prm, const grim_zone_batch& b,
                                               const grim_newton_stats& st, int64_t z, int budget,
                                               bool accumulate, bool lastPhase, double dt, const ZoneGeom& G,
-                                              const ZoneConst<L>& c, const ZoneScale& zs, const EmhdWeights& ew);
+                                              const ZoneConst<L>& c, const ZoneScale& zs, const EmhdWeights& ew,
+                                              const double* start);
 
 template <class L, int MODE, bool WIDE = false>
 __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_batch& b,
@@ -273,12 +274,18 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
                                           bool accumulate, bool lastPhase) {
   const int64_t n = b.n;
   const bool lead = !WIDE || (threadIdx.x % kWide) == 0;
+  // the starting fluid rows: the caller's guess planes on a first pass,
+  // else prim (a later phase resumes from what the earlier one wrote)
+  const double* start = (b.guess && !accumulate) ? b.guess : b.prim;
   if (b.residual_mask && b.residual_mask[z] == 0.) {
     // outside the residual mask (ghost zones of a grid) R = 0: the zone
     // takes its floors and stops at the first test, as in the full path
     if (!lead) return false;
-    b.prim[RHO * n + z] = smax(b.prim[RHO * n + z], prm.rhoFloor);
-    b.prim[UU * n + z] = smax(b.prim[UU * n + z], prm.uFloor);
+    if (start != b.prim)
+#pragma unroll
+      for (int v = U1; v < L::NF; v++) b.prim[v * n + z] = start[v * n + z];
+    b.prim[RHO * n + z] = smax(start[RHO * n + z], prm.rhoFloor);
+    b.prim[UU * n + z] = smax(start[UU * n + z], prm.uFloor);
     if (st.iterations) st.iterations[z] = accumulate ? st.iterations[z] : 0;
     if (st.residual_evals) st.residual_evals[z] = (accumulate ? st.residual_evals[z] : 0) + 1;
     if (st.final_l2) st.final_l2[z] = 0.;
@@ -308,9 +315,9 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
         reinterpret_cast<EmhdWeights*>(zoneSlots + (size_t)kGeomSlot * blockDim.x + (size_t)kWeightSlot * threadIdx.x);
     *gs = G;
     *ws = ew;
-    return solveZoneCore<L, MODE, WIDE>(prm, b, st, z, budget, accumulate, lastPhase, dt, *gs, c, zs, *ws);
+    return solveZoneCore<L, MODE, WIDE>(prm, b, st, z, budget, accumulate, lastPhase, dt, *gs, c, zs, *ws, start);
   } else {
-    return solveZoneCore<L, MODE, WIDE>(prm, b, st, z, budget, accumulate, lastPhase, dt, G, c, zs, ew);
+    return solveZoneCore<L, MODE, WIDE>(prm, b, st, z, budget, accumulate, lastPhase, dt, G, c, zs, ew, start);
   }
 }
 
@@ -318,12 +325,13 @@ template <class L, int MODE, bool WIDE>
 __device__ __forceinline__ bool solveZoneCore(const DevParams& prm, const grim_zone_batch& b,
                                               const grim_newton_stats& st, int64_t z, int budget,
                                               bool accumulate, bool lastPhase, double dt, const ZoneGeom& G,
-                                              const ZoneConst<L>& c, const ZoneScale& zs, const EmhdWeights& ew) {
+                                              const ZoneConst<L>& c, const ZoneScale& zs, const EmhdWeights& ew,
+                                              const double* start) {
   const int64_t n = b.n;
   const bool lead = !WIDE || (threadIdx.x % kWide) == 0;
   double P[L::DOF];
 #pragma unroll
-  for (int v = 0; v < L::DOF; v++) P[v] = b.prim[v * n + z];
+  for (int v = 0; v < L::DOF; v++) P[v] = (v < L::NF ? start : b.prim)[v * n + z];
 
   double R[L::NF];
   bool haveR = false;

@@ -1865,8 +1865,13 @@ inline const double* stageDtPrime(grim_ctx* x, int st) {
 // rhs = consOld - dt' (divF + S_expl + S_impl,old / 2) with the residual
 // mask.  The Newton guess (the stage state's fluid rows) goes into the
 // stage's output buffer.
+//
+// guessInSolve: the stage's own solve reads the fluid rows straight from the
+// stage state (grim_zone_batch.guess), so only the B rows are copied here;
+// the stage-level entry points, whose caller may solve another buffer or
+// inspect the guess, get the whole state copied.
 template <class L>
-int stagePrepare(grim_ctx* x, int st) {
+int stagePrepare(grim_ctx* x, int st, bool guessInSolve = false) {
   const Dims& D = x->D;
   const size_t bytes = sizeof(double) * (size_t)L::DOF * D.n;
   double* primStage = stageIn(x, st);
@@ -1890,7 +1895,13 @@ int stagePrepare(grim_ctx* x, int st) {
                                   st == 0 ? x->consOld : nullptr))
     return rc;
   if (int rc = fluxesAndEmf<L>(x, primStage, false)) return rc;
-  GRIM_HIP_TRY(hipMemcpyAsync(out, primStage, bytes, hipMemcpyDeviceToDevice, x->s));
+  if (guessInSolve) {
+    const size_t fluid = (size_t)L::NF * D.n;
+    GRIM_HIP_TRY(hipMemcpyAsync(out + fluid, primStage + fluid, bytes - sizeof(double) * fluid,
+                                hipMemcpyDeviceToDevice, x->s));
+  } else {
+    GRIM_HIP_TRY(hipMemcpyAsync(out, primStage, bytes, hipMemcpyDeviceToDevice, x->s));
+  }
   const Box b = bulkBox(D);
   launchFold<L, true>(x->a, b, x->s, x->fluxes, x->emf, x->geomC, x->consOld, x->sExp, x->sImp,
                       stageDtPrime(x, st), nullptr, x->rhs, out, x->c.residualScaling ? x->mask : nullptr);
@@ -1917,8 +1928,9 @@ grim_zone_batch stageBatch(grim_ctx* x, int st, double* prim) {
 
 // timeStepper::solve of the stage (solve.cpp:3-201) on prim; the per-zone
 // iteration counts and final residuals stay in the context.
-int stageSolve(grim_ctx* x, int st, double* prim) {
-  const grim_zone_batch zb = stageBatch(x, st, prim);
+int stageSolve(grim_ctx* x, int st, double* prim, const double* guess = nullptr) {
+  grim_zone_batch zb = stageBatch(x, st, prim);
+  zb.guess = guess;
   grim_newton_stats ns = {x->iters, nullptr, x->l2};
   return grim_batched_newton(&x->p, &zb, &x->opts, &ns, x->s);
 }
@@ -1957,8 +1969,8 @@ int stageFinish(grim_ctx* x, int st) {
 
 template <class L>
 int stage(grim_ctx* x, int st, grim_step_stats* stats) {
-  if (int rc = stagePrepare<L>(x, st)) return rc;
-  if (int rc = stageSolve(x, st, stageOut(x, st))) return rc;
+  if (int rc = stagePrepare<L>(x, st, true)) return rc;
+  if (int rc = stageSolve(x, st, stageOut(x, st), stageIn(x, st))) return rc;
   if (stats)
     if (int rc = stageStats(x, st, stats)) return rc;
   return stageFinish<L>(x, st);

@@ -94,6 +94,16 @@ typedef struct grim_zone_batch {
                                     (steps whose dt is computed on the device,
                                     e.g. under HIP graph capture); NULL = use
                                     dt_prime */
+  const double* guess;        /* optional: numFluidVars planes of the starting
+                                 fluid primitives, read instead of prim's own
+                                 fluid rows; the solve's result still goes to
+                                 prim (B rows are always read from prim).  It
+                                 folds into the solve the copy grim makes
+                                 before it (prim = primOld, timestep.cpp:
+                                 72-75), so a caller solving again from the
+                                 same start needs no copy.  Read by
+                                 grim_batched_newton(_noalloc) and the
+                                 autotuner only; NULL = start from prim. */
 } grim_zone_batch;
 
 typedef enum grim_jacobian_kind { GRIM_JAC_FD = 0, GRIM_JAC_ANALYTIC = 1 } grim_jacobian_kind;

@@ -325,6 +325,49 @@ def test_compaction_is_bit_identical_to_one_pass(capi, torch_cuda, case):
     np.testing.assert_array_equal(out[0][2], out[1][2])
 
 
+@pytest.mark.parametrize("case", [("fd", False, 0), ("mixed", True, 0), ("mixed", True, 1), ("escalate", True, 1),
+                                  ("fd_global", False, 0), ("mixed_noalloc", True, 1)])
+def test_guess_input_equals_copy_then_solve(capi, torch_cuda, case):
+    """batch.guess: a solve that starts from the guess planes leaves prim
+    bitwise as copying the guess into prim's fluid rows and solving in place
+    (every row, iteration counts and final sum R^2 equal), reads prim's B
+    rows, leaves the guess untouched; zones outside the mask take the
+    guess's floored values; phase 2 of a compacted solve resumes from prim."""
+    torch = torch_cuda
+    path, emhd, compaction = case
+    n = 65536
+    p, b, rhs, mask = make_problem(capi, torch, n, emhd)
+    mask[::7] = 0.
+    jac = capi.JAC_FD if path.startswith("fd") else capi.JAC_ANALYTIC
+    lu = {"fd": capi.LU_FP64, "fd_global": capi.LU_FP64, "mixed": capi.LU_FP32, "mixed_noalloc": capi.LU_FP32,
+          "escalate": capi.LU_FP32_FP64}[path]
+    sem = capi.NEWTON_GLOBAL if path == "fd_global" else capi.NEWTON_PER_ZONE
+    ws = torch.empty(capi.workspace_bytes(n), dtype=torch.uint8, device="cuda")
+    out = []
+    guess = b["guess"][: p.nf]
+    keep = guess.clone()
+    for use_guess in (False, True):
+        prim = b["guess"].clone()
+        if use_guess:
+            prim[: p.nf] = 3. * prim[: p.nf] + 1.  # overwritten by the solve from the guess
+        it = torch.zeros(n, dtype=torch.int32, device="cuda")
+        l2 = torch.zeros(n, dtype=torch.float64, device="cuda")
+        batch = capi.make_batch(prim, b["geom"], rhs, b["prim_old"], None, b["closure"], residual_mask=mask,
+                                guess=guess if use_guess else None)
+        if path == "mixed_noalloc":
+            o = capi.newton_opts(jac, lu, compaction=compaction, phase1_iters=1, phase2_group=2)
+            capi.batched_newton_noalloc(p, batch, o, iterations=it, final_l2=l2)
+        else:
+            o = capi.newton_opts(jac, lu, semantics=sem, compaction=compaction, phase1_iters=1, phase2_group=2,
+                                 workspace=ws if compaction else None)
+            capi.batched_newton(p, batch, iterations=it, final_l2=l2, opts=o)
+        torch.cuda.synchronize()
+        out.append((host(prim), host(it), host(l2)))
+    assert torch.equal(guess, keep)
+    for a, c in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, c)
+
+
 def test_autotune_returns_a_working_configuration(capi, torch_cuda):
     torch = torch_cuda
     n = 1 << 20
