@@ -6,7 +6,11 @@ into one shared library exporting the C ABI of ``include/grim_hip.h``.
 Flags that matter:
   -ffp-contract=off   no fused multiply-add contraction, so FP64 results are
                       rounded operation by operation as written (every file
-                      except *_fast.hip, which use -ffp-contract=fast);
+                      except *_fast.hip, which use
+                      -ffp-contract=fast-honor-pragmas: a*b+c may fuse,
+                      except inside a `#pragma clang fp contract(off)` block,
+                      which the shared fold arithmetic uses to keep its bits
+                      in both kinds of translation unit);
   -fno-hip-fp32-correctly-rounded-divide-sqrt
                       *_fast.hip only: FP32 division and square root by the
                       hardware reciprocal sequences (the FP32 Jacobian and LU
@@ -61,7 +65,7 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
         flags = list(HIPCC_FLAGS)
         if src.endswith("_fast.hip"):
             # fused translation units: a*b+c may become one FMA
-            flags[flags.index("-ffp-contract=off")] = "-ffp-contract=fast"
+            flags[flags.index("-ffp-contract=off")] = "-ffp-contract=fast-honor-pragmas"
             flags.append("-fno-hip-fp32-correctly-rounded-divide-sqrt")
         cmd = [hipcc(), *flags, *inc, "-c", src, "-o", obj]
         if verbose:

@@ -336,6 +336,104 @@ int noAllocScratchPtr(void** out) {
 
 using namespace grim;
 
+namespace grim {
+
+bool foldFusable(const grim_params& p, const grim_newton_opts& o, int dim) {
+  const bool escalate = o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP32_FP64;
+  const bool mixed = o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP32;
+  const int budget1 = o.phase1_iters ? o.phase1_iters : 2;
+  return dim == 3 && o.semantics == GRIM_NEWTON_PER_ZONE && (escalate || (mixed && o.compaction)) &&
+         budget1 < p.maxNonLinearIter;
+}
+
+int batchedNewton(const grim_params* p, const grim_zone_batch* b, const grim_newton_opts* opts,
+                  grim_newton_stats* stats, hipStream_t stream, const FoldArgs* fold) {
+  GRIM_REQUIRE(validBatch(p, b), "grim_batched_newton: bad params or batch");
+  grim_newton_opts o = {};
+  o.jacobian = GRIM_JAC_FD;
+  o.lu_precision = GRIM_LU_FP64;
+  o.semantics = GRIM_NEWTON_PER_ZONE;
+  if (opts) o = *opts;
+  grim_newton_stats st = {nullptr, nullptr, nullptr};
+  if (stats) st = *stats;
+  GRIM_REQUIRE(o.semantics == GRIM_NEWTON_PER_ZONE || o.semantics == GRIM_NEWTON_GLOBAL,
+               "grim_batched_newton: bad semantics");
+  int mode = -1;
+  if (o.jacobian == GRIM_JAC_FD && o.lu_precision == GRIM_LU_FP64) mode = o.fused ? MODE_FD_FAST : MODE_FD;
+  if (o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP64) mode = MODE_ANALYTIC64;
+  if (o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP32) mode = MODE_MIXED;
+  const bool escalate = o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP32_FP64;
+  if (escalate) mode = MODE_MIXED;
+  GRIM_REQUIRE(mode >= 0, "grim_batched_newton: unsupported Jacobian / LU precision pair");
+  GRIM_REQUIRE(mode == MODE_FD || o.semantics == GRIM_NEWTON_PER_ZONE,
+               "grim_batched_newton: lock-step semantics run grim's exact FD-FP64 algorithm only");
+  GRIM_REQUIRE(o.semantics == GRIM_NEWTON_PER_ZONE || !o.compaction,
+               "grim_batched_newton: compaction applies to per-zone semantics");
+  GRIM_REQUIRE(o.block_size == 0 || o.block_size == 64 || o.block_size == 128 || o.block_size == 256,
+               "grim_batched_newton: block_size must be 64, 128 or 256");
+  GRIM_REQUIRE(o.phase1_iters >= 0 && o.phase2_group >= 0 && o.phase2_group <= kMaxGroup,
+               "grim_batched_newton: bad compaction options");
+  if (b->n == 0) return GRIM_OK;
+  if (int rc = checkDevice()) return rc;
+  hipStream_t s = stream;
+  GRIM_REQUIRE(!fold || foldFusable(*p, o, fold->D.dim), "stage solve: the fold runs inside compacting mixed solves only");
+  if (o.semantics == GRIM_NEWTON_GLOBAL) {
+    // grim's lock-step loop works in prim from the start: copy the guess in
+    if (b->guess && b->guess != b->prim)
+      GRIM_HIP_TRY(hipMemcpyAsync(b->prim, b->guess, sizeof(double) * (size_t)(5 + p->conduction + p->viscosity) *
+                                                         (size_t)b->n,
+                                  hipMemcpyDeviceToDevice, s));
+    return launchGlobal(*p, *b, st, s);
+  }
+
+  LaunchCfg cfg;
+  cfg.block = o.block_size ? o.block_size : kDefaultBlock;
+  cfg.compaction = o.compaction != 0 || escalate;
+  cfg.escalate = escalate;
+  cfg.budget1 = o.phase1_iters ? o.phase1_iters : 2;
+  cfg.group = o.phase2_group ? o.phase2_group : 1;
+  // persistent phase 2: 8 wavefronts per compute unit
+  cfg.persistentWaves = 8 * deviceCUs();
+  cfg.ws = CompactionWorkspace{};
+  void* owned = nullptr;
+  const int64_t need = grim_newton_workspace_bytes(b->n);
+  if (cfg.compaction) {
+    GRIM_REQUIRE(b->n <= (int64_t(1) << 30), "grim_batched_newton: compaction supports at most 2^30 zones");
+    void* ws = o.workspace;
+    if (ws) {
+      GRIM_REQUIRE(o.workspace_bytes >= need, "grim_batched_newton: workspace too small");
+    } else {
+      GRIM_HIP_TRY(hipMallocAsync(&owned, (size_t)need, s));
+      ws = owned;
+    }
+    cfg.ws = carveCompaction(ws, b->n);
+  }
+  const DevParams prm = makeDevParams(*p);
+  const grim_zone_batch bb = *b;
+  int rc;
+  if (mode == MODE_MIXED || mode == MODE_FD_FAST) {
+    rc = launchNewtonFast(mode, *p, bb, st, cfg, s, fold);
+  } else {
+    rc = dispatchLayout(*p, [&]<class L>() -> int {
+      if (mode == MODE_FD)
+        launchSolve<L, MODE_FD>(prm, bb, st, cfg, s);
+      else
+        launchSolve<L, MODE_ANALYTIC64>(prm, bb, st, cfg, s);
+      return checkLaunch("grim_batched_newton");
+    });
+  }
+  if (owned) {
+    hipError_t e = hipFreeAsync(owned, s);
+    if (rc == GRIM_OK && e != hipSuccess) {
+      setError(std::string("hipFreeAsync: ") + hipGetErrorString(e));
+      rc = GRIM_ERR_HIP;
+    }
+  }
+  return rc;
+}
+
+}  // namespace grim
+
 extern "C" {
 
 int grim_residual(const grim_params* p, const grim_zone_batch* b, double* R,
@@ -446,87 +544,7 @@ int64_t grim_newton_workspace_bytes(int64_t n) {
 int grim_batched_newton(const grim_params* p, const grim_zone_batch* b,
                         const grim_newton_opts* opts, grim_newton_stats* stats,
                         grim_stream_t stream) {
-  GRIM_REQUIRE(validBatch(p, b), "grim_batched_newton: bad params or batch");
-  grim_newton_opts o = {};
-  o.jacobian = GRIM_JAC_FD;
-  o.lu_precision = GRIM_LU_FP64;
-  o.semantics = GRIM_NEWTON_PER_ZONE;
-  if (opts) o = *opts;
-  grim_newton_stats st = {nullptr, nullptr, nullptr};
-  if (stats) st = *stats;
-  GRIM_REQUIRE(o.semantics == GRIM_NEWTON_PER_ZONE || o.semantics == GRIM_NEWTON_GLOBAL,
-               "grim_batched_newton: bad semantics");
-  int mode = -1;
-  if (o.jacobian == GRIM_JAC_FD && o.lu_precision == GRIM_LU_FP64) mode = o.fused ? MODE_FD_FAST : MODE_FD;
-  if (o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP64) mode = MODE_ANALYTIC64;
-  if (o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP32) mode = MODE_MIXED;
-  const bool escalate = o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP32_FP64;
-  if (escalate) mode = MODE_MIXED;
-  GRIM_REQUIRE(mode >= 0, "grim_batched_newton: unsupported Jacobian / LU precision pair");
-  GRIM_REQUIRE(mode == MODE_FD || o.semantics == GRIM_NEWTON_PER_ZONE,
-               "grim_batched_newton: lock-step semantics run grim's exact FD-FP64 algorithm only");
-  GRIM_REQUIRE(o.semantics == GRIM_NEWTON_PER_ZONE || !o.compaction,
-               "grim_batched_newton: compaction applies to per-zone semantics");
-  GRIM_REQUIRE(o.block_size == 0 || o.block_size == 64 || o.block_size == 128 || o.block_size == 256,
-               "grim_batched_newton: block_size must be 64, 128 or 256");
-  GRIM_REQUIRE(o.phase1_iters >= 0 && o.phase2_group >= 0 && o.phase2_group <= kMaxGroup,
-               "grim_batched_newton: bad compaction options");
-  if (b->n == 0) return GRIM_OK;
-  if (int rc = checkDevice()) return rc;
-  hipStream_t s = (hipStream_t)stream;
-  if (o.semantics == GRIM_NEWTON_GLOBAL) {
-    // grim's lock-step loop works in prim from the start: copy the guess in
-    if (b->guess && b->guess != b->prim)
-      GRIM_HIP_TRY(hipMemcpyAsync(b->prim, b->guess, sizeof(double) * (size_t)(5 + p->conduction + p->viscosity) *
-                                                         (size_t)b->n,
-                                  hipMemcpyDeviceToDevice, s));
-    return launchGlobal(*p, *b, st, s);
-  }
-
-  LaunchCfg cfg;
-  cfg.block = o.block_size ? o.block_size : kDefaultBlock;
-  cfg.compaction = o.compaction != 0 || escalate;
-  cfg.escalate = escalate;
-  cfg.budget1 = o.phase1_iters ? o.phase1_iters : 2;
-  cfg.group = o.phase2_group ? o.phase2_group : 1;
-  // persistent phase 2: 8 wavefronts per compute unit
-  cfg.persistentWaves = 8 * deviceCUs();
-  cfg.ws = CompactionWorkspace{};
-  void* owned = nullptr;
-  const int64_t need = grim_newton_workspace_bytes(b->n);
-  if (cfg.compaction) {
-    GRIM_REQUIRE(b->n <= (int64_t(1) << 30), "grim_batched_newton: compaction supports at most 2^30 zones");
-    void* ws = o.workspace;
-    if (ws) {
-      GRIM_REQUIRE(o.workspace_bytes >= need, "grim_batched_newton: workspace too small");
-    } else {
-      GRIM_HIP_TRY(hipMallocAsync(&owned, (size_t)need, s));
-      ws = owned;
-    }
-    cfg.ws = carveCompaction(ws, b->n);
-  }
-  const DevParams prm = makeDevParams(*p);
-  const grim_zone_batch bb = *b;
-  int rc;
-  if (mode == MODE_MIXED || mode == MODE_FD_FAST) {
-    rc = launchNewtonFast(mode, *p, bb, st, cfg, s);
-  } else {
-    rc = dispatchLayout(*p, [&]<class L>() -> int {
-      if (mode == MODE_FD)
-        launchSolve<L, MODE_FD>(prm, bb, st, cfg, s);
-      else
-        launchSolve<L, MODE_ANALYTIC64>(prm, bb, st, cfg, s);
-      return checkLaunch("grim_batched_newton");
-    });
-  }
-  if (owned) {
-    hipError_t e = hipFreeAsync(owned, s);
-    if (rc == GRIM_OK && e != hipSuccess) {
-      setError(std::string("hipFreeAsync: ") + hipGetErrorString(e));
-      rc = GRIM_ERR_HIP;
-    }
-  }
-  return rc;
+  return grim::batchedNewton(p, b, opts, stats, (hipStream_t)stream, nullptr);
 }
 
 int grim_batched_newton_noalloc(const grim_params* p, const grim_zone_batch* b,

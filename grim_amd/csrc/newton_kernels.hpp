@@ -3,7 +3,7 @@
 // newton.hip includes this header and compiles with -ffp-contract=off (the
 // reference FD path and the analytic FP64 path round every operation as
 // written, bit-identical to the oracle); newton_fast.hip compiles the same
-// templates with -ffp-contract=fast (fused multiply-adds) for the mixed
+// templates with -ffp-contract=fast-honor-pragmas (fused multiply-adds) for the mixed
 // precision path and the fast FD variant.  Each translation unit
 // instantiates different modes, so no kernel is compiled twice.
 //
@@ -20,6 +20,7 @@
 #include <algorithm>
 
 #include "analytic.hpp"
+#include "fold.hpp"
 #include "grim_hip.h"
 #include "zone_physics.hpp"
 
@@ -266,18 +267,23 @@ __device__ __forceinline__ bool solveZoneCore(const DevParams& prm, const grim_z
                                               const grim_newton_stats& st, int64_t z, int budget,
                                               bool accumulate, bool lastPhase, double dt, const ZoneGeom& G,
                                               const ZoneConst<L>& c, const ZoneScale& zs, const EmhdWeights& ew,
-                                              const double* start);
+                                              const double* start, const double* foldB);
 
-template <class L, int MODE, bool WIDE = false>
+// FOLDED: the zone's right-hand side, field rows and residual mask come from
+// the caller's registers (foldRhs, foldB, foldMask: the stage fold computed
+// in the same kernel) instead of the batch's planes.
+template <class L, int MODE, bool WIDE = false, bool FOLDED = false>
 __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_batch& b,
                                           const grim_newton_stats& st, int64_t z, int budget,
-                                          bool accumulate, bool lastPhase) {
+                                          bool accumulate, bool lastPhase, const double* foldRhs = nullptr,
+                                          const double* foldB = nullptr, double foldMask = 0.) {
   const int64_t n = b.n;
   const bool lead = !WIDE || (threadIdx.x % kWide) == 0;
   // the starting fluid rows: the caller's guess planes on a first pass,
   // else prim (a later phase resumes from what the earlier one wrote)
   const double* start = (b.guess && !accumulate) ? b.guess : b.prim;
-  if (b.residual_mask && b.residual_mask[z] == 0.) {
+  const double maskZ = FOLDED ? foldMask : (b.residual_mask ? b.residual_mask[z] : 1.);
+  if ((FOLDED || b.residual_mask) && maskZ == 0.) {
     // outside the residual mask (ghost zones of a grid) R = 0: the zone
     // takes its floors and stops at the first test, as in the full path
     if (!lead) return false;
@@ -295,7 +301,12 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
   ZoneGeom G;
   loadGeom(b.geom, n, z, G);
   ZoneConst<L> c;
-  loadZoneConst<L>(b, prm, G, z, c);
+  loadZoneConst<L, !FOLDED>(b, prm, G, z, c);
+  if constexpr (FOLDED) {
+#pragma unroll
+    for (int v = 0; v < L::NF; v++) c.rhs[v] = foldRhs[v];
+    c.mask = maskZ;
+  }
   ZoneScale zs;
   if constexpr (ModeTraits<MODE>::fastResidual) {
     zs.invAlpha = 1. / G.alpha;
@@ -315,9 +326,11 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
         reinterpret_cast<EmhdWeights*>(zoneSlots + (size_t)kGeomSlot * blockDim.x + (size_t)kWeightSlot * threadIdx.x);
     *gs = G;
     *ws = ew;
-    return solveZoneCore<L, MODE, WIDE>(prm, b, st, z, budget, accumulate, lastPhase, dt, *gs, c, zs, *ws, start);
+    return solveZoneCore<L, MODE, WIDE>(prm, b, st, z, budget, accumulate, lastPhase, dt, *gs, c, zs, *ws, start,
+                                        FOLDED ? foldB : nullptr);
   } else {
-    return solveZoneCore<L, MODE, WIDE>(prm, b, st, z, budget, accumulate, lastPhase, dt, G, c, zs, ew, start);
+    return solveZoneCore<L, MODE, WIDE>(prm, b, st, z, budget, accumulate, lastPhase, dt, G, c, zs, ew, start,
+                                        FOLDED ? foldB : nullptr);
   }
 }
 
@@ -326,12 +339,13 @@ __device__ __forceinline__ bool solveZoneCore(const DevParams& prm, const grim_z
                                               const grim_newton_stats& st, int64_t z, int budget,
                                               bool accumulate, bool lastPhase, double dt, const ZoneGeom& G,
                                               const ZoneConst<L>& c, const ZoneScale& zs, const EmhdWeights& ew,
-                                              const double* start) {
+                                              const double* start, const double* foldB) {
   const int64_t n = b.n;
   const bool lead = !WIDE || (threadIdx.x % kWide) == 0;
   double P[L::DOF];
 #pragma unroll
-  for (int v = 0; v < L::DOF; v++) P[v] = (v < L::NF ? start : b.prim)[v * n + z];
+  for (int v = 0; v < L::DOF; v++)
+    P[v] = v < L::NF ? start[v * n + z] : (foldB ? foldB[v - L::NF] : b.prim[v * n + z]);
 
   double R[L::NF];
   bool haveR = false;
@@ -421,6 +435,51 @@ __global__ __launch_bounds__(kDefaultBlock) __attribute__((amdgpu_waves_per_eu(2
   const unsigned long long ballot = __ballot(more);
   // lane 0 of a wavefront holding at least one zone (z < n) owns mask z/64;
   // wavefronts wholly past the end of the batch own none
+  if ((threadIdx.x & 63) == 0 && z < b.n) masks[z >> 6] = ballot;
+}
+
+// Phase 1 of a grid stage's compacting solve with the stage fold in front of
+// each zone's Newton iterations: a bulk zone forms its right-hand side, its
+// new field rows and its residual scale from the face fluxes, EMFs, consOld
+// and sources (foldZone, the fold kernel's arithmetic bit for bit), writes
+// the field rows and the mask plane, and keeps the right-hand side in
+// registers; only the zones left unconverged store it for phase 2.  The
+// fold's memory traffic overlaps the solves' arithmetic.
+template <class L, int MODE, int DIM>
+__global__ __launch_bounds__(kDefaultBlock) __attribute__((amdgpu_waves_per_eu(2))) void newtonPhase1Fold(
+    DevParams prm, grim_zone_batch b, grim_newton_stats st, int budget1, unsigned long long* __restrict__ masks,
+    FoldArgs f) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool more = false;
+  if (z < b.n) {
+    const int64_t n = b.n;
+    int i, j, k;
+    zoneIndex(f.D, z, i, j, k);
+    const bool bulk = i >= f.bulk.lo[0] && i < f.bulk.hi[0] && j >= f.bulk.lo[1] && j < f.bulk.hi[1] &&
+                      k >= f.bulk.lo[2] && k < f.bulk.hi[2];
+    double rhs[L::NF], B[3], mask = 0.;
+#pragma unroll
+    for (int v = 0; v < L::NF; v++) rhs[v] = 0.;
+    B[0] = B[1] = B[2] = 0.;
+    if (bulk) {
+      double rmax;
+      foldZone<L, DIM>(f.D, f.dX, f.F, f.emf, f.geomC, f.consOld, f.sExp, f.sImp, batchDt(b), z, i, j,
+                       f.fluxFilter, rhs, B, rmax);
+#pragma unroll
+      for (int c = 0; c < 3; c++) b.prim[(L::NF + c) * n + z] = B[c];
+      if (f.scaling) {
+        mask = 1. / rmax;
+        f.mask[z] = mask;
+      } else {
+        mask = f.mask[z];
+      }
+    }
+    more = solveZone<L, MODE, false, true>(prm, b, st, z, budget1, false, false, rhs, B, mask);
+    if (more)
+#pragma unroll
+      for (int v = 0; v < L::NF; v++) f.rhs[v * n + z] = rhs[v];
+  }
+  const unsigned long long ballot = __ballot(more);
   if ((threadIdx.x & 63) == 0 && z < b.n) masks[z >> 6] = ballot;
 }
 
@@ -598,9 +657,11 @@ constexpr size_t ldsBytes(int block) {
 
 // Enqueue the per-zone solve of mode MODE on stream s (phase 2 of the
 // compacting solve in MODE2).
+// fold: a grid stage's fold runs inside phase 1 (newtonPhase1Fold; the
+// caller has checked that the solve compacts, see foldFusable).
 template <class L, int MODE, int MODE2 = MODE>
 void launchSolve(const DevParams& prm, const grim_zone_batch& b, const grim_newton_stats& st,
-                 const LaunchCfg& cfg, hipStream_t s) {
+                 const LaunchCfg& cfg, hipStream_t s, const FoldArgs* fold = nullptr) {
   const unsigned grid = (unsigned)((b.n + cfg.block - 1) / cfg.block);
   if (!cfg.compaction || cfg.budget1 >= prm.maxIter) {
     if constexpr (MODE == MODE_FD) {
@@ -615,7 +676,16 @@ void launchSolve(const DevParams& prm, const grim_zone_batch& b, const grim_newt
   }
   static_assert(MODE2 == MODE || (MODE == MODE_MIXED && MODE2 == MODE_ANALYTIC64F), "phase-2 mode");
   const CompactionWorkspace& w = cfg.ws;
-  newtonPhase1<L, MODE><<<grid, cfg.block, ldsBytes<L, MODE>(cfg.block), s>>>(prm, b, st, cfg.budget1, w.masks);
+  if constexpr (MODE == MODE_MIXED) {
+    if (fold)
+      newtonPhase1Fold<L, MODE, 3><<<grid, cfg.block, ldsBytes<L, MODE>(cfg.block), s>>>(prm, b, st, cfg.budget1,
+                                                                                        w.masks, *fold);
+    else
+      newtonPhase1<L, MODE><<<grid, cfg.block, ldsBytes<L, MODE>(cfg.block), s>>>(prm, b, st, cfg.budget1,
+                                                                                 w.masks);
+  } else {
+    newtonPhase1<L, MODE><<<grid, cfg.block, ldsBytes<L, MODE>(cfg.block), s>>>(prm, b, st, cfg.budget1, w.masks);
+  }
   const int64_t nMasks = (b.n + 63) / 64;
   const int64_t nScan = (nMasks + kScanBlock - 1) / kScanBlock;
   scanMasksKernel<<<(unsigned)nScan, kScanBlock, 0, s>>>(nMasks, w.masks, w.maskOff, w.blockSum);
@@ -629,6 +699,7 @@ void launchSolve(const DevParams& prm, const grim_zone_batch& b, const grim_newt
 
 // Host launcher of the fast translation unit (newton_fast.hip).
 int launchNewtonFast(int mode, const grim_params& p, const grim_zone_batch& b,
-                     const grim_newton_stats& st, const LaunchCfg& cfg, hipStream_t s);
+                     const grim_newton_stats& st, const LaunchCfg& cfg, hipStream_t s,
+                     const FoldArgs* fold = nullptr);
 
 }  // namespace grim

@@ -27,6 +27,7 @@
 #include "runtime.hpp"
 #include "batchgen.hpp"
 #include "step_fast.hpp"
+#include "fold.hpp"
 #include "step_physics.hpp"
 #include "torus.hpp"
 #include "zone_physics.hpp"
@@ -36,15 +37,6 @@ struct grim_ctx;
 namespace grim {
 namespace {
 
-
-// (i, j, k) of zone z, in 32-bit arithmetic (grids hold fewer than 2^31 zones)
-__device__ __forceinline__ void zoneIndex(const Dims& D, int64_t z, int& i, int& j, int& k) {
-  const unsigned u = (unsigned)z, n0 = (unsigned)D.NT[0], n1 = (unsigned)D.NT[1];
-  i = (int)(u % n0);
-  const unsigned r = u / n0;
-  j = (int)(r % n1);
-  k = (int)(r / n1);
-}
 
 inline unsigned blocksFor(int64_t count) { return (unsigned)((count + kBlock - 1) / kBlock); }
 
@@ -368,74 +360,6 @@ __global__ __launch_bounds__(kBlock) void emfKernel(Dims D, Box box, const doubl
   }
 }
 
-// Flux of variable v through the lower face of zone z along d, after
-// timeStepper::fluxCT (constrainedtransport.cpp:3-47) for dim >= 2.
-template <class L, int DIM = 0>
-__device__ __forceinline__ double ctFlux(const Dims& D, const double* __restrict__ F,
-                                         const double* __restrict__ emf, int d, int v, int64_t z) {
-  const int dim = DIM ? DIM : D.dim;  // DIM 0: the grid's dimension read at run time
-  const int64_t n = D.n;
-  if (dim >= 2 && v >= L::B1) {
-    const double* e1 = emf;
-    const double* e2 = emf + n;
-    const double* e3 = emf + 2 * n;
-    const int64_t s1 = D.s[0], s2 = D.s[1], s3 = D.s[2];
-    const int b = v - L::B1;
-    if (d == 0) {
-      if (b == 0) return 0.;
-      if (b == 1) return 0.5 * (e3[z] + e3[z + s2]);
-      if (dim == 3) return -0.5 * (e2[z] + e2[z + s3]);
-    } else if (d == 1) {
-      if (b == 0) return -0.5 * (e3[z] + e3[z + s1]);
-      if (b == 1) return 0.;
-      if (dim == 3) return 0.5 * (e1[z] + e1[z + s3]);
-    } else {
-      if (b == 0) return 0.5 * (e2[z] + e2[z + s1]);
-      if (b == 1) return -0.5 * (e1[z] + e1[z + s2]);
-      return 0.;
-    }
-  }
-  return F[((size_t)d * L::DOF + v) * n + z];
-}
-
-// Divergence of the fluxes of zone z, variable v (fvmfluxes.cpp:36-46,
-// 91-103, 164-180): dim < 3 is af::convolve with {1, -1, 0}/dX,
-// dim 3 is (shift(F, -1) - F)/dX.  With the dimension and the variable known
-// at compile time every branch folds away and the flux filter becomes
-// selects, so the loads of all faces issue together.
-template <class L, int DIM>
-__device__ __forceinline__ double divergence(const Dims& D, const double* dX,
-                                             const double* __restrict__ F,
-                                             const double* __restrict__ emf, int v, int64_t z,
-                                             int i, int j, int filter) {
-  double acc = 0.;
-#pragma unroll
-  for (int d = 0; d < DIM; d++) {
-    double f0 = ctFlux<L, DIM>(D, F, emf, d, v, z);
-    double f1 = ctFlux<L, DIM>(D, F, emf, d, v, z + D.s[d]);
-    // The torus's flux filter (problem.cpp:1243-1285), after flux-CT: no
-    // mass inflow through the radial boundary faces, nothing through the
-    // polar faces.
-    if (d == 0 && v == RHO) {
-      f0 = (filter && i == D.g[0]) ? stdmin(f0, 0.) : f0;
-      f1 = (filter && i + 1 == D.N[0] + D.g[0]) ? stdmax(f1, 0.) : f1;
-    }
-    if (d == 1) {
-      f0 = (filter && j == D.g[1]) ? 0. : f0;
-      f1 = (filter && j + 1 == D.N[1] + D.g[1]) ? 0. : f1;
-    }
-    double term;
-    if constexpr (DIM < 3) {
-      const double w = 1. / dX[d];
-      term = f1 * w + f0 * (-w);
-    } else {
-      term = (f1 - f0) / dX[d];
-    }
-    acc = (d == 0) ? term : acc + term;
-  }
-  return acc;
-}
-
 // Bulk zones of a DIM-dimensional grid.  FOLD: the stage fold
 //   rhs = consOld - dt'((divF + S_expl) + 1/2 S_implOld)      (fluid rows)
 //   B   = (consOld_B - dt' divF_B) / g                        (timestep.cpp:80-89)
@@ -459,27 +383,18 @@ __global__ __launch_bounds__(kBlock) void foldKernel(StepArgs a, Box box, const 
   box.at(t, i, j, k);
   const Dims& D = a.D;
   const int64_t n = D.n, z = flatIndex(D, i, j, k);
-  const double dtPrime = FOLD ? *dtPrimeDev : 0.;
-  double rmax = 0.;
+  if constexpr (!FOLD) {
 #pragma unroll
-  for (int v = 0; v < L::DOF; v++) {
-    const double div = divergence<L, DIM>(D, a.dX, F, emf, v, z, i, j, a.fluxFilter);
-    if constexpr (!FOLD) {
-      divOut[v * n + z] = div;
-    } else {
-      const double U = consOld[v * n + z];
-      if (v < L::NF) {
-        const double si = L::EMHD ? sImp[v * n + z] : 0.;
-        const double r = U - dtPrime * ((div + sExp[v * n + z]) + 0.5 * si);
-        rhs[v * n + z] = r;
-        rmax = stdmax(rmax, fabs(r));
-      } else {
-        const double consB = U - dtPrime * div;
-        primNew[v * n + z] = consB / geomC[GEOM_G * n + z];
-      }
-    }
+    for (int v = 0; v < L::DOF; v++) divOut[v * n + z] = divergence<L, DIM>(D, a.dX, F, emf, v, z, i, j, a.fluxFilter);
+  } else {
+    double r[L::NF], B[3], rmax;
+    foldZone<L, DIM>(D, a.dX, F, emf, geomC, consOld, sExp, sImp, *dtPrimeDev, z, i, j, a.fluxFilter, r, B, rmax);
+#pragma unroll
+    for (int v = 0; v < L::NF; v++) rhs[v * n + z] = r[v];
+#pragma unroll
+    for (int c = 0; c < 3; c++) primNew[(L::NF + c) * n + z] = B[c];
+    if (mask) mask[z] = 1. / rmax;
   }
-  if (FOLD && mask) mask[z] = 1. / rmax;
 }
 
 // foldKernel for the grid's dimension.
@@ -796,6 +711,40 @@ __global__ __launch_bounds__(kBlock) void elementKernel(StepArgs a, int what, in
 // ---------------------------------------------------------------------------
 // dt and stats
 // ---------------------------------------------------------------------------
+// computeDt's per-zone term (timestep.cpp:336-391): the sum over the
+// active directions of max(|min speed|, max speed) / dX.
+template <class L>
+__device__ __forceinline__ double zoneInvDt(const StepArgs& a, const double* P, const GeomFull& G) {
+  Kinematics kk;
+  Closure cc;
+  kinematics<L>(P, G.zg, a.prm, kk);
+  elementClosure<L>(P, kk, a.prm, G.r, cc);
+  double maxSpeed = 0.;
+  for (int d = 0; d < a.D.dim; d++) {
+    double mn, mx;
+    if (d == 0) charSpeeds<L, 0>(kk, cc, G, a.prm, mn, mx);
+    else if (d == 1) charSpeeds<L, 1>(kk, cc, G, a.prm, mn, mx);
+    else charSpeeds<L, 2>(kk, cc, G, a.prm, mn, mx);
+    mn = mn / a.dX[d];
+    mx = mx / a.dX[d];
+    const double s = stdmax(mx, fabs(mn));
+    maxSpeed = (d == 0) ? s : maxSpeed + s;
+  }
+  return maxSpeed;
+}
+
+// The workgroup's maximum of m (a fixed tree, so the result does not depend
+// on timing) into blockMax[blockIdx.x].
+__device__ __forceinline__ void blockMaxStore(double* red, double m, double* __restrict__ blockMax) {
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = stdmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blockMax[blockIdx.x] = red[0];
+}
+
 template <class L>
 __global__ __launch_bounds__(kBlock) void dtKernel(StepArgs a, const double* __restrict__ prim,
                                                    const double* __restrict__ geomC,
@@ -810,30 +759,9 @@ __global__ __launch_bounds__(kBlock) void dtKernel(StepArgs a, const double* __r
     for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
     GeomFull G;
     loadGeomFull(geomC, n, z, G);
-    Kinematics kk;
-    Closure cc;
-    kinematics<L>(P, G.zg, a.prm, kk);
-    elementClosure<L>(P, kk, a.prm, G.r, cc);
-    double maxSpeed = 0.;
-    for (int d = 0; d < a.D.dim; d++) {
-      double mn, mx;
-      if (d == 0) charSpeeds<L, 0>(kk, cc, G, a.prm, mn, mx);
-      else if (d == 1) charSpeeds<L, 1>(kk, cc, G, a.prm, mn, mx);
-      else charSpeeds<L, 2>(kk, cc, G, a.prm, mn, mx);
-      mn = mn / a.dX[d];
-      mx = mx / a.dX[d];
-      const double s = stdmax(mx, fabs(mn));
-      maxSpeed = (d == 0) ? s : maxSpeed + s;
-    }
-    m = maxSpeed;
+    m = zoneInvDt<L>(a, P, G);
   }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] = stdmax(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) blockMax[blockIdx.x] = red[0];
+  blockMaxStore(red, m, blockMax);
 }
 
 // The step's clock lives in device memory so that a whole step can be
@@ -872,24 +800,6 @@ __global__ void setClockKernel(double* __restrict__ clock, double time, double d
   clock[CLOCK_TIME] = time;
   clock[CLOCK_DT] = dt;
   clock[CLOCK_HALF_DT] = dt / 2.;
-}
-
-// dst1 = dst2 = src over `count` doubles: the end of the step writes the new
-// state into primOld and into the state computeDt reads next, reading it once.
-__global__ __launch_bounds__(kBlock) void copy2Kernel(int64_t count, const double* __restrict__ src,
-                                                      double* __restrict__ dst1, double* __restrict__ dst2,
-                                                      double* __restrict__ clock) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) clock[CLOCK_TIME] += clock[CLOCK_DT];  // the step's end: time += dt
-  const int64_t pairs = count / 2;
-  if (t < pairs) {
-    const double2 v = reinterpret_cast<const double2*>(src)[t];
-    reinterpret_cast<double2*>(dst1)[t] = v;
-    reinterpret_cast<double2*>(dst2)[t] = v;
-  } else if (t == pairs && (count & 1)) {
-    dst1[count - 1] = src[count - 1];
-    dst2[count - 1] = src[count - 1];
-  }
 }
 
 // Clears counters with a kernel (a memset node in a captured graph is not
@@ -1224,6 +1134,56 @@ __global__ __launch_bounds__(kBlock) void nanZeroKernel(int64_t n, int nvars, co
     for (int v = 0; v < nvars; v++) prim[v * n + z] = 0.;
 }
 
+// The end of grim_time_step in one pass over every zone (workgroups and
+// zones mapped as in nanZeroKernel, floorKernel and dtKernel): the torus's
+// full-stage NaN scrub (when the bulk count is positive) and applyFloor
+// (fullStepDiagnostics, torus/problem.cpp:719-746) written back into prim,
+// the result into primOld and primDt (timestep.cpp:290-300), time += dt, and
+// the next
+// computeDt's per-workgroup maxima of the inverse time step of the result
+// (timestep.cpp:336-391), which the next step's reduction reads.  Each
+// value is the separate kernels' bit for bit.
+template <class L, bool TORUS>
+__global__ __launch_bounds__(kBlock) void endStepKernel(StepArgs a, grim_torus_params t,
+                                                        const double* __restrict__ geomC,
+                                                        const double* __restrict__ floors,
+                                                        const unsigned long long* __restrict__ nanCount,
+                                                        double* __restrict__ prim, double* __restrict__ primOld,
+                                                        double* __restrict__ primDt, double* __restrict__ clock,
+                                                        double* __restrict__ blockMax) {
+  __shared__ double red[kBlock];
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = a.D.n;
+  if (z == 0) clock[CLOCK_TIME] += clock[CLOCK_DT];
+  double m = -INFINITY;
+  if (z < n) {
+    double P[L::DOF];
+#pragma unroll
+    for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
+    GeomFull G;
+    loadGeomFull(geomC, n, z, G);
+    if constexpr (TORUS) {
+      if (nanCount[0] != 0) {
+        bool bad = false;
+#pragma unroll
+        for (int v = 0; v < L::DOF; v++) bad = bad || isnan(P[v]);
+        if (bad)
+#pragma unroll
+          for (int v = 0; v < L::DOF; v++) P[v] = 0.;
+      }
+      applyFloorZone<L>(a.prm, t, G, floors[z], floors[n + z], P);
+    }
+#pragma unroll
+    for (int v = 0; v < L::DOF; v++) {
+      if constexpr (TORUS) prim[v * n + z] = P[v];
+      primOld[v * n + z] = P[v];
+      primDt[v * n + z] = P[v];
+    }
+    m = zoneInvDt<L>(a, P, G);
+  }
+  blockMaxStore(red, m, blockMax);
+}
+
 // On-the-fly observers of fullStepDiagnostics (:757-776), defined in
 // src/problem/torus/ObserversTorus.hpp, of the element set at prim:
 //  ComputeEnergyIntegrals (:6-81)   sums over the bulk of rho, b^2 and u
@@ -1450,6 +1410,10 @@ struct grim_ctx {
   grim_grid_config c;
   grim_newton_opts opts;
   hipStream_t s;
+  // the stage's explicit sources run on `side`, beside the face fluxes on s
+  // (fork and join through the two events; default priority)
+  hipStream_t side;
+  hipEvent_t evFork, evJoin;
   Dims D;
   StepArgs a;
   CoordGrid cg;
@@ -1462,6 +1426,10 @@ struct grim_ctx {
   double *geomC, *geomX1, *geomX2, *gamma;
   double* gamma40;  // explicitFast: the 40 symmetric connection planes
   double* red;
+  // computeDt's per-workgroup maxima of primDt (blocksFor(n) + 1 doubles):
+  // written by grim_set_time and by the end of every step, reduced at the
+  // start of the next
+  double* dtPart;
   double* obs;  // observers: per-workgroup partials, then the result
   int* iters;
   double* l2;
@@ -1616,26 +1584,34 @@ int fluxesAndEmf(grim_ctx* x, const double* prim, bool allFaces) {
 
 template <class L>
 int explicitSources(grim_ctx* x, const double* prim, const double* primOld, double* sExp, double* sImp,
-                    double* consOld) {
+                    double* consOld, bool onSide = false) {
   const Box b = bulkBox(x->D);
   if constexpr (L::EMHD)
     uTKernel<L><<<blocksFor(x->D.n), kBlock, 0, x->s>>>(x->a, prim, x->geomC, x->uT);
+  hipStream_t ss = x->s;
+  if (onSide) {
+    // fork: the sources wait for everything enqueued on s so far (the stage
+    // state and u_mu, T); the caller joins them back before the fold
+    GRIM_HIP_TRY(hipEventRecord(x->evFork, x->s));
+    GRIM_HIP_TRY(hipStreamWaitEvent(x->side, x->evFork, 0));
+    ss = x->side;
+  }
   if (x->c.explicitFast)
     return launchSourcesFast(x->p, x->a, x->c.reconstruction, b, prim, primOld, x->geomC, x->gamma40, x->uT, sExp,
-                             sImp, consOld, x->s);
+                             sImp, consOld, ss);
   const unsigned nb = blocksFor(b.count());
   switch (x->c.reconstruction) {
     case GRIM_RECON_WENO5:
-      sourcesKernel<L, GRIM_RECON_WENO5><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, primOld, x->geomC, x->gamma,
-                                                                  x->uT, sExp, sImp, consOld);
-      break;
-    case GRIM_RECON_PPM:
-      sourcesKernel<L, GRIM_RECON_PPM><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, primOld, x->geomC, x->gamma,
+      sourcesKernel<L, GRIM_RECON_WENO5><<<nb, kBlock, 0, ss>>>(x->a, b, prim, primOld, x->geomC, x->gamma,
                                                                 x->uT, sExp, sImp, consOld);
       break;
+    case GRIM_RECON_PPM:
+      sourcesKernel<L, GRIM_RECON_PPM><<<nb, kBlock, 0, ss>>>(x->a, b, prim, primOld, x->geomC, x->gamma,
+                                                              x->uT, sExp, sImp, consOld);
+      break;
     default:
-      sourcesKernel<L, GRIM_RECON_MINMOD><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, primOld, x->geomC,
-                                                                   x->gamma, x->uT, sExp, sImp, consOld);
+      sourcesKernel<L, GRIM_RECON_MINMOD><<<nb, kBlock, 0, ss>>>(x->a, b, prim, primOld, x->geomC, x->gamma,
+                                                                 x->uT, sExp, sImp, consOld);
   }
   return checkLaunch("sources");
 }
@@ -1870,8 +1846,10 @@ inline const double* stageDtPrime(grim_ctx* x, int st) {
 // stage state (grim_zone_batch.guess), so only the B rows are copied here;
 // the stage-level entry points, whose caller may solve another buffer or
 // inspect the guess, get the whole state copied.
+// foldInSolve: the stage's compacting solve folds each zone's right-hand
+// side itself (newtonPhase1Fold), so no fold kernel runs here.
 template <class L>
-int stagePrepare(grim_ctx* x, int st, bool guessInSolve = false) {
+int stagePrepare(grim_ctx* x, int st, bool guessInSolve = false, bool foldInSolve = false) {
   const Dims& D = x->D;
   const size_t bytes = sizeof(double) * (size_t)L::DOF * D.n;
   double* primStage = stageIn(x, st);
@@ -1891,10 +1869,14 @@ int stagePrepare(grim_ctx* x, int st, bool guessInSolve = false) {
     sideStateKernel<L><<<blocksFor(count), kBlock, 0, x->s>>>(D, x->sideState, primStage);
     if (int rc = checkLaunch("X1 side state")) return rc;
   }
+  // the sources (bandwidth-bound) on the side stream, overlapping the face
+  // fluxes (issue-bound) on s; neither reads what the other writes
   if (int rc = explicitSources<L>(x, primStage, x->primOld, x->sExp, L::EMHD ? x->sImp : nullptr,
-                                  st == 0 ? x->consOld : nullptr))
+                                  st == 0 ? x->consOld : nullptr, true))
     return rc;
   if (int rc = fluxesAndEmf<L>(x, primStage, false)) return rc;
+  GRIM_HIP_TRY(hipEventRecord(x->evJoin, x->side));
+  GRIM_HIP_TRY(hipStreamWaitEvent(x->s, x->evJoin, 0));
   if (guessInSolve) {
     const size_t fluid = (size_t)L::NF * D.n;
     GRIM_HIP_TRY(hipMemcpyAsync(out + fluid, primStage + fluid, bytes - sizeof(double) * fluid,
@@ -1902,11 +1884,17 @@ int stagePrepare(grim_ctx* x, int st, bool guessInSolve = false) {
   } else {
     GRIM_HIP_TRY(hipMemcpyAsync(out, primStage, bytes, hipMemcpyDeviceToDevice, x->s));
   }
+  if (foldInSolve) return GRIM_OK;
   const Box b = bulkBox(D);
   launchFold<L, true>(x->a, b, x->s, x->fluxes, x->emf, x->geomC, x->consOld, x->sExp, x->sImp,
                       stageDtPrime(x, st), nullptr, x->rhs, out, x->c.residualScaling ? x->mask : nullptr);
   return checkLaunch("fold");
 }
+
+// The stage solve folds in phase 1 when its options compact with the mixed
+// analytic Jacobian on a 3D grid (the grid-step path only: the stage-level
+// entry points keep the separate fold, whose rhs the caller may read).
+bool foldInSolve(const grim_ctx* x) { return foldFusable(x->p, x->opts, x->D.dim); }
 
 // The stage's implicit system as a batch of independent zones (the frozen
 // terms folded into rhs; elemOld = primOld, the coefficient element primOld
@@ -1928,11 +1916,26 @@ grim_zone_batch stageBatch(grim_ctx* x, int st, double* prim) {
 
 // timeStepper::solve of the stage (solve.cpp:3-201) on prim; the per-zone
 // iteration counts and final residuals stay in the context.
-int stageSolve(grim_ctx* x, int st, double* prim, const double* guess = nullptr) {
+int stageSolve(grim_ctx* x, int st, double* prim, const double* guess = nullptr, bool fold = false) {
   grim_zone_batch zb = stageBatch(x, st, prim);
   zb.guess = guess;
   grim_newton_stats ns = {x->iters, nullptr, x->l2};
-  return grim_batched_newton(&x->p, &zb, &x->opts, &ns, x->s);
+  if (!fold) return grim_batched_newton(&x->p, &zb, &x->opts, &ns, x->s);
+  FoldArgs f;
+  f.D = x->D;
+  f.bulk = bulkBox(x->D);
+  for (int d = 0; d < 3; d++) f.dX[d] = x->a.dX[d];
+  f.fluxFilter = x->a.fluxFilter;
+  f.scaling = x->c.residualScaling ? 1 : 0;
+  f.F = x->fluxes;
+  f.emf = x->emf;
+  f.geomC = x->geomC;
+  f.consOld = x->consOld;
+  f.sExp = x->sExp;
+  f.sImp = x->sImp;
+  f.rhs = x->rhs;
+  f.mask = x->mask;
+  return batchedNewton(&x->p, &zb, &x->opts, &ns, x->s, &f);
 }
 
 // Unconverged / non-finite bulk zones and the largest iteration count of the
@@ -1969,8 +1972,9 @@ int stageFinish(grim_ctx* x, int st) {
 
 template <class L>
 int stage(grim_ctx* x, int st, grim_step_stats* stats) {
-  if (int rc = stagePrepare<L>(x, st, true)) return rc;
-  if (int rc = stageSolve(x, st, stageOut(x, st), stageIn(x, st))) return rc;
+  const bool fold = foldInSolve(x);
+  if (int rc = stagePrepare<L>(x, st, true, fold)) return rc;
+  if (int rc = stageSolve(x, st, stageOut(x, st), stageIn(x, st), fold)) return rc;
   if (stats)
     if (int rc = stageStats(x, st, stats)) return rc;
   return stageFinish<L>(x, st);
@@ -1980,9 +1984,28 @@ int stage(grim_ctx* x, int st, grim_step_stats* stats) {
 // state the next computeDt reads, and time += dt (timestep.cpp:290-300).
 template <class L>
 int stepEnd(grim_ctx* x) {
-  const int64_t count = (int64_t)L::DOF * x->D.n;
-  copy2Kernel<<<blocksFor(count / 2 + 1), kBlock, 0, x->s>>>(count, x->primNew, x->primOld, x->primDt, x->clock);
+  endStepKernel<L, false><<<blocksFor(x->D.n), kBlock, 0, x->s>>>(x->a, x->tp, x->geomC, x->floors, x->counters,
+                                                                  x->primNew, x->primOld, x->primDt, x->clock,
+                                                                  x->dtPart);
   return checkLaunch("end of step");
+}
+
+// The full stage's finish and the end of the step in one pass (the torus's
+// NaN scrub and floors, the copies, the clock and the next dt's partial
+// maxima): stageFinish(1) followed by stepEnd.
+template <class L>
+int finishStep(grim_ctx* x) {
+  if (int rc = communicate(x, x->primNew, L::DOF)) return rc;
+  if (x->torus) {
+    const Box b = bulkBox(x->D);
+    zeroCountersKernel<<<1, 64, 0, x->s>>>(x->counters, 1);
+    nanCountKernel<<<blocksFor(b.count()), kBlock, 0, x->s>>>(x->D, b, L::DOF, x->primNew, x->counters);
+    endStepKernel<L, true><<<blocksFor(x->D.n), kBlock, 0, x->s>>>(x->a, x->tp, x->geomC, x->floors, x->counters,
+                                                                   x->primNew, x->primOld, x->primDt, x->clock,
+                                                                   x->dtPart);
+    return checkLaunch("end of step");
+  }
+  return stepEnd<L>(x);
 }
 
 int validConfig(const grim_params* p, const grim_grid_config* c) {
@@ -2087,6 +2110,7 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
       {(void**)&x->geomX2, locB},
       {(void**)&x->gamma, sizeof(double) * 64 * (size_t)n},
       {(void**)&x->red, sizeof(double) * ((size_t)nb + 1)},
+      {(void**)&x->dtPart, sizeof(double) * ((size_t)nb + 1)},
       {(void**)&x->obs, sizeof(double) * ((size_t)kObsBlocks + 1) * OBS_WORDS},
       {(void**)&x->iters, sizeof(int) * (size_t)n},
       {(void**)&x->l2, sizeof(double) * (size_t)n},
@@ -2104,6 +2128,13 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
   }
   x->opts.workspace = x->newtonWs;
   x->opts.workspace_bytes = grim_newton_workspace_bytes(n);
+  if (hipStreamCreateWithFlags(&x->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&x->evFork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&x->evJoin, hipEventDisableTiming) != hipSuccess) {
+    setError("grim_create: side stream or event creation failed");
+    grim_destroy(x);
+    return GRIM_ERR_HIP;
+  }
   if (hipHostMalloc((void**)&x->hostBuf, 128) != hipSuccess) {
     setError("grim_create: hipHostMalloc failed");
     grim_destroy(x);
@@ -2153,9 +2184,13 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
 int grim_destroy(grim_ctx* x) {
   if (!x) return GRIM_OK;
   if (x->s) (void)hipStreamSynchronize(x->s);
+  if (x->side) (void)hipStreamSynchronize(x->side);
+  if (x->evFork) (void)hipEventDestroy(x->evFork);
+  if (x->evJoin) (void)hipEventDestroy(x->evJoin);
+  if (x->side) (void)hipStreamDestroy(x->side);
   void* ptrs[] = {x->primOld, x->primHalf, x->primNew, x->primDt, x->consOld, x->fluxes, x->emf,
                   x->sExp,    x->sImp,     x->uT,      x->rhs,    x->mask,    x->geomC,  x->geomX1,
-                  x->geomX2,  x->gamma,    x->red,     x->iters,  x->l2,      x->counters,
+                  x->geomX2,  x->gamma,    x->red,     x->dtPart, x->iters,  x->l2,      x->counters,
                   x->newtonWs, x->clock,  x->obs,  x->gamma40};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -2216,7 +2251,10 @@ int grim_set_time(grim_ctx* x, double time, double dt) {
   // primDt = primOld: computeDt of the first step reads the initial state.
   GRIM_HIP_TRY(hipMemcpyAsync(x->primDt, x->primOld, sizeof(double) * (size_t)x->dof * x->D.n,
                               hipMemcpyDeviceToDevice, x->s));
-  return GRIM_OK;
+  return withLayout(x, [&]<class L>() -> int {
+    dtKernel<L><<<blocksFor(x->D.n), kBlock, 0, x->s>>>(x->a, x->primDt, x->geomC, x->dtPart);
+    return checkLaunch("grim_set_time");
+  });
 }
 
 int grim_get_time(grim_ctx* x, double* time, double* dt) {
@@ -2231,12 +2269,21 @@ int grim_get_time(grim_ctx* x, double* time, double* dt) {
 int grim_time_step(grim_ctx* x, grim_step_stats* stats) {
   GRIM_REQUIRE(x && x->clockSet, "grim_time_step: no context or dt not set (grim_set_time)");
   return withLayout(x, [&]<class L>() -> int {
-    if (int rc = computeDtOnDevice<L>(x, x->primDt)) return rc;
+    // computeDt of primDt from the partial maxima the previous step's end
+    // (or grim_set_time) left
+    const unsigned nb = blocksFor(x->D.n);
+    maxReduceKernel<<<1, kBlock, 0, x->s>>>(nb, x->dtPart, x->dtPart + nb, x->clock, x->c.CourantFactor,
+                                            x->c.maxDtIncrement);
+    if (int rc = checkLaunch("computeDt")) return rc;
     if (int rc = stage<L>(x, 0, stats)) return rc;
-    if (int rc = stage<L>(x, 1, stats)) return rc;
+    const bool fold = foldInSolve(x);
+    if (int rc = stagePrepare<L>(x, 1, true, fold)) return rc;
+    if (int rc = stageSolve(x, 1, x->primNew, x->primHalf, fold)) return rc;
+    if (stats)
+      if (int rc = stageStats(x, 1, stats)) return rc;
     // primOld = the full stage's result, and elemOld of the next computeDt
     // (the problem's fullStepDiagnostics set it from this step's primOld).
-    if (int rc = stepEnd<L>(x)) return rc;
+    if (int rc = finishStep<L>(x)) return rc;
     if (stats) {
       GRIM_HIP_TRY(hipMemcpyAsync(x->hostBuf + 4, x->clock, 2 * sizeof(double), hipMemcpyDeviceToHost, x->s));
       GRIM_HIP_TRY(hipStreamSynchronize(x->s));

@@ -2,7 +2,7 @@
 // fluxes (riemannSolver::solve with reconstruction, riemannsolver.cpp:
 // 202-328) and the explicit sources (fluidElement::computeExplicitSources
 // with computeEMHDGradients, physics.cpp:360-586), compiled with
-// -ffp-contract=fast.
+// -ffp-contract=fast-honor-pragmas (fused multiply-adds).
 //
 // Same mathematics as the exact kernels of step.hip, reorganised for fewer
 // instructions:
@@ -22,6 +22,7 @@
 // results are tested against the exact kernels and the oracle to 1e-12.
 #include <hip/hip_runtime.h>
 
+#include "fast_ops.hpp"
 #include "grim_hip.h"
 #include "runtime.hpp"
 #include "step_fast.hpp"
@@ -33,37 +34,38 @@ namespace {
 
 // WENO5 face value of one zone from y[0..4] = in[i-2 .. i+2]: SIDE 0 the
 // value at i + 1/2 (grim's primRight), SIDE 1 at i - 1/2 (primLeft).
+// grim's smoothness indicators (weno5.cpp:165-193) are the quadratic forms
+// beta_k = 13/12 (second difference)^2 + 1/4 (first-difference combination)^2;
+// they are formed here as 12 beta_k (eps2 scaled with them), a factor that
+// cancels in the normalised weights.  The weights c_k / beta_k^2 and their
+// sum are scaled by beta_1^2 beta_2^2 beta_3^2 and the linear weights folded
+// into the candidate stencils, which leaves one quotient.
 template <int SIDE>
 __device__ __forceinline__ double weno5Fast(const double* y) {
-  const double eps2 = 1.0e-17;
+  const double eps = 12. * 1.0e-17;
   const double y0 = y[0], y1 = y[1], y2 = y[2], y3 = y[3], y4 = y[4];
-  const double beta1 = ((4.0 / 3.0) * y0 * y0 - (19.0 / 3.0) * y0 * y1 + (25.0 / 3.0) * y1 * y1 +
-                        (11.0 / 3.0) * y0 * y2 - (31.0 / 3.0) * y1 * y2 + (10.0 / 3.0) * y2 * y2) +
-                       eps2 * (1.0 + fabs(y0) + fabs(y1) + fabs(y2));
-  const double beta2 = ((4.0 / 3.0) * y1 * y1 - (13.0 / 3.0) * y1 * y2 + (13.0 / 3.0) * y2 * y2 +
-                        (5.0 / 3.0) * y1 * y3 - (13.0 / 3.0) * y2 * y3 + (4.0 / 3.0) * y3 * y3) +
-                       eps2 * (1.0 + fabs(y1) + fabs(y2) + fabs(y3));
-  const double beta3 = ((10.0 / 3.0) * y2 * y2 - (31.0 / 3.0) * y2 * y3 + (25.0 / 3.0) * y3 * y3 +
-                        (11.0 / 3.0) * y2 * y4 - (19.0 / 3.0) * y3 * y4 + (4.0 / 3.0) * y4 * y4) +
-                       eps2 * (1.0 + fabs(y2) + fabs(y3) + fabs(y4));
+  const double a12 = fabs(y1) + fabs(y2), a23 = fabs(y2) + fabs(y3);
+  const double d1 = y0 - 2. * y1 + y2, e1 = y0 - 4. * y1 + 3. * y2;
+  const double d2 = y1 - 2. * y2 + y3, e2 = y1 - y3;
+  const double d3 = y2 - 2. * y3 + y4, e3 = 3. * y2 - 4. * y3 + y4;
+  const double beta1 = 13. * d1 * d1 + 3. * e1 * e1 + eps * (1. + fabs(y0) + a12);
+  const double beta2 = 13. * d2 * d2 + 3. * e2 * e2 + eps * (1. + a12 + fabs(y3));
+  const double beta3 = 13. * d3 * d3 + 3. * e3 * e3 + eps * (1. + a23 + fabs(y4));
   const double b1 = beta1 * beta1, b2 = beta2 * beta2, b3 = beta3 * beta3;
-  double w1, w2, w3, u1, u2, u3;
+  const double p1 = b2 * b3, p2 = b1 * b3, p3 = b1 * b2;
+  double num, den;
   if constexpr (SIDE == 0) {
-    w1 = b2 * b3;
-    w2 = 10. * (b1 * b3);
-    w3 = 5. * (b1 * b2);
-    u1 = 0.375 * y0 - 1.25 * y1 + 1.875 * y2;
-    u2 = -0.125 * y1 + 0.75 * y2 + 0.375 * y3;
-    u3 = 0.375 * y2 + 0.75 * y3 - 0.125 * y4;
+    // linear weights (1, 10, 5) / 16 of the candidates at i + 1/2
+    num = p1 * (0.375 * y0 - 1.25 * y1 + 1.875 * y2) + p2 * (-1.25 * y1 + 7.5 * y2 + 3.75 * y3) +
+          p3 * (1.875 * y2 + 3.75 * y3 - 0.625 * y4);
+    den = p1 + 10. * p2 + 5. * p3;
   } else {
-    w1 = 5. * (b2 * b3);
-    w2 = 10. * (b1 * b3);
-    w3 = b1 * b2;
-    u1 = -0.125 * y0 + 0.75 * y1 + 0.375 * y2;
-    u2 = 0.375 * y1 + 0.75 * y2 - 0.125 * y3;
-    u3 = 1.875 * y2 - 1.25 * y3 + 0.375 * y4;
+    // (5, 10, 1) / 16 at i - 1/2
+    num = p1 * (-0.625 * y0 + 3.75 * y1 + 1.875 * y2) + p2 * (3.75 * y1 + 7.5 * y2 - 1.25 * y3) +
+          p3 * (1.875 * y2 - 1.25 * y3 + 0.375 * y4);
+    den = 5. * p1 + 10. * p2 + p3;
   }
-  return (w1 * u1 + w2 * u2 + w3 * u3) / (w1 + w2 + w3);
+  return fastDiv(num, den);
 }
 
 template <int RECON, int SIDE>
@@ -87,12 +89,12 @@ __device__ __forceinline__ void kinematicsFast(const double* P, const ZoneGeom& 
   const double u1 = P[U1], u2 = P[U2], u3 = P[U3];
   const double B1 = P[L::B1], B2 = P[L::B2], B3 = P[L::B3];
   k.pressure = (prm.gam - 1.) * k.u;
-  k.temperature = smax(k.pressure / k.rho, prm.tFloor);
+  k.temperature = smax(fastDiv(k.pressure, k.rho), prm.tFloor);
   const double gamma = sqrt(1. + G.cov(1, 1) * u1 * u1 + G.cov(2, 2) * u2 * u2 + G.cov(3, 3) * u3 * u3 +
                             2. * (G.cov(1, 2) * u1 * u2 + G.cov(1, 3) * u1 * u3 + G.cov(2, 3) * u2 * u3));
   k.gammaLF = gamma;
-  k.uCon[0] = gamma / G.alpha;
-  const double invU0 = G.alpha / gamma;
+  k.uCon[0] = fastDiv(gamma, G.alpha);
+  const double invU0 = fastDiv(G.alpha, gamma);
   const double ga = gamma * G.alpha;
   k.uCon[1] = u1 - ga * G.gcon0[1];
   k.uCon[2] = u2 - ga * G.gcon0[2];
@@ -119,7 +121,7 @@ __device__ __forceinline__ void kinematicsFast(const double* P, const ZoneGeom& 
 template <class L>
 __device__ __forceinline__ void closureFast(const double* P, const Kinematics& k, const DevParams& prm,
                                             double radius, Closure& c) {
-  const double cs2 = prm.gam * k.pressure / (k.rho + prm.gam * k.u);
+  const double cs2 = fastDiv(prm.gam * k.pressure, k.rho + prm.gam * k.u);
   c.soundSpeed = sqrt(cs2);
   c.bNorm = sqrt(k.bSqr);
   c.q = 0.;
@@ -141,28 +143,28 @@ __device__ __forceinline__ void closureFast(const double* P, const Kinematics& k
       c.nu = prm.viscAlpha * cs2 * c.tau;
     } else if (prm.closure == GRIM_CLOSURE_BUOYANCY) {
       c.chi = sqrt(radius);
-      c.tau = 1.2 * c.chi / cs2;
+      c.tau = fastDiv(1.2 * c.chi, cs2);
     } else {
       const double dyn = radius * sqrt(radius);
       double tau = dyn;
       const double cs3 = cs2 * c.soundSpeed;
       if constexpr (L::COND_ON) {
-        double y = fabs(c.q) / (prm.condClosureFactor * k.rho * cs3);
+        double y = fastDiv(fabs(c.q), prm.condClosureFactor * k.rho * cs3);
         y = exp((1. - y) * 100.);
-        tau = smin(tau, dyn * (y / (y + 1.) + 1.e-5));
+        tau = smin(tau, dyn * (fastDiv(y, y + 1.) + 1.e-5));
       }
       if constexpr (L::VISC_ON) {
         const double p = k.pressure, dP = c.deltaP, bs = k.bSqr;
-        const double dPmod = smax(p - (2. / 3.) * dP, prm.bSqrFloor) / smax(p + (1. / 3.) * dP, prm.bSqrFloor);
+        const double dPmod = fastDiv(smax(p - (2. / 3.) * dP, prm.bSqrFloor), smax(p + (1. / 3.) * dP, prm.bSqrFloor));
         const double dPmaxPlus = smin(prm.viscClosureFactor * bs * 0.5 * dPmod, (1.49 / 1.07) * p);
         const double dPmaxMinus = smax(-prm.viscClosureFactor * bs, (-2.99 / 1.07) * p);
         const double dPmax = (dP > 0.) ? dPmaxPlus : dPmaxMinus;
-        double y = fabs(dP) / (fabs(dPmax) + prm.bSqrFloor);
+        double y = fastDiv(fabs(dP), fabs(dPmax) + prm.bSqrFloor);
         y = exp((1. - y) * 100.);
-        tau = smin(tau, dyn * (y / (y + 1.) + 1.e-5));
+        tau = smin(tau, dyn * (fastDiv(y, y + 1.) + 1.e-5));
       }
       c.tau = tau;
-      const double maxAlpha = (1. - cs2) / (2. * cs2 + 1.e-12);
+      const double maxAlpha = fastDiv(1. - cs2, 2. * cs2 + 1.e-12);
       c.chi = smin(maxAlpha, prm.condAlpha) * cs2 * c.tau;
       c.nu = smin(maxAlpha, prm.viscAlpha) * cs2 * c.tau;
     }
@@ -178,8 +180,8 @@ struct StressInv {
 template <class L>
 __device__ __forceinline__ StressInv stressInv(const Kinematics& k, const Closure& c) {
   StressInv s{0., 0.};
-  if constexpr (L::COND_ON) s.qOverB = c.q / c.bNorm;
-  if constexpr (L::VISC_ON) s.dPOverB2 = c.deltaP / k.bSqr;
+  if constexpr (L::COND_ON) s.qOverB = fastDiv(c.q, c.bNorm);
+  if constexpr (L::VISC_ON) s.dPOverB2 = fastDiv(c.deltaP, k.bSqr);
   return s;
 }
 
@@ -227,15 +229,16 @@ __device__ __forceinline__ void fluxesFast(const double* P, const Kinematics& k,
 // The unit covectors A = dX^(DIR+1), B = dt pick single inverse-metric
 // entries: A.A = g^{ss}, B.B = g^{00}, A.B = g^{s0}, A.u = u^s, B.u = u^0.
 template <class L, int DIR>
-__device__ __forceinline__ void charSpeedsFast(const Kinematics& k, const Closure& c, const GeomFull& G,
-                                               const DevParams& prm, double& minSpeed, double& maxSpeed) {
+__device__ __forceinline__ void charSpeedsFast(const Kinematics& k, const Closure& c, double gconSS, double gcon00,
+                                               double gcon0S, const DevParams& prm, double& minSpeed,
+                                               double& maxSpeed) {
   const double rhoGu = k.rho + prm.gam * k.u;
-  const double cAlvenSqr = k.bSqr / (rhoGu + k.bSqr);
+  const double cAlvenSqr = fastDiv(k.bSqr, rhoGu + k.bSqr);
   double csSqr = c.soundSpeed * c.soundSpeed;
   double cVisSqr = 0., cConSqr = 0.;
   if constexpr (L::EMHD) {
-    const double invTau = 1. / c.tau;
-    if constexpr (L::VISC_ON) cVisSqr = (4. / 3.) * k.rho * c.nu * invTau / rhoGu;
+    const double invTau = fastRcp(c.tau);
+    if constexpr (L::VISC_ON) cVisSqr = fastDiv((4. / 3.) * k.rho * c.nu * invTau, rhoGu);
     if constexpr (L::COND_ON) cConSqr = (prm.gam - 1.) * c.chi * invTau;
   }
   cConSqr = 0.5 * (csSqr + cConSqr + sqrt(csSqr * csSqr + cConSqr * cConSqr));
@@ -243,13 +246,13 @@ __device__ __forceinline__ void charSpeedsFast(const Kinematics& k, const Closur
   csSqr = csSqr + cAlvenSqr - csSqr * cAlvenSqr;
   csSqr = csSqr > 1. ? 1. : csSqr;
   constexpr int sdir = DIR + 1;
-  const double ASqr = G.con(sdir, sdir), BSqr = G.con(0, 0), ADotB = G.con(sdir, 0);
+  const double ASqr = gconSS, BSqr = gcon00, ADotB = gcon0S;
   const double ADotU = k.uCon[sdir], BDotU = k.uCon[0];
   const double A = (BDotU * BDotU) - (BSqr + BDotU * BDotU) * csSqr;
   const double B = 2. * (ADotU * BDotU - (ADotB + ADotU * BDotU) * csSqr);
   const double C = ADotU * ADotU - (ASqr + ADotU * ADotU) * csSqr;
   const double discr = sqrt(smax(B * B - 4. * A * C, 1.e-16));
-  const double inv2A = 0.5 / A;
+  const double inv2A = 0.5 * fastRcp(A);
   double mn = (B - discr) * inv2A;
   double mx = (B + discr) * inv2A;
   mn = mn > -1.e-15 ? -1.e-15 : mn;
@@ -258,24 +261,49 @@ __device__ __forceinline__ void charSpeedsFast(const Kinematics& k, const Closur
   maxSpeed = mx;
 }
 
+// The geometry one side of a face needs: the zone geometry, the inverse
+// metric entry g^{ss} of the face direction and r (the torus closure).
+struct FaceGeom {
+  ZoneGeom zg;
+  double gconSS, r;
+};
+
+template <int DIR>
+__device__ __forceinline__ void loadFaceGeom(const double* geom, int64_t n, uint32_t zb, FaceGeom& G) {
+#pragma unroll
+  for (int k = 0; k < 10; k++) G.zg.gcov[k] = ldOff(geom + k * n, zb);
+#pragma unroll
+  for (int k = 0; k < 4; k++) G.zg.gcon0[k] = ldOff(geom + (GEOM_GCON0 + k) * n, zb);
+  G.zg.alpha = ldOff(geom + GEOM_ALPHA * n, zb);
+  G.zg.g = ldOff(geom + GEOM_G * n, zb);
+  // planes 16..21 hold g^{11} g^{12} g^{13} g^{22} g^{23} g^{33}
+  constexpr int ss = DIR == 0 ? 0 : (DIR == 1 ? 3 : 5);
+  G.gconSS = ldOff(geom + (16 + ss) * n, zb);
+  G.r = ldOff(geom + LOC_R * n, zb);
+}
+
 // One side of a face (as faceState in step.hip): the face value of every
 // variable, the element at the face geometry, its flux, conserved
-// variables and speeds.
+// variables and speeds.  zb: the byte offset of the face's zone in a plane.
 template <class L, int DIR, int RECON, int SIDE>
-__device__ __forceinline__ void faceStateFast(const StepArgs& a, int64_t z, const double* __restrict__ prim,
+__device__ __forceinline__ void faceStateFast(const StepArgs& a, uint32_t zb, const double* __restrict__ prim,
                                               const double* __restrict__ geomFace, double (&F)[L::DOF],
                                               double (&U)[L::DOF], double& mn, double& mx) {
-  const int64_t n = a.D.n, s = a.D.s[DIR];
+  const int64_t n = a.D.n;
+  // the stencil's offsets along DIR, shared by every variable's plane
+  uint32_t off[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) off[q] = zb + (uint32_t)((q - 3 + SIDE) * a.D.s[DIR] * 8);
   double P[L::DOF];
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) {
     double y[5];
 #pragma unroll
-    for (int q = 0; q < 5; q++) y[q] = prim[v * n + z + (q - 3 + SIDE) * s];
+    for (int q = 0; q < 5; q++) y[q] = ldOff(prim + v * n, off[q]);
     P[v] = faceValue<RECON, SIDE>(y, a.theta);
   }
-  GeomFull G;
-  loadGeomFull(geomFace, n, z, G);
+  FaceGeom G;
+  loadFaceGeom<DIR>(geomFace, n, zb, G);
   Kinematics kk;
   Closure cc;
   kinematicsFast<L>(P, G.zg, a.prm, kk);
@@ -283,7 +311,7 @@ __device__ __forceinline__ void faceStateFast(const StepArgs& a, int64_t z, cons
   const StressInv si = stressInv<L>(kk, cc);
   fluxesFast<L, DIR + 1>(P, kk, cc, si, G.zg.g, F);
   fluxesFast<L, 0>(P, kk, cc, si, G.zg.g, U);
-  charSpeedsFast<L, DIR>(kk, cc, G, a.prm, mn, mx);
+  charSpeedsFast<L, DIR>(kk, cc, G.gconSS, G.zg.gcon0[0], G.zg.gcon0[DIR + 1], a.prm, mn, mx);
 }
 
 // The split face kernel of step.hip with the fast element: wavefronts 0-1
@@ -298,16 +326,16 @@ __global__ __launch_bounds__(kBlock) void faceFluxFastKernel(StepArgs a, Box box
   const int64_t f = (int64_t)blockIdx.x * kFacesPerBlock + lf;
   const bool active = f < box.count();
   const int64_t n = a.D.n;
-  int64_t z = 0;
+  uint32_t zb = 0;
   double F[L::DOF], U[L::DOF], mn = 0., mx = 0.;
   if (active) {
     int i, j, k;
     box.at(f, i, j, k);
-    z = flatIndex(a.D, i, j, k);
+    zb = ((uint32_t)i + (uint32_t)j * (uint32_t)a.D.s[1] + (uint32_t)k * (uint32_t)a.D.s[2]) * 8u;
     if (side == 0) {
-      faceStateFast<L, DIR, RECON, 0>(a, z, prim, geomFace, F, U, mn, mx);
+      faceStateFast<L, DIR, RECON, 0>(a, zb, prim, geomFace, F, U, mn, mx);
     } else {
-      faceStateFast<L, DIR, RECON, 1>(a, z, prim, geomFace, F, U, mn, mx);
+      faceStateFast<L, DIR, RECON, 1>(a, zb, prim, geomFace, F, U, mn, mx);
 #pragma unroll
       for (int v = 0; v < L::DOF; v++) {
         sh[v][lf] = F[v];
@@ -322,19 +350,19 @@ __global__ __launch_bounds__(kBlock) void faceFluxFastKernel(StepArgs a, Box box
   const double minSpeed = stdmin(mn, sh[2 * L::DOF][lf]);
   const double maxSpeed = stdmax(mx, sh[2 * L::DOF + 1][lf]);
   if (a.riemann == GRIM_RIEMANN_HLL) {
-    const double inv = 1. / (maxSpeed - minSpeed);
+    const double inv = fastRcp(maxSpeed - minSpeed);
     const double mm = minSpeed * maxSpeed;
 #pragma unroll
     for (int v = 0; v < L::DOF; v++) {
       const double FR = sh[v][lf], UR = sh[L::DOF + v][lf];
-      flux[v * n + z] = (maxSpeed * F[v] - minSpeed * FR + mm * (UR - U[v])) * inv;
+      stOff(flux + v * n, zb, (maxSpeed * F[v] - minSpeed * FR + mm * (UR - U[v])) * inv);
     }
   } else {
     const double c = stdmax(maxSpeed, -minSpeed);
 #pragma unroll
     for (int v = 0; v < L::DOF; v++) {
       const double FR = sh[v][lf], UR = sh[L::DOF + v][lf];
-      flux[v * n + z] = 0.5 * (F[v] + FR - c * (UR - U[v]));
+      stOff(flux + v * n, zb, 0.5 * (F[v] + FR - c * (UR - U[v])));
     }
   }
 }
@@ -521,6 +549,7 @@ inline unsigned blocks(int64_t count, int per) { return (unsigned)((count + per 
 
 int launchFaceFast(const grim_params& p, const StepArgs& a, int dir, int recon, const Box& b, const double* prim,
                    const double* geomFace, double* flux, hipStream_t s) {
+  GRIM_REQUIRE(planeOffsetsFit(a.D.n), "fast explicit stage: planes must hold fewer than 2^29 zones");
   const unsigned nb = blocks(b.count(), kFacesPerBlock);
   return dispatchLayout(p, [&]<class L>() -> int {
     auto go = [&]<int DIR>() {
@@ -545,6 +574,7 @@ int launchFaceFast(const grim_params& p, const StepArgs& a, int dir, int recon, 
 int launchSourcesFast(const grim_params& p, const StepArgs& a, int recon, const Box& b, const double* prim,
                       const double* primOld, const double* geomC, const double* gamma40, const double* uT,
                       double* sExp, double* sImp, double* consOld, hipStream_t s) {
+  GRIM_REQUIRE(planeOffsetsFit(a.D.n), "fast explicit stage: planes must hold fewer than 2^29 zones");
   const unsigned nb = blocks(b.count(), kBlock);
   return dispatchLayout(p, [&]<class L>() -> int {
     switch (recon) {

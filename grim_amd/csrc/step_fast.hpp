@@ -1,5 +1,5 @@
 // Host launchers of the fast explicit stage (step_fast.hip, compiled with
-// -ffp-contract=fast): the face fluxes and the explicit sources of grim's
+// -ffp-contract=fast-honor-pragmas): the face fluxes and the explicit sources of grim's
 // stage evaluated with shared reciprocals, fused multiply-adds and the
 // 40-plane symmetric connection.  The results agree with the exact kernels
 // of step.hip (grim's expression order, one rounding per operation) to

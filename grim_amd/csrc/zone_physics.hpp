@@ -244,13 +244,16 @@ struct ZoneConst {
   double normQ, normDP;  // residual row normalisation, residual.cpp:61-93
 };
 
-template <class L>
+// RHS_IN_MEMORY false: the caller sets rhs and mask itself.
+template <class L, bool RHS_IN_MEMORY = true>
 __device__ __forceinline__ void loadZoneConst(const grim_zone_batch& b, const DevParams& prm,
                                               const ZoneGeom& G, int64_t z, ZoneConst<L>& c) {
   const int64_t n = b.n;
+  if constexpr (RHS_IN_MEMORY) {
 #pragma unroll
-  for (int v = 0; v < L::NF; v++) c.rhs[v] = b.rhs[v * n + z];
-  c.mask = b.residual_mask ? b.residual_mask[z] : 1.;
+    for (int v = 0; v < L::NF; v++) c.rhs[v] = b.rhs[v * n + z];
+    c.mask = b.residual_mask ? b.residual_mask[z] : 1.;
+  }
   c.alphaC = b.closure ? b.closure[z] : 1.;
   c.alphaV = b.closure ? b.closure[n + z] : 1.;
   c.radius = b.radius ? b.radius[z] : 0.;

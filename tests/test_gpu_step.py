@@ -834,6 +834,41 @@ def test_fast_explicit_pieces_match_exact(torch, emhd):
     ts1.close()
 
 
+@pytest.mark.parametrize("fast", [0, 1], ids=["exact", "fast_explicit"])
+def test_torus_time_step_equals_its_stages(torch, fast):
+    """grim_time_step on the 3D EMHD torus with the escalating mixed solve
+    (the stage fold runs inside the compacting solve's first phase, and the
+    scrub, floors, copies and next dt's partial maxima run as one kernel)
+    against the same two steps written out stage by stage (grim_step_begin,
+    grim_fold_rhs, grim_solve, grim_stage_finish, grim_step_end: the separate
+    fold kernel, floor kernel and dt reduction), from the same state: equal
+    bit for bit, time and dt included."""
+    from grim_amd import capi
+
+    opts = capi.newton_opts(capi.JAC_ANALYTIC, capi.LU_FP32_FP64)
+    fused = torus_case(True, N1=64, N2=48, N3=8, opts=opts, fast=fast)[0]
+    staged, g = torus_case(True, N1=64, N2=48, N3=8, opts=opts, fast=fast)[:2]
+    assert np.array_equal(fused.primOld, staged.primOld)
+    for ts in (fused, staged):
+        ts.set_time(0., 0.005)
+    for step in range(2):
+        fused.timeStep()
+        staged.stepBegin()
+        for st in range(2):
+            staged.foldRHS(st)
+            staged.solve(st)
+            staged.stageFinish(st)
+        staged.stepEnd()
+        a, b = fused.primOld, staged.primOld
+        assert np.isfinite(bulk(g, a)).all()
+        assert np.array_equal(a, b), f"step {step}: max |diff| {np.nanmax(np.abs(a - b)):.3e}"
+        assert fused.time == staged.time and fused.dt == staged.dt, f"step {step}"
+    print(f"torus EMHD mixed, explicitFast={fast}: 2 fused steps bitwise equal to the staged ones, "
+          f"dt {fused.dt:.6e}")
+    fused.close()
+    staged.close()
+
+
 @pytest.mark.parametrize("solver", ["fd_fp64", "mixed_escalating"])
 def test_fast_explicit_torus_steps_match_oracle(torch, solver):
     """Two EMHD torus steps with the fast explicit stage against the oracle's
