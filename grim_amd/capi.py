@@ -91,6 +91,7 @@ class ZoneBatch(C.Structure):
         ("dt_prime", C.c_double),
         ("dt_prime_device", C.c_void_p),
         ("guess", C.c_void_p),
+        ("geom_columns", C.c_int64),
     ]
 
 
@@ -242,6 +243,7 @@ SIGNATURES = {
     "grim_destroy": (_i, [_vp]),
     "grim_prim": (_i, [_vp, _pvp]),
     "grim_geometry": (_i, [_vp, _i, _pvp]),
+    "grim_geometry_columns": (C.c_int64, [_vp]),
     "grim_set_prim": (_i, [_vp, _vp, _i]),
     "grim_get_prim": (_i, [_vp, _vp, _i]),
     "grim_get_geometry": (_i, [_vp, _i, _vp, _i]),

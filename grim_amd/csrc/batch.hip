@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void knownRootKernel(DevParams prm, grim_zo
   if (z >= b.n) return;
   const int64_t n = b.n;
   ZoneGeom G;
-  loadGeom(b.geom, n, z, G);
+  loadBatchGeom(b, z, G);
   ZoneConst<L> c;
   loadZoneConst<L>(b, prm, G, z, c);
   double P[L::DOF], U[L::NF], Si[L::NF], Std[L::NF];

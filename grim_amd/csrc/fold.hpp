@@ -118,7 +118,7 @@ __device__ __forceinline__ void foldZone(const Dims& D, const double* dX, const 
       rmax = stdmax(rmax, fabs(r));
     } else {
       const double consB = U - dtPrime * div;
-      B[v - L::NF] = consB / geomC[GEOM_G * n + z];
+      B[v - L::NF] = consB / geomC[GEOM_G * D.ng + geomIndex(D, i, j)];
     }
   }
 }

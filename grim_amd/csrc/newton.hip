@@ -33,7 +33,7 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonGlobalCheck(DevParams prm
   if (z < b.n) {
     const int64_t n = b.n;
     ZoneGeom G;
-    loadGeom(b.geom, n, z, G);
+    loadBatchGeom(b, z, G);
     ZoneConst<L> c;
     loadZoneConst<L>(b, prm, G, z, c);
     double P[L::DOF];
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonGlobalUpdate(DevParams pr
   if (b.residual_mask && b.residual_mask[z] == 0.) return;
   const int64_t n = b.n;
   ZoneGeom G;
-  loadGeom(b.geom, n, z, G);
+  loadBatchGeom(b, z, G);
   ZoneConst<L> c;
   loadZoneConst<L>(b, prm, G, z, c);
   double P[L::DOF], R[L::NF], Rt[L::NF];
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kDefaultBlock) void newtonGlobalFinish(DevParams pr
   if (!stopped) {
     const int64_t n = b.n;
     ZoneGeom G;
-    loadGeom(b.geom, n, z, G);
+    loadBatchGeom(b, z, G);
     ZoneConst<L> c;
     loadZoneConst<L>(b, prm, G, z, c);
     double P[L::DOF], R[L::NF];
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kDefaultBlock) void residualKernel(DevParams prm, g
   if (z >= b.n) return;
   const int64_t n = b.n;
   ZoneGeom G;
-  loadGeom(b.geom, n, z, G);
+  loadBatchGeom(b, z, G);
   ZoneConst<L> c;
   loadZoneConst<L>(b, prm, G, z, c);
   double P[L::DOF], R[L::NF];
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kDefaultBlock) void fdJacobianKernel(DevParams prm,
   if (z >= b.n) return;
   const int64_t n = b.n;
   ZoneGeom G;
-  loadGeom(b.geom, n, z, G);
+  loadBatchGeom(b, z, G);
   ZoneConst<L> c;
   loadZoneConst<L>(b, prm, G, z, c);
   double P[L::DOF], R[L::NF], A[L::NF][L::NF];
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(kDefaultBlock) void analyticJacobianKernel(DevParam
   if (z >= b.n) return;
   const int64_t n = b.n;
   ZoneGeom G;
-  loadGeom(b.geom, n, z, G);
+  loadBatchGeom(b, z, G);
   ZoneConst<L> c;
   loadZoneConst<L>(b, prm, G, z, c);
   EmhdWeights ew;
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(kDefaultBlock) void derivativeProductKernel(DevPara
   const int64_t n = b.n;
   const double dt = batchDt(b);
   ZoneGeom G;
-  loadGeom(b.geom, n, z, G);
+  loadBatchGeom(b, z, G);
   ZoneConst<L> c;
   loadZoneConst<L>(b, prm, G, z, c);
   EmhdWeights ew;
@@ -262,6 +262,9 @@ int validBatch(const grim_params* p, const grim_zone_batch* b) {
       !b->radius)
     return 0;
   if (!b->dt_prime_device && !(b->dt_prime > 0.)) return 0;
+  // column geometry: whole columns, 32-bit zone indices
+  if (b->geom_columns < 0) return 0;
+  if (b->geom_columns > 0 && (b->n % b->geom_columns != 0 || b->n > INT32_MAX)) return 0;
   return 1;
 }
 

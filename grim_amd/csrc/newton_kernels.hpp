@@ -299,7 +299,7 @@ __device__ __forceinline__ bool solveZone(const DevParams& prm, const grim_zone_
   }
   const double dt = batchDt(b);
   ZoneGeom G;
-  loadGeom(b.geom, n, z, G);
+  loadBatchGeom(b, z, G);
   ZoneConst<L> c;
   loadZoneConst<L, !FOLDED>(b, prm, G, z, c);
   if constexpr (FOLDED) {

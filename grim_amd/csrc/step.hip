@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "grim_hip.h"
 #include "runtime.hpp"
@@ -57,18 +58,19 @@ __device__ __forceinline__ void XCoordsAt(const Dims& D, const CoordGrid& cg, in
   X[2] = cg.start[2] + (i3 + 0.5) * cg.dX[2];
 }
 
+// One thread per (X1, X2) column (geomIndex): the metric does not depend on
+// X3, so the column's value is every zone's of that column.
 __global__ __launch_bounds__(kBlock) void geometryKernel(Dims D, CoordMap cm, CoordGrid cg, int loc,
                                                          double* __restrict__ out) {
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (z >= D.n) return;
-  int i, j, k;
-  zoneIndex(D, z, i, j, k);
+  if (z >= D.ng) return;
+  const int i = (int)(z % D.NT[0]), j = (int)(z / D.NT[0]), k = 0;
   double X[3], x[3], G[4][4], C[4][4], det;
   XCoordsAt(D, cg, i, j, k, loc, X);
   cm.toX(X, x);
   cm.gCov(X, G);
   gDetAndgCon(G, det, C);
-  const int64_t n = D.n;
+  const int64_t n = D.ng;
   const int cov[10][2] = {{0, 0}, {0, 1}, {0, 2}, {0, 3}, {1, 1}, {1, 2}, {1, 3}, {2, 2}, {2, 3}, {3, 3}};
 #pragma unroll
   for (int q = 0; q < 10; q++) out[q * n + z] = G[cov[q][0]][cov[q][1]];
@@ -85,9 +87,8 @@ __global__ __launch_bounds__(kBlock) void geometryKernel(Dims D, CoordMap cm, Co
 __global__ __launch_bounds__(kBlock) void connectionKernel(Dims D, CoordMap cm, CoordGrid cg,
                                                            double* __restrict__ gamma) {
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (z >= D.n) return;
-  int i, j, k;
-  zoneIndex(D, z, i, j, k);
+  if (z >= D.ng) return;
+  const int i = (int)(z % D.NT[0]), j = (int)(z / D.NT[0]), k = 0;
   const double EPS = 1.e-5;
   double X[3], G[4][4], C[4][4], det;
   XCoordsAt(D, cg, i, j, k, 0, X);
@@ -101,7 +102,6 @@ __global__ __launch_bounds__(kBlock) void connectionKernel(Dims D, CoordMap cm, 
     cm.gCov(Xp, Gp[d]);
     cm.gCov(Xm, Gm[d]);
   }
-  const int64_t n = D.n;
   for (int mu = 0; mu < 4; mu++)
     for (int nu = 0; nu < 4; nu++)
       for (int la = 0; la < 4; la++) {
@@ -122,8 +122,19 @@ __global__ __launch_bounds__(kBlock) void connectionKernel(Dims D, CoordMap cm, 
           }
           s += C[mu][eta] * out;
         }
-        gamma[(mu * 16 + nu * 4 + la) * n + z] = s;
+        gamma[(mu * 16 + nu * 4 + la) * D.ng + z] = s;
       }
+}
+
+// dst[q n + z] = src[q ng + z mod ng]: planes of one value per (X1, X2)
+// column written out for every zone.
+__global__ __launch_bounds__(kBlock) void expandColumnsKernel(int64_t count, int64_t n, int64_t ng,
+                                                              const double* __restrict__ src,
+                                                              double* __restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int64_t q = t / n, z = t - q * n;
+  dst[t] = src[q * ng + z % ng];
 }
 
 __global__ __launch_bounds__(kBlock) void maskKernel(Dims D, double* __restrict__ mask) {
@@ -252,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void reconstructKernel(Dims D, Box box, int
 constexpr int GIVEN_STATES = -1;
 
 template <class L, int DIR, int RECON, int SIDE>
-__device__ __forceinline__ void faceState(const StepArgs& a, int64_t z, const double* __restrict__ prim,
+__device__ __forceinline__ void faceState(const StepArgs& a, int64_t z, int64_t zg, const double* __restrict__ prim,
                                           const double* __restrict__ geomFace, double (&F)[L::DOF],
                                           double (&U)[L::DOF], double& mn, double& mx) {
   const int64_t n = a.D.n, s = a.D.s[DIR];
@@ -271,7 +282,7 @@ __device__ __forceinline__ void faceState(const StepArgs& a, int64_t z, const do
     }
   }
   GeomFull G;
-  loadGeomFull(geomFace, n, z, G);
+  loadGeomFull(geomFace, a.D.ng, zg, G);
   Kinematics kk;
   Closure cc;
   kinematics<L>(P, G.zg, a.prm, kk);
@@ -301,10 +312,11 @@ __global__ __launch_bounds__(kBlock) void faceFluxSplitKernel(
     int i, j, k;
     box.at(f, i, j, k);
     z = flatIndex(D, i, j, k);
+    const int64_t zg = geomIndex(D, i, j);
     if (side == 0) {
-      faceState<L, DIR, RECON, 0>(a, z, RECON == GIVEN_STATES ? primRight : prim, geomFace, F, U, mn, mx);
+      faceState<L, DIR, RECON, 0>(a, z, zg, RECON == GIVEN_STATES ? primRight : prim, geomFace, F, U, mn, mx);
     } else {
-      faceState<L, DIR, RECON, 1>(a, z, prim, geomFace, F, U, mn, mx);
+      faceState<L, DIR, RECON, 1>(a, z, zg, prim, geomFace, F, U, mn, mx);
 #pragma unroll
       for (int v = 0; v < L::DOF; v++) {
         sh[v][lf] = F[v];
@@ -447,7 +459,7 @@ __global__ __launch_bounds__(kBlock) void uTKernel(StepArgs a, const double* __r
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
   ZoneGeom G;
-  loadGeom(geomC, n, z, G);
+  loadGeom(geomC, a.D.ng, geomIndexOf(a.D, z), G);
   Kinematics kk;
   kinematics<L>(P, G, a.prm, kk);
 #pragma unroll
@@ -475,11 +487,12 @@ __global__ __launch_bounds__(kBlock) void sourcesKernel(StepArgs a, Box box,
   box.at(t, i, j, k);
   const Dims& D = a.D;
   const int64_t n = D.n, z = flatIndex(D, i, j, k);
+  const int64_t ng = D.ng, zg = geomIndex(D, i, j);
   double P[L::DOF];
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
   GeomFull G;
-  loadGeomFull(geomC, n, z, G);
+  loadGeomFull(geomC, ng, zg, G);
   Kinematics kk;
   Closure cc;
   kinematics<L>(P, G.zg, a.prm, kk);
@@ -503,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void sourcesKernel(StepArgs a, Box box,
       for (int lamda = 0; lamda < 4; lamda++)
 #pragma unroll
         for (int nu = 0; nu < 4; nu++)
-          src[UU + nu] -= g * Tk[lamda] * gamma[(lamda * 16 + kappa * 4 + nu) * n + z];
+          src[UU + nu] -= g * Tk[lamda] * gamma[(lamda * 16 + kappa * 4 + nu) * ng + zg];
     };
     {
       double T[4];
@@ -527,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void sourcesKernel(StepArgs a, Box box,
     }
   }
   if constexpr (L::EMHD) {
-    auto gam = [&](int l, int p, int q) { return gamma[(l * 16 + p * 4 + q) * n + z]; };
+    auto gam = [&](int l, int p, int q) { return gamma[(l * 16 + p * 4 + q) * ng + zg]; };
     auto slopeOf = [&](const double* arr, int d) {
       double y[5];
 #pragma unroll
@@ -630,7 +643,7 @@ __global__ __launch_bounds__(kBlock) void elementKernel(StepArgs a, int what, in
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
   GeomFull G;
-  loadGeomFull(geom, n, z, G);
+  loadGeomFull(geom, a.D.ng, geomIndexOf(a.D, z), G);
   Kinematics kk;
   Closure cc;
   kinematics<L>(P, G.zg, a.prm, kk);
@@ -758,7 +771,7 @@ __global__ __launch_bounds__(kBlock) void dtKernel(StepArgs a, const double* __r
 #pragma unroll
     for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
     GeomFull G;
-    loadGeomFull(geomC, n, z, G);
+    loadGeomFull(geomC, a.D.ng, geomIndexOf(a.D, z), G);
     m = zoneInvDt<L>(a, P, G);
   }
   blockMaxStore(red, m, blockMax);
@@ -934,7 +947,7 @@ __global__ __launch_bounds__(kBlock) void floorPlanesKernel(Dims D, grim_torus_p
                                                             double* __restrict__ floors) {
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= D.n) return;
-  const double Radius = geomC[LOC_R * D.n + z];
+  const double Radius = geomC[LOC_R * D.ng + geomIndexOf(D, z)];
   floors[z] = pow(Radius, t.RhoFloorSlope) * t.RhoFloorAmpl;
   floors[D.n + z] = pow(Radius, t.UFloorSlope) * t.UFloorAmpl;
 }
@@ -952,12 +965,14 @@ __global__ __launch_bounds__(kBlock) void torusICKernel(StepArgs a, grim_torus_p
   int i, j, k;
   zoneIJK(D, z, i, j, k);
   const double X2 = cg.start[1] + ((double)j - D.g[1] + 0.5) * cg.dX[1];
-  const double beta[4] = {0., geomC[11 * n + z], geomC[12 * n + z], geomC[13 * n + z]};
+  const int64_t ng = D.ng, zg = geomIndex(D, i, j);
+  const double beta[4] = {0., geomC[11 * ng + zg], geomC[12 * ng + zg], geomC[13 * ng + zg]};
   double P[L::DOF];
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) P[v] = 0.;
-  torusZone(t, aBH, hSlope, a.prm.gam, a.prm.rhoFloor, a.prm.uFloor, geomC[LOC_R * n + z],
-            geomC[LOC_THETA * n + z], X2, geomC[GEOM_ALPHA * n + z], beta, uniform01(t.seed, (uint64_t)z, 0), P);
+  torusZone(t, aBH, hSlope, a.prm.gam, a.prm.rhoFloor, a.prm.uFloor, geomC[LOC_R * ng + zg],
+            geomC[LOC_THETA * ng + zg], X2, geomC[GEOM_ALPHA * ng + zg], beta, uniform01(t.seed, (uint64_t)z, 0),
+            P);
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) prim[v * n + z] = P[v];
 }
@@ -1006,10 +1021,11 @@ __global__ __launch_bounds__(kBlock) void vectorPotentialKernel(Dims D, grim_tor
   else
     rhoAvg = rhoAvg / 4.;
   const double zero = rhoAvg * 0.;
+  const int64_t zg = geomIndex(D, I[0], I[1]);
   if (t.UseMADdisk)
-    Avec[z] = stdmax(zero, rhoAvg * rhoAvg * pow(geomC[LOC_R * D.n + z], 4));
+    Avec[z] = stdmax(zero, rhoAvg * rhoAvg * pow(geomC[LOC_R * D.ng + zg], 4));
   else
-    Avec[z] = stdmax(rhoAvg - 0.2, zero) * cos(geomC[LOC_THETA * D.n + z] * (t.MagneticLoops - 1));
+    Avec[z] = stdmax(rhoAvg - 0.2, zero) * cos(geomC[LOC_THETA * D.ng + zg] * (t.MagneticLoops - 1));
 }
 
 // B = curl A in the bulk, zero in ghost zones (:382-404).
@@ -1026,7 +1042,7 @@ __global__ __launch_bounds__(kBlock) void curlKernel(Dims D, double dX1, double 
     const int ii = (I[0] + di) % D.NT[0], jj = (I[1] + dj) % D.NT[1];
     return Avec[flatIndex(D, ii, jj, I[2])];
   };
-  const double g = geomC[GEOM_G * n + z];
+  const double g = geomC[GEOM_G * D.ng + geomIndex(D, I[0], I[1])];
   const double A00 = A(0, 0), A01 = A(0, 1), A10 = A(1, 0), A11 = A(1, 1);
   const double B1 = (A01 - A00 + A11 - A10) / (2. * dX2 * g);
   const double B2 = (A00 - A10 + A01 - A11) / (2. * dX1 * g);
@@ -1053,7 +1069,7 @@ __global__ __launch_bounds__(kBlock) void betaKernel(StepArgs a, int mad, const 
 #pragma unroll
     for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
     ZoneGeom G;
-    loadGeom(geomC, n, z, G);
+    loadGeom(geomC, a.D.ng, geomIndexOf(a.D, z), G);
     Kinematics k;
     kinematics<L>(P, G, a.prm, k);
     if (mad) {
@@ -1104,7 +1120,7 @@ __global__ __launch_bounds__(kBlock) void floorKernel(StepArgs a, grim_torus_par
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
   GeomFull G;
-  loadGeomFull(geomC, n, z, G);
+  loadGeomFull(geomC, a.D.ng, geomIndexOf(a.D, z), G);
   applyFloorZone<L>(a.prm, t, G, floors[z], floors[n + z], P);
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) prim[v * n + z] = P[v];
@@ -1139,10 +1155,9 @@ __global__ __launch_bounds__(kBlock) void nanZeroKernel(int64_t n, int nvars, co
 // full-stage NaN scrub (when the bulk count is positive) and applyFloor
 // (fullStepDiagnostics, torus/problem.cpp:719-746) written back into prim,
 // the result into primOld and primDt (timestep.cpp:290-300), time += dt, and
-// the next
-// computeDt's per-workgroup maxima of the inverse time step of the result
-// (timestep.cpp:336-391), which the next step's reduction reads.  Each
-// value is the separate kernels' bit for bit.
+// the next computeDt's per-workgroup maxima of the inverse time step of the
+// result (timestep.cpp:336-391), which the next step's reduction reads.
+// Each value is the separate kernels' bit for bit.
 template <class L, bool TORUS>
 __global__ __launch_bounds__(kBlock) void endStepKernel(StepArgs a, grim_torus_params t,
                                                         const double* __restrict__ geomC,
@@ -1161,7 +1176,7 @@ __global__ __launch_bounds__(kBlock) void endStepKernel(StepArgs a, grim_torus_p
 #pragma unroll
     for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
     GeomFull G;
-    loadGeomFull(geomC, n, z, G);
+    loadGeomFull(geomC, a.D.ng, geomIndexOf(a.D, z), G);
     if constexpr (TORUS) {
       if (nanCount[0] != 0) {
         bool bad = false;
@@ -1236,7 +1251,7 @@ __global__ __launch_bounds__(kBlock) void observersKernel(StepArgs a, Box box, d
 #pragma unroll
     for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
     ZoneGeom G;
-    loadGeom(geomC, n, z, G);
+    loadGeom(geomC, a.D.ng, geomIndex(a.D, i, j), G);
     Kinematics e;
     kinematics<L>(P, G, a.prm, e);
     const double rho = P[RHO], u = P[UU];
@@ -1324,14 +1339,15 @@ __global__ __launch_bounds__(kBlock) void inflowCheckKernel(StepArgs a, grim_tor
   double P[L::DOF];
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
+  const int64_t ng = D.ng;
   ZoneGeom G;
-  loadGeom(geomC, n, z, G);
+  loadGeom(geomC, ng, geomIndex(D, i, j), G);
   Kinematics kk;
   kinematics<L>(P, G, a.prm, kk);
   double u[3] = {P[U1] / kk.gammaLF, P[U2] / kk.gammaLF, P[U3] / kk.gammaLF};
   if (side == 1) {
-    const int64_t zl = flatIndex(D, D.N[0] + D.g[0] - 1, j, k);
-    const double minUr = geomC[(GEOM_GCON0 + 1) * n + zl] * geomC[GEOM_ALPHA * n + zl];
+    const int64_t zl = geomIndex(D, D.N[0] + D.g[0] - 1, j);
+    const double minUr = geomC[(GEOM_GCON0 + 1) * ng + zl] * geomC[GEOM_ALPHA * ng + zl];
     u[0] = stdmax(minUr, u[0]);
   } else {
     u[0] = stdmin(G.gcon0[1] * G.alpha, u[0]);
@@ -1367,7 +1383,7 @@ __global__ __launch_bounds__(kBlock) void fixPolesKernel(Dims D, const double* _
   const int idx2 = pole == 0 ? g2 + 2 : idx0 - 2;
   auto at = [&](int v, int j) -> double& { return prim[v * n + flatIndex(D, i, j, k)]; };
   auto Theta = [&](int j) {
-    const double th = geomC[LOC_THETA * n + flatIndex(D, i, j, k)];
+    const double th = geomC[LOC_THETA * D.ng + geomIndex(D, i, j)];
     return pole == 0 ? th : th * (-1.) + M_PI;
   };
   const int rows[2] = {idx0, idx1};
@@ -1712,12 +1728,13 @@ __global__ __launch_bounds__(kBlock) void dumpVarsKernel(StepArgs a, Box box, Co
   double P[L::DOF];
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
+  const int64_t zg = geomIndex(a.D, i, j);
   ZoneGeom G;
-  loadGeom(geomC, n, z, G);
+  loadGeom(geomC, a.D.ng, zg, G);
   Kinematics e;
   kinematics<L>(P, G, a.prm, e);
   Closure c;
-  closure<L>(P, e, a.prm, 1., 1., geomC[(int64_t)LOC_R * n + z], c);
+  closure<L>(P, e, a.prm, 1., 1., geomC[(int64_t)LOC_R * a.D.ng + zg], c);
   double ux[4], bx[4];
   ux[0] = e.uCon[0];
   bx[0] = e.bCon[0];
@@ -1908,7 +1925,8 @@ grim_zone_batch stageBatch(grim_ctx* x, int st, double* prim) {
   zb.prim_old = x->primOld;
   zb.prim_coef = st == 0 ? x->primOld : x->primHalf;
   zb.closure = nullptr;
-  zb.radius = x->geomC + (size_t)LOC_R * x->D.n;
+  zb.radius = x->geomC + (size_t)LOC_R * x->D.ng;
+  zb.geom_columns = x->D.ng;
   zb.residual_mask = x->mask;
   zb.dt_prime_device = stageDtPrime(x, st);
   return zb;
@@ -2153,10 +2171,25 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
       return GRIM_ERR_HIP;
     }
   }
-  geometryKernel<<<nb, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, 0, x->geomC);
-  geometryKernel<<<nb, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, 1, x->geomX1);
-  geometryKernel<<<nb, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, 2, x->geomX2);
-  connectionKernel<<<nb, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, x->gamma);
+  // geometry planes hold one value per (X1, X2) column (D.ng): the rest of
+  // each allocation is never read, and holds NaN so that a read of it shows
+  const struct {
+    void* ptr;
+    size_t bytes;
+  } nans[] = {{x->geomC, locB}, {x->geomX1, locB}, {x->geomX2, locB}, {x->gamma, sizeof(double) * 64 * (size_t)n}};
+  for (auto& nr : nans) {
+    hipError_t e = hipMemsetAsync(nr.ptr, 0xff, nr.bytes, x->s);
+    if (e != hipSuccess) {
+      setError(std::string("grim_create: hipMemsetAsync: ") + hipGetErrorString(e));
+      grim_destroy(x);
+      return GRIM_ERR_HIP;
+    }
+  }
+  const unsigned nbCols = blocksFor(x->D.ng);
+  geometryKernel<<<nbCols, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, 0, x->geomC);
+  geometryKernel<<<nbCols, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, 1, x->geomX1);
+  geometryKernel<<<nbCols, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, 2, x->geomX2);
+  connectionKernel<<<nbCols, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, x->gamma);
   if (c->explicitFast) {
     hipError_t e = hipMalloc((void**)&x->gamma40, sizeof(double) * 40 * (size_t)n);
     if (e != hipSuccess) {
@@ -2164,7 +2197,7 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
       grim_destroy(x);
       return GRIM_ERR_OOM;
     }
-    launchGamma40(n, x->gamma, x->gamma40, x->s);
+    launchGamma40(x->D.ng, x->gamma, x->gamma40, x->s);
   }
   maskKernel<<<nb, kBlock, 0, x->s>>>(x->D, x->mask);
   if (int rc = checkLaunch("grim_create")) {
@@ -2232,14 +2265,42 @@ int grim_get_prim(grim_ctx* x, double* dst, int on_device) {
   return GRIM_OK;
 }
 
+int64_t grim_geometry_columns(const grim_ctx* x) { return x ? x->D.ng : 0; }
+
 int grim_get_geometry(grim_ctx* x, int which, double* dst, int on_device) {
   GRIM_REQUIRE(x && dst && which >= 0 && which <= 4, "grim_get_geometry: bad argument");
   GRIM_REQUIRE(which != 4 || x->floors, "grim_get_geometry: no floors before grim_init_torus");
   double* g[5] = {x->geomC, x->geomX1, x->geomX2, x->gamma, x->floors};
   const size_t planes = which == 3 ? 64 : (which == 4 ? 2 : GRIM_LOC_PLANES);
-  GRIM_HIP_TRY(hipMemcpyAsync(dst, g[which], sizeof(double) * planes * x->D.n,
-                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, x->s));
+  const int64_t n = x->D.n;
+  if (which == 4) {
+    // the floor planes hold every zone
+    GRIM_HIP_TRY(hipMemcpyAsync(dst, g[which], sizeof(double) * planes * n,
+                                on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, x->s));
+    GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+    return GRIM_OK;
+  }
+  // every zone's value from its (X1, X2) column's
+  const int64_t ng = x->D.ng;
+  if (on_device) {
+    const int64_t count = (int64_t)planes * n;
+    expandColumnsKernel<<<blocksFor(count), kBlock, 0, x->s>>>(count, n, ng, g[which], dst);
+    if (int rc = checkLaunch("grim_get_geometry")) return rc;
+    GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+    return GRIM_OK;
+  }
+  std::vector<double> cols;
+  try {
+    cols.resize(planes * (size_t)ng);
+  } catch (const std::bad_alloc&) {
+    setError("grim_get_geometry: host allocation failed");
+    return GRIM_ERR_OOM;
+  }
+  GRIM_HIP_TRY(hipMemcpyAsync(cols.data(), g[which], sizeof(double) * planes * ng, hipMemcpyDeviceToHost, x->s));
   GRIM_HIP_TRY(hipStreamSynchronize(x->s));
+  for (size_t q = 0; q < planes; q++)
+    for (int64_t k = 0; k < n / ng; k++)
+      std::memcpy(dst + q * n + k * ng, cols.data() + q * ng, sizeof(double) * ng);
   return GRIM_OK;
 }
 

@@ -284,9 +284,11 @@ __device__ __forceinline__ void loadFaceGeom(const double* geom, int64_t n, uint
 
 // One side of a face (as faceState in step.hip): the face value of every
 // variable, the element at the face geometry, its flux, conserved
-// variables and speeds.  zb: the byte offset of the face's zone in a plane.
+// variables and speeds.  zb: the byte offset of the face's zone in a plane,
+// zgb: that of its (X1, X2) column in a geometry plane.
 template <class L, int DIR, int RECON, int SIDE>
-__device__ __forceinline__ void faceStateFast(const StepArgs& a, uint32_t zb, const double* __restrict__ prim,
+__device__ __forceinline__ void faceStateFast(const StepArgs& a, uint32_t zb, uint32_t zgb,
+                                              const double* __restrict__ prim,
                                               const double* __restrict__ geomFace, double (&F)[L::DOF],
                                               double (&U)[L::DOF], double& mn, double& mx) {
   const int64_t n = a.D.n;
@@ -303,7 +305,7 @@ __device__ __forceinline__ void faceStateFast(const StepArgs& a, uint32_t zb, co
     P[v] = faceValue<RECON, SIDE>(y, a.theta);
   }
   FaceGeom G;
-  loadFaceGeom<DIR>(geomFace, n, zb, G);
+  loadFaceGeom<DIR>(geomFace, a.D.ng, zgb, G);
   Kinematics kk;
   Closure cc;
   kinematicsFast<L>(P, G.zg, a.prm, kk);
@@ -331,11 +333,12 @@ __global__ __launch_bounds__(kBlock) void faceFluxFastKernel(StepArgs a, Box box
   if (active) {
     int i, j, k;
     box.at(f, i, j, k);
-    zb = ((uint32_t)i + (uint32_t)j * (uint32_t)a.D.s[1] + (uint32_t)k * (uint32_t)a.D.s[2]) * 8u;
+    const uint32_t zgb = ((uint32_t)i + (uint32_t)j * (uint32_t)a.D.s[1]) * 8u;
+    zb = zgb + (uint32_t)k * (uint32_t)a.D.s[2] * 8u;
     if (side == 0) {
-      faceStateFast<L, DIR, RECON, 0>(a, zb, prim, geomFace, F, U, mn, mx);
+      faceStateFast<L, DIR, RECON, 0>(a, zb, zgb, prim, geomFace, F, U, mn, mx);
     } else {
-      faceStateFast<L, DIR, RECON, 1>(a, zb, prim, geomFace, F, U, mn, mx);
+      faceStateFast<L, DIR, RECON, 1>(a, zb, zgb, prim, geomFace, F, U, mn, mx);
 #pragma unroll
       for (int v = 0; v < L::DOF; v++) {
         sh[v][lf] = F[v];
@@ -389,11 +392,12 @@ __global__ __launch_bounds__(kBlock) void sourcesFastKernel(StepArgs a, Box box,
   box.at(t, i, j, k);
   const Dims& D = a.D;
   const int64_t n = D.n, z = flatIndex(D, i, j, k);
+  const int64_t ng = D.ng, zg = geomIndex(D, i, j);
   double P[L::DOF];
 #pragma unroll
   for (int v = 0; v < L::DOF; v++) P[v] = prim[v * n + z];
   GeomFull G;
-  loadGeomFull(geomC, n, z, G);
+  loadGeomFull(geomC, ng, zg, G);
   Kinematics kk;
   Closure cc;
   kinematicsFast<L>(P, G.zg, a.prm, kk);
@@ -440,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void sourcesFastKernel(StepArgs a, Box box,
     for (int l = 0; l < 4; l++) {
       double gam[10];
 #pragma unroll
-      for (int q = 0; q < 10; q++) gam[q] = gamma40[(l * 10 + q) * n + z];
+      for (int q = 0; q < 10; q++) gam[q] = gamma40[(l * 10 + q) * ng + zg];
       if (curved) {
 #pragma unroll
         for (int nu = 0; nu < 4; nu++) {

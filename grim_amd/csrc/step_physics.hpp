@@ -28,7 +28,20 @@ struct Dims {
   int N[3], g[3], NT[3], dim;
   int64_t n;
   int64_t s[3];  // plane strides of X1, X2, X3
+  int64_t ng;    // values per geometry plane: one per (X1, X2) column
 };
+
+// Geometry planes (metric, connection, floors) hold one value per (X1, X2)
+// column: the metrics this build supports (modified Kerr-Schild, Minkowski)
+// do not depend on X3.  Zone (i, j, k) reads them at i + NT1 j, so a plane
+// holds D.ng = NT1 NT2 values (the zone count itself below three dimensions).
+__host__ __device__ __forceinline__ int64_t geomIndex(const Dims& D, int i, int j) {
+  return i + (int64_t)D.NT[0] * j;
+}
+// The same from a flat zone index (32-bit: grids hold fewer than 2^31 zones).
+__host__ __device__ __forceinline__ int64_t geomIndexOf(const Dims& D, int64_t z) {
+  return (int64_t)((uint32_t)z % (uint32_t)D.ng);
+}
 
 inline Dims makeDims(const grim_grid_config& c) {
   Dims D;
@@ -44,6 +57,7 @@ inline Dims makeDims(const grim_grid_config& c) {
   D.s[0] = 1;
   D.s[1] = D.NT[0];
   D.s[2] = (int64_t)D.NT[0] * D.NT[1];
+  D.ng = (int64_t)D.NT[0] * D.NT[1];
   return D;
 }
 

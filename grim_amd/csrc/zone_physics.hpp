@@ -244,6 +244,20 @@ struct ZoneConst {
   double normQ, normDP;  // residual row normalisation, residual.cpp:61-93
 };
 
+// Geometry plane size and the geometry entry of zone z of a batch: one per
+// zone, or one per column when geom_columns > 0 (grim_zone_batch).
+__device__ __forceinline__ int64_t batchGeomPlane(const grim_zone_batch& b) {
+  return b.geom_columns > 0 ? b.geom_columns : b.n;
+}
+__device__ __forceinline__ int64_t batchGeomIndex(const grim_zone_batch& b, int64_t z) {
+  return b.geom_columns > 0 ? (int64_t)((uint32_t)z % (uint32_t)b.geom_columns) : z;
+}
+
+// The zone's geometry from the batch's planes.
+__device__ __forceinline__ void loadBatchGeom(const grim_zone_batch& b, int64_t z, ZoneGeom& G) {
+  loadGeom(b.geom, batchGeomPlane(b), batchGeomIndex(b, z), G);
+}
+
 // RHS_IN_MEMORY false: the caller sets rhs and mask itself.
 template <class L, bool RHS_IN_MEMORY = true>
 __device__ __forceinline__ void loadZoneConst(const grim_zone_batch& b, const DevParams& prm,
@@ -256,7 +270,7 @@ __device__ __forceinline__ void loadZoneConst(const grim_zone_batch& b, const De
   }
   c.alphaC = b.closure ? b.closure[z] : 1.;
   c.alphaV = b.closure ? b.closure[n + z] : 1.;
-  c.radius = b.radius ? b.radius[z] : 0.;
+  c.radius = b.radius ? b.radius[batchGeomIndex(b, z)] : 0.;
   if constexpr (L::EMHD) {
     double P[L::DOF];
     Kinematics k;

@@ -9,12 +9,14 @@
 //
 // grim builds a geometry from a coordinates grid at one location
 // (geometry.cpp:4-59); the standalone constructor does the same through a
-// context of its own.  A timeStepper's geometries view its context's arrays.
-// The RIGHT and TOP locations (the upper faces of each zone) are the LEFT and
-// BOTTOM planes of the next zone along X1 / X2, copied shifted by one zone;
-// the zones whose upper face does not exist in the array (the last ghost
-// zone of each row, or of the last X2 row) hold a neighbour's face, which no
-// bulk stencil reads.
+// context of its own.  The library keeps one value per (X1, X2) column (the
+// metric does not depend on X3); a geometry holds every zone's value, as
+// grim's arrays do, in device planes of its own written from the context's
+// (grim_get_geometry).  The RIGHT and TOP locations (the upper faces of each
+// zone) are the LEFT and BOTTOM planes of the next zone along X1 / X2,
+// shifted by one zone; the zones whose upper face does not exist in the
+// array (the last ghost zone of each row, or of the last X2 row) hold a
+// neighbour's face, which no bulk stencil reads.
 #pragma once
 
 #include <memory>
@@ -25,7 +27,8 @@ namespace grim {
 
 class geometry {
   grim_ctx* ownCtx = nullptr;  // standalone geometry: its own context
-  double* shifted = nullptr;   // RIGHT / TOP: the face planes, one zone on
+  double* own = nullptr;       // this location's planes, every zone
+  double* conn = nullptr;      // CENTER: the connection's 64 planes, every zone
   int64_t n = 0;
 
   // upper-triangle packing of a symmetric 4x4 (00 01 02 03 11 12 13 22 23 33)
@@ -95,7 +98,8 @@ class geometry {
     attach(context, loc);
   }
   ~geometry() {
-    if (shifted) (void)hipFree(shifted);
+    if (own) (void)hipFree(own);
+    if (conn) (void)hipFree(conn);
     if (ownCtx) grim_destroy(ownCtx);
   }
   geometry(const geometry&) = delete;
@@ -127,23 +131,25 @@ class geometry {
     n = grim_grid_zones(&c);
     const int which = (loc == CENTER || loc == FRONT || loc == BACK) ? 0 : ((loc == LEFT || loc == RIGHT) ? 1 : 2);
     if (which == 2 && dim < 2) throw std::runtime_error("geometry: no X2 faces on a 1D grid");
-    double* base = nullptr;
-    check(grim_geometry(ctx, which, &base), "geometry: grim_geometry");
-    planes = base;
+    const size_t locBytes = sizeof(double) * GRIM_LOC_PLANES * (size_t)n;
+    checkHip(hipMalloc((void**)&own, locBytes), "geometry: hipMalloc");
+    check(grim_get_geometry(ctx, which, own, 1), "geometry: grim_get_geometry");
     if (loc == RIGHT || loc == TOP) {
       // face i+1/2 of zone i is face (i+1)-1/2 of zone i+1
       const int64_t s = loc == RIGHT ? 1 : (int64_t)(N1 + 2 * numGhost);
-      checkHip(hipMalloc((void**)&shifted, sizeof(double) * GRIM_LOC_PLANES * (size_t)n), "geometry: hipMalloc");
+      double* every = own;
+      checkHip(hipMalloc((void**)&own, locBytes), "geometry: hipMalloc");
       for (int q = 0; q < GRIM_LOC_PLANES; q++) {
-        const double* src = base + (size_t)q * n;
-        double* dst = shifted + (size_t)q * n;
+        const double* src = every + (size_t)q * n;
+        double* dst = own + (size_t)q * n;
         checkHip(hipMemcpy(dst, src + s, sizeof(double) * (size_t)(n - s), hipMemcpyDeviceToDevice),
                  "geometry: shift");
         checkHip(hipMemcpy(dst + (n - s), src + (n - s), sizeof(double) * (size_t)s, hipMemcpyDeviceToDevice),
                  "geometry: shift (last row)");
       }
-      planes = shifted;
+      (void)hipFree(every);
     }
+    planes = own;
     alpha = planes + (size_t)14 * n;
     g = planes + (size_t)15 * n;
     for (int mu = 0; mu < 4; mu++)
@@ -154,12 +160,14 @@ class geometry {
         else
           gCon[mu][nu] = planes + (size_t)(16 + packed3(mu - 1, nu - 1)) * n;
       }
-    double* gamma = nullptr;
-    if (which == 0) check(grim_geometry(ctx, 3, &gamma), "geometry: grim_geometry (connection)");
+    if (which == 0) {
+      checkHip(hipMalloc((void**)&conn, sizeof(double) * 64 * (size_t)n), "geometry: hipMalloc");
+      check(grim_get_geometry(ctx, 3, conn, 1), "geometry: grim_get_geometry (connection)");
+    }
     for (int l = 0; l < 4; l++)
       for (int k = 0; k < 4; k++)
         for (int m = 0; m < 4; m++)
-          gammaUpDownDown[l][k][m] = gamma ? gamma + (size_t)(l * 16 + k * 4 + m) * n : nullptr;
+          gammaUpDownDown[l][k][m] = conn ? conn + (size_t)(l * 16 + k * 4 + m) * n : nullptr;
   }
 };
 

@@ -104,6 +104,13 @@ typedef struct grim_zone_batch {
                                  same start needs no copy.  Read by
                                  grim_batched_newton(_noalloc) and the
                                  autotuner only; NULL = start from prim. */
+  int64_t geom_columns;       /* 0: geom and radius hold one value per zone
+                                 (planes of n).  > 0: they hold one value per
+                                 column (planes of geom_columns values), zone
+                                 z reading entry z % geom_columns: a grid
+                                 whose metric does not depend on X3 keeps one
+                                 value per (X1, X2) column, z % (NT1 NT2)
+                                 (grim_geometry_columns).  Must divide n. */
 } grim_zone_batch;
 
 typedef enum grim_jacobian_kind { GRIM_JAC_FD = 0, GRIM_JAC_ANALYTIC = 1 } grim_jacobian_kind;
@@ -335,12 +342,19 @@ int grim_destroy(grim_ctx* ctx);
 
 /* Device pointer of primOld (dof planes of grim_grid_zones doubles), and
  * of a geometry array: which = 0 centre, 1 X1 faces (LEFT), 2 X2 faces
- * (BOTTOM), 3 connection (64 planes). */
+ * (BOTTOM), 3 connection (64 planes).  The metrics supported (modified
+ * Kerr-Schild, Minkowski) do not depend on X3, so a geometry plane holds
+ * one value per (X1, X2) column: grim_geometry_columns(ctx) = NT1 NT2
+ * values, zone (i, j, k) at i + NT1 j (one per zone below three
+ * dimensions). */
 int grim_prim(grim_ctx* ctx, double** prim_old);
 int grim_geometry(grim_ctx* ctx, int which, double** geom);
+int64_t grim_geometry_columns(const grim_ctx* ctx);
 /* Copies of primOld and of a geometry array to / from host (on_device = 0)
  * or device (on_device = 1) memory; synchronise the stream (grim's
- * grid::setVars / getVars, src/grid/gridPy.pyx:185-230). */
+ * grid::setVars / getVars, src/grid/gridPy.pyx:185-230).  grim_get_geometry
+ * writes every zone's value (planes of grim_grid_zones doubles, each
+ * column's value repeated along X3), as grim's geometry arrays hold them. */
 int grim_set_prim(grim_ctx* ctx, const double* src, int on_device);
 int grim_get_prim(grim_ctx* ctx, double* dst, int on_device);
 int grim_get_geometry(grim_ctx* ctx, int which, double* dst, int on_device);
