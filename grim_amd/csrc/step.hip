@@ -1426,10 +1426,6 @@ struct grim_ctx {
   grim_grid_config c;
   grim_newton_opts opts;
   hipStream_t s;
-  // the stage's explicit sources run on `side`, beside the face fluxes on s
-  // (fork and join through the two events; default priority)
-  hipStream_t side;
-  hipEvent_t evFork, evJoin;
   Dims D;
   StepArgs a;
   CoordGrid cg;
@@ -1600,33 +1596,25 @@ int fluxesAndEmf(grim_ctx* x, const double* prim, bool allFaces) {
 
 template <class L>
 int explicitSources(grim_ctx* x, const double* prim, const double* primOld, double* sExp, double* sImp,
-                    double* consOld, bool onSide = false) {
+                    double* consOld) {
   const Box b = bulkBox(x->D);
   if constexpr (L::EMHD)
     uTKernel<L><<<blocksFor(x->D.n), kBlock, 0, x->s>>>(x->a, prim, x->geomC, x->uT);
-  hipStream_t ss = x->s;
-  if (onSide) {
-    // fork: the sources wait for everything enqueued on s so far (the stage
-    // state and u_mu, T); the caller joins them back before the fold
-    GRIM_HIP_TRY(hipEventRecord(x->evFork, x->s));
-    GRIM_HIP_TRY(hipStreamWaitEvent(x->side, x->evFork, 0));
-    ss = x->side;
-  }
   if (x->c.explicitFast)
     return launchSourcesFast(x->p, x->a, x->c.reconstruction, b, prim, primOld, x->geomC, x->gamma40, x->uT, sExp,
-                             sImp, consOld, ss);
+                             sImp, consOld, x->s);
   const unsigned nb = blocksFor(b.count());
   switch (x->c.reconstruction) {
     case GRIM_RECON_WENO5:
-      sourcesKernel<L, GRIM_RECON_WENO5><<<nb, kBlock, 0, ss>>>(x->a, b, prim, primOld, x->geomC, x->gamma,
+      sourcesKernel<L, GRIM_RECON_WENO5><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, primOld, x->geomC, x->gamma,
                                                                 x->uT, sExp, sImp, consOld);
       break;
     case GRIM_RECON_PPM:
-      sourcesKernel<L, GRIM_RECON_PPM><<<nb, kBlock, 0, ss>>>(x->a, b, prim, primOld, x->geomC, x->gamma,
+      sourcesKernel<L, GRIM_RECON_PPM><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, primOld, x->geomC, x->gamma,
                                                               x->uT, sExp, sImp, consOld);
       break;
     default:
-      sourcesKernel<L, GRIM_RECON_MINMOD><<<nb, kBlock, 0, ss>>>(x->a, b, prim, primOld, x->geomC, x->gamma,
+      sourcesKernel<L, GRIM_RECON_MINMOD><<<nb, kBlock, 0, x->s>>>(x->a, b, prim, primOld, x->geomC, x->gamma,
                                                                  x->uT, sExp, sImp, consOld);
   }
   return checkLaunch("sources");
@@ -1886,14 +1874,10 @@ int stagePrepare(grim_ctx* x, int st, bool guessInSolve = false, bool foldInSolv
     sideStateKernel<L><<<blocksFor(count), kBlock, 0, x->s>>>(D, x->sideState, primStage);
     if (int rc = checkLaunch("X1 side state")) return rc;
   }
-  // the sources (bandwidth-bound) on the side stream, overlapping the face
-  // fluxes (issue-bound) on s; neither reads what the other writes
   if (int rc = explicitSources<L>(x, primStage, x->primOld, x->sExp, L::EMHD ? x->sImp : nullptr,
-                                  st == 0 ? x->consOld : nullptr, true))
+                                  st == 0 ? x->consOld : nullptr))
     return rc;
   if (int rc = fluxesAndEmf<L>(x, primStage, false)) return rc;
-  GRIM_HIP_TRY(hipEventRecord(x->evJoin, x->side));
-  GRIM_HIP_TRY(hipStreamWaitEvent(x->s, x->evJoin, 0));
   if (guessInSolve) {
     const size_t fluid = (size_t)L::NF * D.n;
     GRIM_HIP_TRY(hipMemcpyAsync(out + fluid, primStage + fluid, bytes - sizeof(double) * fluid,
@@ -2105,7 +2089,9 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
   const int64_t n = x->D.n;
   const size_t dofB = sizeof(double) * (size_t)x->dof * n;
   const size_t nfB = sizeof(double) * (size_t)x->nf * n;
-  const size_t locB = sizeof(double) * (size_t)GRIM_LOC_PLANES * n;
+  // geometry planes: one value per (X1, X2) column (D.ng)
+  const int64_t ng = x->D.ng;
+  const size_t locB = sizeof(double) * (size_t)GRIM_LOC_PLANES * ng;
   const unsigned nb = blocksFor(n);
   struct Req {
     void** ptr;
@@ -2126,7 +2112,7 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
       {(void**)&x->geomC, locB},
       {(void**)&x->geomX1, locB},
       {(void**)&x->geomX2, locB},
-      {(void**)&x->gamma, sizeof(double) * 64 * (size_t)n},
+      {(void**)&x->gamma, sizeof(double) * 64 * (size_t)ng},
       {(void**)&x->red, sizeof(double) * ((size_t)nb + 1)},
       {(void**)&x->dtPart, sizeof(double) * ((size_t)nb + 1)},
       {(void**)&x->obs, sizeof(double) * ((size_t)kObsBlocks + 1) * OBS_WORDS},
@@ -2146,13 +2132,6 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
   }
   x->opts.workspace = x->newtonWs;
   x->opts.workspace_bytes = grim_newton_workspace_bytes(n);
-  if (hipStreamCreateWithFlags(&x->side, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&x->evFork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&x->evJoin, hipEventDisableTiming) != hipSuccess) {
-    setError("grim_create: side stream or event creation failed");
-    grim_destroy(x);
-    return GRIM_ERR_HIP;
-  }
   if (hipHostMalloc((void**)&x->hostBuf, 128) != hipSuccess) {
     setError("grim_create: hipHostMalloc failed");
     grim_destroy(x);
@@ -2171,27 +2150,13 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
       return GRIM_ERR_HIP;
     }
   }
-  // geometry planes hold one value per (X1, X2) column (D.ng): the rest of
-  // each allocation is never read, and holds NaN so that a read of it shows
-  const struct {
-    void* ptr;
-    size_t bytes;
-  } nans[] = {{x->geomC, locB}, {x->geomX1, locB}, {x->geomX2, locB}, {x->gamma, sizeof(double) * 64 * (size_t)n}};
-  for (auto& nr : nans) {
-    hipError_t e = hipMemsetAsync(nr.ptr, 0xff, nr.bytes, x->s);
-    if (e != hipSuccess) {
-      setError(std::string("grim_create: hipMemsetAsync: ") + hipGetErrorString(e));
-      grim_destroy(x);
-      return GRIM_ERR_HIP;
-    }
-  }
   const unsigned nbCols = blocksFor(x->D.ng);
   geometryKernel<<<nbCols, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, 0, x->geomC);
   geometryKernel<<<nbCols, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, 1, x->geomX1);
   geometryKernel<<<nbCols, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, 2, x->geomX2);
   connectionKernel<<<nbCols, kBlock, 0, x->s>>>(x->D, x->cm, x->cg, x->gamma);
   if (c->explicitFast) {
-    hipError_t e = hipMalloc((void**)&x->gamma40, sizeof(double) * 40 * (size_t)n);
+    hipError_t e = hipMalloc((void**)&x->gamma40, sizeof(double) * 40 * (size_t)ng);
     if (e != hipSuccess) {
       setError(std::string("grim_create: hipMalloc: ") + hipGetErrorString(e));
       grim_destroy(x);
@@ -2217,10 +2182,6 @@ int grim_create(const grim_params* p, const grim_grid_config* c, const grim_newt
 int grim_destroy(grim_ctx* x) {
   if (!x) return GRIM_OK;
   if (x->s) (void)hipStreamSynchronize(x->s);
-  if (x->side) (void)hipStreamSynchronize(x->side);
-  if (x->evFork) (void)hipEventDestroy(x->evFork);
-  if (x->evJoin) (void)hipEventDestroy(x->evJoin);
-  if (x->side) (void)hipStreamDestroy(x->side);
   void* ptrs[] = {x->primOld, x->primHalf, x->primNew, x->primDt, x->consOld, x->fluxes, x->emf,
                   x->sExp,    x->sImp,     x->uT,      x->rhs,    x->mask,    x->geomC,  x->geomX1,
                   x->geomX2,  x->gamma,    x->red,     x->dtPart, x->iters,  x->l2,      x->counters,
