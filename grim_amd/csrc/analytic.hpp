@@ -641,7 +641,7 @@ __device__ __forceinline__ void equilibratedSolve(T (&A)[N][N], T (&b)[N], const
       A[i][k] = A[i][k] * cs[k];
       m = fmax(m, fabs(A[i][k]));
     }
-    const T r = (m > T(0)) ? T(1) / m : T(1);
+    const T r = (m > T(0)) ? lowRecip(m) : T(1);
 #pragma unroll
     for (int k = 0; k < N; k++) A[i][k] = A[i][k] * r;
     b[i] = b[i] * r;

@@ -413,6 +413,39 @@ __global__ __launch_bounds__(kDefaultBlock) __attribute__((amdgpu_waves_per_eu(2
   solveZone<L, MODE>(prm, b, st, z, prm.maxIter, false, true);
 }
 
+// One-pass per-zone solve with the stragglers gathered inside the workgroup:
+// every zone first gets budget1 updates; the zones still unconverged are then
+// packed, by index, onto the workgroup's first lanes, which continue them for
+// the remaining updates.  A zone's iterations are a function of its own
+// primitives, so the results are the one-pass kernel's bit for bit, but the
+// wavefronts left without a straggler end instead of idling through the last
+// updates of a few lanes (about a quarter of the zones of a Kerr-Schild batch
+// need a third update).  The zones the continuing lanes take are the
+// workgroup's own, so their inputs come back from the caches.  Used for the
+// FD solve, whose updates cost six residual evaluations each.
+template <class L, int MODE>
+__global__ __launch_bounds__(kDefaultBlock) __attribute__((amdgpu_waves_per_eu(2))) void newtonPerZoneSplit(
+    DevParams prm, grim_zone_batch b, grim_newton_stats st, int budget1) {
+  __shared__ int list[kDefaultBlock];
+  __shared__ int waveCount[kDefaultBlock / 64];
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool more = false;
+  if (z < b.n) more = solveZone<L, MODE>(prm, b, st, z, budget1, false, false);
+  const unsigned long long ballot = __ballot(more);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) waveCount[wave] = __popcll(ballot);
+  __syncthreads();
+  int base = 0, total = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+    base += w < wave ? waveCount[w] : 0;
+    total += waveCount[w];
+  }
+  if (more) list[base + __popcll(ballot & ((1ull << lane) - 1ull))] = (int)z;
+  __syncthreads();
+  if ((int)threadIdx.x < total)
+    solveZone<L, MODE>(prm, b, st, (int64_t)list[threadIdx.x], prm.maxIter - budget1, true, true);
+}
+
 // The one-pass FD solve of a small batch, kWide lanes per zone (every lane
 // of a group shares z, so groups leave together).
 template <class L>
@@ -671,7 +704,13 @@ void launchSolve(const DevParams& prm, const grim_zone_batch& b, const grim_newt
         return;
       }
     }
-    newtonPerZone<L, MODE><<<grid, cfg.block, ldsBytes<L, MODE>(cfg.block), s>>>(prm, b, st);
+    // grim's FD solve (six residual evaluations per update) gains from
+    // gathering its stragglers in the workgroup; the cheaper updates of the
+    // other modes measured faster without the two barriers and the restart
+    if (MODE == MODE_FD && cfg.budget1 < prm.maxIter)
+      newtonPerZoneSplit<L, MODE><<<grid, cfg.block, ldsBytes<L, MODE>(cfg.block), s>>>(prm, b, st, cfg.budget1);
+    else
+      newtonPerZone<L, MODE><<<grid, cfg.block, ldsBytes<L, MODE>(cfg.block), s>>>(prm, b, st);
     return;
   }
   static_assert(MODE2 == MODE || (MODE == MODE_MIXED && MODE2 == MODE_ANALYTIC64F), "phase-2 mode");

@@ -412,6 +412,12 @@ __device__ __forceinline__ double sumSq(const double* R) {
 // bit-identical to the search-then-swap order (exact ties aside).  Every
 // access stays at a compile-time index: a swap with a computed row index
 // would force the matrix out of registers into scratch memory.
+// 1/x: FP64 correctly rounded; FP32 the hardware reciprocal (within one ulp;
+// the FP32 factorisation is an approximate solve refined by the FP64 Newton
+// updates around it).
+__device__ __forceinline__ double lowRecip(double x) { return 1. / x; }
+__device__ __forceinline__ float lowRecip(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <class Real, int N>
 __device__ __forceinline__ void luSolve(Real (&A)[N][N], Real (&b)[N]) {
   Real rinv[N];
@@ -430,7 +436,7 @@ __device__ __forceinline__ void luSolve(Real (&A)[N][N], Real (&b)[N]) {
       b[k] = swap ? lo : hi;
       b[i] = swap ? hi : lo;
     }
-    rinv[k] = Real(1.) / A[k][k];
+    rinv[k] = lowRecip(A[k][k]);
 #pragma unroll
     for (int i = k + 1; i < N; i++) {
       const Real m = A[i][k] * rinv[k];
