@@ -411,20 +411,23 @@ __device__ __forceinline__ double sumSq(const double* R) {
 // eliminated by the same operations wherever it sits, so the solution is
 // bit-identical to the search-then-swap order (exact ties aside).  Every
 // access stays at a compile-time index: a swap with a computed row index
-// would force the matrix out of registers into scratch memory.
+// would force the matrix out of registers into scratch memory.  PIVOT false
+// factorises in the given row order (the mixed path's equilibrated FP32
+// systems).
+
 // 1/x: FP64 correctly rounded; FP32 the hardware reciprocal (within one ulp;
 // the FP32 factorisation is an approximate solve refined by the FP64 Newton
 // updates around it).
 __device__ __forceinline__ double lowRecip(double x) { return 1. / x; }
 __device__ __forceinline__ float lowRecip(float x) { return __builtin_amdgcn_rcpf(x); }
 
-template <class Real, int N>
+template <class Real, int N, bool PIVOT = true>
 __device__ __forceinline__ void luSolve(Real (&A)[N][N], Real (&b)[N]) {
   Real rinv[N];
 #pragma unroll
   for (int k = 0; k < N; k++) {
 #pragma unroll
-    for (int i = k + 1; i < N; i++) {
+    for (int i = k + 1; i < N && PIVOT; i++) {
       const bool swap = fabs(A[i][k]) > fabs(A[k][k]);
 #pragma unroll
       for (int j = k; j < N; j++) {
