@@ -42,6 +42,10 @@ extern "C" int grim_autotune_newton(const grim_params* p, const grim_zone_batch*
     cands.push_back(o);
     for (int k1 : {1, 2, 3}) {
       if (k1 >= p->maxNonLinearIter) continue;
+      o.compaction = 2;
+      o.phase1_iters = k1;
+      o.phase2_group = 0;
+      cands.push_back(o);
       for (int g : {2, 4, 8}) {
         o.compaction = 1;
         o.phase1_iters = k1;

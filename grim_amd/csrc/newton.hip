@@ -345,7 +345,7 @@ bool foldFusable(const grim_params& p, const grim_newton_opts& o, int dim) {
   const bool escalate = o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP32_FP64;
   const bool mixed = o.jacobian == GRIM_JAC_ANALYTIC && o.lu_precision == GRIM_LU_FP32;
   const int budget1 = o.phase1_iters ? o.phase1_iters : 2;
-  return dim == 3 && o.semantics == GRIM_NEWTON_PER_ZONE && (escalate || (mixed && o.compaction)) &&
+  return dim == 3 && o.semantics == GRIM_NEWTON_PER_ZONE && (escalate || (mixed && o.compaction == 1)) &&
          budget1 < p.maxNonLinearIter;
 }
 
@@ -370,6 +370,7 @@ int batchedNewton(const grim_params* p, const grim_zone_batch* b, const grim_new
   GRIM_REQUIRE(mode >= 0, "grim_batched_newton: unsupported Jacobian / LU precision pair");
   GRIM_REQUIRE(mode == MODE_FD || o.semantics == GRIM_NEWTON_PER_ZONE,
                "grim_batched_newton: lock-step semantics run grim's exact FD-FP64 algorithm only");
+  GRIM_REQUIRE(o.compaction >= 0 && o.compaction <= 2, "grim_batched_newton: compaction must be 0, 1 or 2");
   GRIM_REQUIRE(o.semantics == GRIM_NEWTON_PER_ZONE || !o.compaction,
                "grim_batched_newton: compaction applies to per-zone semantics");
   GRIM_REQUIRE(o.block_size == 0 || o.block_size == 64 || o.block_size == 128 || o.block_size == 256,
@@ -391,7 +392,8 @@ int batchedNewton(const grim_params* p, const grim_zone_batch* b, const grim_new
 
   LaunchCfg cfg;
   cfg.block = o.block_size ? o.block_size : kDefaultBlock;
-  cfg.compaction = o.compaction != 0 || escalate;
+  cfg.compaction = o.compaction == 1 || escalate;
+  cfg.workgroupSplit = o.compaction == 2 && !escalate;
   cfg.escalate = escalate;
   cfg.budget1 = o.phase1_iters ? o.phase1_iters : 2;
   cfg.group = o.phase2_group ? o.phase2_group : 1;

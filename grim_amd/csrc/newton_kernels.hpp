@@ -421,8 +421,12 @@ __global__ __launch_bounds__(kDefaultBlock) __attribute__((amdgpu_waves_per_eu(2
 // wavefronts left without a straggler end instead of idling through the last
 // updates of a few lanes (about a quarter of the zones of a Kerr-Schild batch
 // need a third update).  The zones the continuing lanes take are the
-// workgroup's own, so their inputs come back from the caches.  Used for the
-// FD solve, whose updates cost six residual evaluations each.
+// workgroup's own, so their inputs come back from the caches.  It pays where
+// updates are costly and a tail of zones needs many (grim's FD solve of the
+// EMHD batch: 20.7 -> 15.0 ms); where the extra updates are few and cheap
+// the two barriers and the restart cost more than they save (the mixed
+// solve; the FD solve on the torus, 4.55 -> 5.05 ms per stage), so it is an
+// option (grim_newton_opts.compaction = 2) the autotuner weighs.
 template <class L, int MODE>
 __global__ __launch_bounds__(kDefaultBlock) __attribute__((amdgpu_waves_per_eu(2))) void newtonPerZoneSplit(
     DevParams prm, grim_zone_batch b, grim_newton_stats st, int budget1) {
@@ -675,6 +679,7 @@ inline CompactionWorkspace carveCompaction(void* base, int64_t n) {
 struct LaunchCfg {
   int block;                 // threads per workgroup: 64, 128 or 256
   bool compaction;           // two-phase compacting solve
+  bool workgroupSplit;       // one pass, stragglers gathered inside each workgroup (newtonPerZoneSplit)
   int budget1;               // phase-1 update budget
   int group;                 // phase-2 grab: chunks of 64 list entries per atomic (1..kMaxGroup)
   int persistentWaves;       // phase-2 grid, in wavefronts
@@ -704,10 +709,7 @@ void launchSolve(const DevParams& prm, const grim_zone_batch& b, const grim_newt
         return;
       }
     }
-    // grim's FD solve (six residual evaluations per update) gains from
-    // gathering its stragglers in the workgroup; the cheaper updates of the
-    // other modes measured faster without the two barriers and the restart
-    if (MODE == MODE_FD && cfg.budget1 < prm.maxIter)
+    if (cfg.workgroupSplit && cfg.budget1 < prm.maxIter)
       newtonPerZoneSplit<L, MODE><<<grid, cfg.block, ldsBytes<L, MODE>(cfg.block), s>>>(prm, b, st, cfg.budget1);
     else
       newtonPerZone<L, MODE><<<grid, cfg.block, ldsBytes<L, MODE>(cfg.block), s>>>(prm, b, st);

@@ -133,7 +133,11 @@ typedef struct grim_newton_opts {
   int compaction;    /* 1: every zone gets phase1_iters updates, then the
                         still unconverged zones are compacted into a dense
                         work list that a persistent kernel drains (at most
-                        2^30 zones) */
+                        2^30 zones); 2: one pass in which, after
+                        phase1_iters updates, each workgroup gathers its
+                        unconverged zones onto its first lanes (no work
+                        list).  Results equal the one-pass solve's bit for
+                        bit in every case. */
   int block_size;    /* threads per workgroup (0 = library default) */
   int fused;         /* FD path only: 0 = grim's operation order, no fused
                         multiply-add (bit-reproducible against a CPU FP64
