@@ -139,6 +139,9 @@ struct FoldArgs {
   const double* sImp;
   double* rhs;   // written for the zones left to the second phase
   double* mask;  // the residual mask plane
+  // the stage state's field rows (3 planes): zones outside the bulk take
+  // them into prim's field rows, so no copy of the rows precedes the solve
+  const double* fieldIn;
 };
 
 // Host: whether a grid stage's solve with these options compacts with the

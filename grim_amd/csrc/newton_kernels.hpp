@@ -510,6 +510,9 @@ __global__ __launch_bounds__(kDefaultBlock) __attribute__((amdgpu_waves_per_eu(2
       } else {
         mask = f.mask[z];
       }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) b.prim[(L::NF + c) * n + z] = f.fieldIn[c * n + z];
     }
     more = solveZone<L, MODE, false, true>(prm, b, st, z, budget1, false, false, rhs, B, mask);
     if (more)

@@ -1878,6 +1878,10 @@ int stagePrepare(grim_ctx* x, int st, bool guessInSolve = false, bool foldInSolv
                                   st == 0 ? x->consOld : nullptr))
     return rc;
   if (int rc = fluxesAndEmf<L>(x, primStage, false)) return rc;
+  // the solve starts from primStage's fluid rows (guessInSolve); the field
+  // rows come from the fold in the bulk and from primStage elsewhere, which
+  // a fold inside the solve copies itself
+  if (foldInSolve) return GRIM_OK;
   if (guessInSolve) {
     const size_t fluid = (size_t)L::NF * D.n;
     GRIM_HIP_TRY(hipMemcpyAsync(out + fluid, primStage + fluid, bytes - sizeof(double) * fluid,
@@ -1885,7 +1889,6 @@ int stagePrepare(grim_ctx* x, int st, bool guessInSolve = false, bool foldInSolv
   } else {
     GRIM_HIP_TRY(hipMemcpyAsync(out, primStage, bytes, hipMemcpyDeviceToDevice, x->s));
   }
-  if (foldInSolve) return GRIM_OK;
   const Box b = bulkBox(D);
   launchFold<L, true>(x->a, b, x->s, x->fluxes, x->emf, x->geomC, x->consOld, x->sExp, x->sImp,
                       stageDtPrime(x, st), nullptr, x->rhs, out, x->c.residualScaling ? x->mask : nullptr);
@@ -1937,6 +1940,7 @@ int stageSolve(grim_ctx* x, int st, double* prim, const double* guess = nullptr,
   f.sImp = x->sImp;
   f.rhs = x->rhs;
   f.mask = x->mask;
+  f.fieldIn = guess + (size_t)(x->dof - 3) * x->D.n;
   return batchedNewton(&x->p, &zb, &x->opts, &ns, x->s, &f);
 }
 
